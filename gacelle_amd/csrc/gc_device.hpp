@@ -1,0 +1,94 @@
+// gc_device.hpp -- device utilities for the gfx950 kernels: counter-based
+// RNG, uniform/normal transforms, wavefront reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GC_DEV __device__ __forceinline__
+
+namespace gc {
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11), from the paper's round
+// function: two 32x32->64 multiplies per round, Weyl key schedule.
+GC_DEV void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                          uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+    c1 = uint32_t(p1);
+    c3 = uint32_t(p0);
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+// Native stream: counter = {iteration, chain_lo, chain_hi, purpose}, key = seed.
+enum : uint32_t { PURPOSE_NORMAL0 = 0, PURPOSE_ACCEPT = 15, PURPOSE_ZZ = 16, PURPOSE_INIT = 32 };
+
+GC_DEV void draw4(uint64_t seed, uint32_t it, uint64_t chain, uint32_t purpose, uint32_t out[4]) {
+  philox4x32_10(it, uint32_t(chain), uint32_t(chain >> 32), purpose, uint32_t(seed), uint32_t(seed >> 32), out);
+}
+
+// uint32 -> (0,1): ((x >> 8) + 0.5) * 2^-24, exact in float.
+GC_DEV float u01(uint32_t x) { return (float(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+
+// Box-Muller pair from two uniforms.
+GC_DEV void box_muller(float u1, float u2, float& n0, float& n1) {
+  const float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincosf(2.0f * kPiF * u2, &s, &c);
+  n0 = r * c;
+  n1 = r * s;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront (64-lane) reductions.
+GC_DEV float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Reduce four values across the wavefront with a butterfly transpose
+// (7 lane exchanges instead of 24).  On return lane L holds the total of
+// value index ((L & 1) << 1) | ((L >> 1) & 1); lanes 0..3 carry indices
+// 0, 2, 1, 3.
+GC_DEV float wave_sum4(float v0, float v1, float v2, float v3, int lane) {
+  const bool b0 = lane & 1;
+  const float k0 = b0 ? v2 : v0, k1 = b0 ? v3 : v1;
+  const float s0 = b0 ? v0 : v2, s1 = b0 ? v1 : v3;
+  const float a0 = k0 + __shfl_xor(s0, 1, 64);
+  const float a1 = k1 + __shfl_xor(s1, 1, 64);
+  const bool b1 = lane & 2;
+  float k = b1 ? a1 : a0;
+  const float s = b1 ? a0 : a1;
+  k += __shfl_xor(s, 2, 64);
+  k += __shfl_xor(k, 4, 64);
+  k += __shfl_xor(k, 8, 64);
+  k += __shfl_xor(k, 16, 64);
+  k += __shfl_xor(k, 32, 64);
+  return k;
+}
+GC_DEV int wave_sum4_index(int lane) { return ((lane & 1) << 1) | ((lane >> 1) & 1); }
+
+GC_DEV double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+GC_DEV float sgnf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+
+}  // namespace gc

@@ -1,0 +1,848 @@
+// gc_kernels.hpp -- gfx950 kernels of the voxelwise-estimation path.
+//
+//  k_askadam  fused forward + adjoint + transform chain + loss + optimiser
+//             update, K iterations per launch with a voxel's data and
+//             optimiser state held in registers; per-iteration block
+//             partial sums (loss, step, |z|^2, |g|^2) for the host-side
+//             stopping logic (utils/askadam.m:261-299).
+//  k_eval     one evaluation of model_gradient per voxel (FWD, adjoint,
+//             residual outputs) for the operator pair and finalisation.
+//  k_mh       Metropolis-Hastings chains (utils/mcmc.m:82-190).
+//  k_gw       Goodman-Weare ensembles, simultaneous and red-black
+//             (utils/mcmc.m:192-383).
+//  k_reduce   deterministic FP64 reduction of block partials.
+#pragma once
+#include "gc_device.hpp"
+#include "gc_models.hpp"
+
+namespace gc {
+
+constexpr int kBlock = 256;
+constexpr int kRing = 16;  // iterations buffered in LDS before a partial flush
+
+// ---- loss functions of askadam.model_gradient (utils/askadam.m:86-95) -----
+template <int LOSS>
+GC_DEV void loss_fn(float r, float w, float& l, float& dl) {
+  if (LOSS == GC_LOSS_L1) {
+    l = w * fabsf(r);
+    dl = w * sgnf(r);
+  } else if (LOSS == GC_LOSS_L2) {
+    l = w * r * r;
+    dl = 2.f * w * r;
+  } else if (LOSS == GC_LOSS_HUBER) {
+    const float a = fabsf(r);
+    if (a <= 1.f) { l = 0.5f * w * r * r; dl = w * r; }
+    else { l = w * (a - 0.5f); dl = w * sgnf(r); }
+  } else {  // mse: half squared error, unweighted
+    l = 0.5f * r * r;
+    dl = r;
+  }
+}
+
+// ---- parameter transform (utils/askadam.m:1063-1073, forward_pass :424) ---
+// Branch-free on purpose: the selection is a select, never control flow
+// around values that are later combined (see the build notes on SLP).
+GC_DEV void transform_fwd(int transform, float z, float lb, float ub, float& th, float& dth) {
+  const float r = ub - lb;
+  const float sg = 1.f / (1.f + __expf(-z));
+  const float zc = fminf(fmaxf(z, 0.f), 1.f);
+  const bool sig = transform == GC_TRANSFORM_SIGMOID;
+  const bool lin = transform == GC_TRANSFORM_LINEAR;
+  const float th_sig = lb + r * sg, d_sig = r * sg * (1.f - sg);
+  const float th_lin = zc * r + lb, d_lin = (z >= 0.f && z <= 1.f) ? r : 0.f;
+  th = sig ? th_sig : (lin ? th_lin : z);
+  dth = sig ? d_sig : (lin ? d_lin : 1.f);
+}
+
+// ---- askadam block kernel -------------------------------------------------
+struct EpochScal {
+  float lr;  // adam: lr*sqrt(1-b2^t)/(1-b1^t) (effective form) or lr; sgdm/rmsprop: lr
+  float c1;  // 1/(1-b1^t) (Kingma-Ba form)
+  float c2;  // 1/(1-b2^t) (Kingma-Ba form)
+  float pad;
+};
+
+struct VolDev {
+  const float* y;
+  const float* w;
+  const float* ex;
+  const float* z_in;
+  const float* m_in;
+  const float* v_in;
+  float* z_out;
+  float* m_out;
+  float* v_out;
+  float* z_last;     // z (pre-update) at the launch's last iteration, or null
+  float* partials;   // [K][4][nblk]
+  const EpochScal* es;
+  int64_t nv;
+  int32_t blk0, nblk;
+  int32_t epoch0, n_epochs, iteration, pad;
+  float inv_count;   // 1/(M*nv)
+  float pad2[3];
+};
+
+struct AskParams {
+  const VolDev* vols;
+  int32_t n_vol;
+  int32_t opt;
+  int32_t transform;
+  int32_t adam_form;
+  float beta1, beta2, eps, momentum, rms_decay, rms_eps;
+};
+
+GC_DEV int find_volume(const VolDev* vols, int n_vol, int blk) {
+  int vi = 0;
+  for (int i = 1; i < n_vol; ++i)
+    if (blk >= vols[i].blk0) vi = i;
+  return vi;
+}
+
+template <class Model, int LOSS, bool HAS_W>
+__global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
+  const gc_model& md = *mdp;
+  constexpr int NP = Model::NP;
+  constexpr int MR = Model::MR;
+  const int vi = find_volume(ap.vols, ap.n_vol, blockIdx.x);
+  const VolDev V = ap.vols[vi];
+  if (V.n_epochs <= 0) return;
+  const int lblk = blockIdx.x - V.blk0;
+  const int64_t nv = V.nv;
+  const int64_t v = int64_t(lblk) * kBlock + threadIdx.x;
+  const bool valid = v < nv;
+  const int64_t vv = valid ? v : (nv - 1);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+  float yr[MR];
+  float wr[HAS_W ? MR : 1];
+#pragma unroll
+  for (int r = 0; r < MR; ++r) {
+    if (Model::valid(md, r)) {
+      const int64_t m = Model::meas(md, r);
+      yr[r] = V.y[m * nv + vv];
+      if (HAS_W) wr[HAS_W ? r : 0] = V.w[m * nv + vv];
+    } else {
+      yr[r] = 0.f;
+      if (HAS_W) wr[HAS_W ? r : 0] = 0.f;
+    }
+  }
+  float ex[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
+
+  float z[NP], m1[NP], m2[NP], lbp[NP], ubp[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    z[p] = V.z_in[int64_t(p) * nv + vv];
+    m1[p] = V.m_in[int64_t(p) * nv + vv];
+    m2[p] = V.v_in[int64_t(p) * nv + vv];
+    lbp[p] = md.lb[p];
+    ubp[p] = md.ub[p];
+  }
+  // slots the model never reads: constant, counted in |z|^2 only
+  float dead_z2 = 0.f;
+  for (int p = NP; p < md.n_par; ++p) {
+    const float zd = V.z_in[int64_t(p) * nv + vv];
+    dead_z2 += zd * zd;
+  }
+
+  __shared__ float red[kRing][kBlock / 64][4];
+  const float inv_count = V.inv_count;
+  for (int k = 0; k < V.n_epochs; ++k) {
+    const int e = V.epoch0 + k;
+    float th[NP], dth[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) transform_fwd(ap.transform, z[p], lbp[p], ubp[p], th[p], dth[p]);
+    float lsum = 0.f;
+    float gth[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+    Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
+      const bool fin = isfinite(s);  // utils.set_nan_inf_zero
+      const float sv = fin ? s : 0.f;
+      float l, dl;
+      loss_fn<LOSS>(sv - yr[r], HAS_W ? wr[HAS_W ? r : 0] : 1.f, l, dl);
+      lsum += l;
+      if (fin) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) gth[p] += dl * J[p];
+      }
+    });
+    float den = dead_z2, gn = 0.f, step = 0.f;
+    float g[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      g[p] = gth[p] * dth[p] * inv_count;
+      den += z[p] * z[p];
+      gn += g[p] * g[p];
+    }
+    if (k == V.n_epochs - 1 && V.z_last != nullptr && valid) {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
+    }
+    if (e < V.iteration) {  // update_parameters skips the last iteration (:677)
+      const EpochScal es = V.es[k];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        float dz;
+        if (ap.opt == GC_OPT_ADAM) {
+          m1[p] = ap.beta1 * m1[p] + (1.f - ap.beta1) * g[p];
+          m2[p] = ap.beta2 * m2[p] + (1.f - ap.beta2) * g[p] * g[p];
+          if (ap.adam_form == GC_ADAM_KINGMA_BA)
+            dz = -es.lr * (m1[p] * es.c1) / (sqrtf(m2[p] * es.c2) + ap.eps);
+          else
+            dz = -es.lr * m1[p] / (sqrtf(m2[p]) + ap.eps);
+        } else if (ap.opt == GC_OPT_SGDM) {
+          m1[p] = ap.momentum * m1[p] - es.lr * g[p];
+          dz = m1[p];
+        } else {
+          m2[p] = ap.rms_decay * m2[p] + (1.f - ap.rms_decay) * g[p] * g[p];
+          dz = -es.lr * g[p] / (sqrtf(m2[p]) + ap.rms_eps);
+        }
+        const float zn = z[p] + dz;
+        const float d = zn - z[p];
+        step += d * d;
+        z[p] = zn;
+      }
+    }
+    if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
+    const float tot = wave_sum4(lsum, step, den, gn, lane);
+    const int slot = k & (kRing - 1);
+    if (lane < 4) red[slot][wave][wave_sum4_index(lane)] = tot;
+    if (slot == kRing - 1 || k == V.n_epochs - 1) {
+      __syncthreads();
+      const int cnt = slot + 1;
+      const int kbase = k - slot;
+      if (threadIdx.x < cnt * 4) {
+        const int kk = threadIdx.x >> 2, j = threadIdx.x & 3;
+        float s = 0.f;
+#pragma unroll
+        for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
+        V.partials[(int64_t(kbase + kk) * 4 + j) * V.nblk + lblk] = s;
+      }
+      __syncthreads();
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      V.z_out[int64_t(p) * nv + v] = z[p];
+      V.m_out[int64_t(p) * nv + v] = m1[p];
+      V.v_out[int64_t(p) * nv + v] = m2[p];
+    }
+    for (int p = NP; p < md.n_par; ++p) {
+      const float zd = V.z_in[int64_t(p) * nv + v];
+      V.z_out[int64_t(p) * nv + v] = zd;
+      V.m_out[int64_t(p) * nv + v] = 0.f;
+      V.v_out[int64_t(p) * nv + v] = 0.f;
+      if (V.z_last != nullptr) V.z_last[int64_t(p) * nv + v] = zd;
+    }
+  }
+}
+
+// ---- deterministic FP64 reduction: out[j] = sum_b in[j*nb + b] ------------
+__global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb,
+                                                         double* __restrict__ out) {
+  const int64_t row = blockIdx.x;
+  double s = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kBlock) s += double(in[row * nb + b]);
+  s = wave_sum_d(s);
+  __shared__ double sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+    out[row] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_rows_d(const double* __restrict__ in, int64_t nb,
+                                                           double* __restrict__ out) {
+  const int64_t row = blockIdx.x;
+  double s = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kBlock) s += in[row * nb + b];
+  s = wave_sum_d(s);
+  __shared__ double sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+    out[row] = t;
+  }
+}
+
+// ---- single evaluation (operator pair / finalisation) ---------------------
+struct EvalParams {
+  const float* x;       // [P][nv] theta or z
+  const float* y;       // [M][nv] (adjoint / residual modes)
+  const float* w;
+  const float* ex;
+  float* s_out;         // [M][nv] model signal (forward mode)
+  float* grad;          // [P][nv] d(scale*sum loss)/dx
+  float* residual;      // [M][nv] elementwise loss
+  float* resloss;       // [nv] mean elementwise loss per voxel
+  double* partial;      // [nblk] block loss sums
+  int64_t nv;
+  int32_t transform;
+  int32_t mode;         // 0 forward, 1 loss (+ optional outputs)
+  float scale;
+};
+
+template <class Model, int LOSS, bool HAS_W>
+__global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ mdp, const EvalParams ep) {
+  const gc_model& md = *mdp;
+  constexpr int NP = Model::NP;
+  constexpr int MR = Model::MR;
+  const int64_t nv = ep.nv;
+  const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const bool valid = v < nv;
+  const int64_t vv = valid ? v : (nv - 1);
+  float ex[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? ep.ex[int64_t(e) * nv + vv] : 0.f;
+  float th[NP], dth[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+    transform_fwd(ep.transform, ep.x[int64_t(p) * nv + vv], md.lb[p], md.ub[p], th[p], dth[p]);
+  double lsum = 0.0;
+  if (ep.mode == 0) {
+    Model::template eval<false>(md, th, ex, [&](int r, float s, const float*) {
+      if (valid) ep.s_out[int64_t(Model::meas(md, r)) * nv + v] = isfinite(s) ? s : 0.f;
+    });
+  } else {
+    float gth[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+    float lvox = 0.f;
+    Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
+      const int64_t m = Model::meas(md, r);
+      const bool fin = isfinite(s);
+      const float sv = fin ? s : 0.f;
+      const float wv = HAS_W ? ep.w[m * nv + vv] : 1.f;
+      float l, dl;
+      loss_fn<LOSS>(sv - ep.y[m * nv + vv], wv, l, dl);
+      lvox += l;
+      if (ep.residual != nullptr && valid) ep.residual[m * nv + v] = l;
+      if (fin) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) gth[p] += dl * J[p];
+      }
+    });
+    lsum = valid ? double(lvox) : 0.0;
+    if (valid && ep.resloss != nullptr) ep.resloss[v] = lvox / float(md.n_meas);
+    if (valid && ep.grad != nullptr) {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) ep.grad[int64_t(p) * nv + v] = gth[p] * dth[p] * ep.scale;
+      for (int p = NP; p < md.n_par; ++p) ep.grad[int64_t(p) * nv + v] = 0.f;
+    }
+    if (ep.partial != nullptr) {
+      lsum = wave_sum_d(lsum);
+      __shared__ double sh[kBlock / 64];
+      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = lsum;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+        ep.partial[blockIdx.x] = t;
+      }
+    }
+  }
+}
+
+// ---- MCMC -------------------------------------------------------------------
+struct McmcParams {
+  const float* y;
+  const float* w;
+  const float* ex;
+  const float* x0;
+  const float* normals;
+  const float* uniforms;
+  const int32_t* partners;
+  float* posterior;
+  float* mean;
+  float* stdv;
+  float* accept;
+  float* xfinal;
+  uint8_t* trace;
+  int64_t nv;
+  uint64_t seed;
+  int32_t iteration, thinning, n_burnin, rep, n_walker, n_sample, rng, ens, q, noise;
+  float step_size, start_range;
+  float x_step[GC_MAX_PAR];
+};
+
+GC_DEV bool keep_sample(const McmcParams& p, int k) {
+  return k > p.n_burnin && ((k - p.n_burnin + 1) % p.thinning) == 0;  // utils/mcmc.m:170
+}
+
+// MATLAB min(x,1) ignoring NaN (utils/mcmc.m:160)
+GC_DEV float min1(float x) { return isnan(x) ? 1.f : fminf(x, 1.f); }
+
+// logP_Gaussian, utils/logP_Gaussian.m:18-22
+GC_DEV float logp_gaussian(float sos, float noise, int nm) {
+  const float n2 = noise * noise;
+  return -sos / (2.f * n2) - 0.5f * float(nm) * logf(2.f * kPiF * n2);
+}
+
+template <class Model, bool HAS_W>
+GC_DEV float model_sos(const gc_model& md, const float* th, const float* ex, const float* yr, const float* wr) {
+  float acc = 0.f;
+  Model::template eval<false>(md, th, ex, [&](int r, float s, const float*) {
+    const float d = s - yr[r];
+    acc += (HAS_W ? wr[HAS_W ? r : 0] : 1.f) * d * d;
+  });
+  return acc;
+}
+
+// Group reduction of per-chain (sum, sumsq) into posterior mean / std for
+// the voxel owning `gsz` consecutive threads of the block.
+template <int Q>
+GC_DEV void group_stats(const McmcParams& P, const double* s1, const double* s2, int gsz, int64_t v, bool active,
+                        const float* xref, double n_total, double* sh) {
+  // sh: [Q][2][kBlock]
+  for (int q = 0; q < Q; ++q) {
+    sh[(q * 2 + 0) * kBlock + threadIdx.x] = active ? s1[q] : 0.0;
+    sh[(q * 2 + 1) * kBlock + threadIdx.x] = active ? s2[q] : 0.0;
+  }
+  __syncthreads();
+  const int g0 = (threadIdx.x / gsz) * gsz;
+  if (active && threadIdx.x == g0) {
+    for (int q = 0; q < Q; ++q) {
+      double a = 0.0, b = 0.0;
+      for (int i = 0; i < gsz; ++i) {
+        a += sh[(q * 2 + 0) * kBlock + g0 + i];
+        b += sh[(q * 2 + 1) * kBlock + g0 + i];
+      }
+      if (P.mean) P.mean[int64_t(q) * P.nv + v] = float(double(xref[q]) + a / n_total);
+      if (P.stdv) {
+        const double var = (n_total > 1) ? (b - a * a / n_total) / (n_total - 1) : 0.0;
+        P.stdv[int64_t(q) * P.nv + v] = float(sqrt(fmax(var, 0.0)));
+      }
+    }
+  }
+}
+
+// Metropolis-Hastings, utils/mcmc.m:82-190.  Thread layout: voxel-major,
+// the `rep` chains (repetitions) of a voxel in consecutive lanes.
+template <class Model, bool HAS_W>
+__global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp, const McmcParams P) {
+  const gc_model& md = *mdp;
+  constexpr int NP = Model::NP;
+  constexpr int Q = NP + 1;
+  constexpr int MR = Model::MR;
+  const int R = P.rep;
+  const int gpb = kBlock / R;  // voxels per block
+  const int tl = threadIdx.x;
+  const int gi = tl / R, r = tl % R;
+  const int64_t v = int64_t(blockIdx.x) * gpb + gi;
+  const bool active = (gi < gpb) && (v < P.nv);
+  const int64_t nv = P.nv;
+  const int64_t vv = active ? v : 0;
+
+  float yr[MR];
+  float wr[HAS_W ? MR : 1];
+#pragma unroll
+  for (int rr = 0; rr < MR; ++rr) {
+    if (Model::valid(md, rr)) {
+      const int64_t m = Model::meas(md, rr);
+      yr[rr] = P.y[m * nv + vv];
+      if (HAS_W) wr[HAS_W ? rr : 0] = P.w[m * nv + vv];
+    } else {
+      yr[rr] = 0.f;
+      if (HAS_W) wr[HAS_W ? rr : 0] = 0.f;
+    }
+  }
+  float ex[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * nv + vv] : 0.f;
+  float x0[Q], x[Q], lbq[Q], ubq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    lbq[q] = md.lb[q];
+    ubq[q] = md.ub[q];
+    x0[q] = fminf(fmaxf(P.x0[int64_t(q) * nv + vv], lbq[q]), ubq[q]);  // :126-128
+    x[q] = x0[q];
+  }
+  const int M = md.n_meas;
+  const float lp0 = logp_gaussian(model_sos<Model, HAS_W>(md, x0, ex, yr, wr), x0[Q - 1], M);  // :129
+  float lp = lp0;
+  double s1[Q], s2[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) { s1[q] = 0.0; s2[q] = 0.0; }
+  int counter = 0;
+  int nacc = 0;
+  const uint64_t chain = uint64_t(r) * uint64_t(nv) + uint64_t(vv);
+  for (int k = 1; k <= P.iteration; ++k) {
+    const int64_t step = int64_t(r) * P.iteration + (k - 1);
+    float n[Q];
+    if (P.rng == GC_RNG_REPLAY) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) n[q] = P.normals[(step * nv + vv) * Q + q];
+    } else {
+#pragma unroll
+      for (int g4 = 0; g4 < (Q + 3) / 4; ++g4) {
+        uint32_t o[4];
+        draw4(P.seed, uint32_t(k), chain, PURPOSE_NORMAL0 + g4, o);
+        float a0, a1, a2, a3;
+        box_muller(u01(o[0]), u01(o[1]), a0, a1);
+        if (4 * g4 + 2 < Q) box_muller(u01(o[2]), u01(o[3]), a2, a3);
+        if (4 * g4 + 0 < Q) n[4 * g4 + 0] = a0;
+        if (4 * g4 + 1 < Q) n[4 * g4 + 1] = a1;
+        if (4 * g4 + 2 < Q) n[4 * g4 + 2] = a2;
+        if (4 * g4 + 3 < Q) n[4 * g4 + 3] = a3;
+      }
+    }
+    float xp[Q];
+    bool oob = false;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      xp[q] = x[q] + P.x_step[q] * n[q];                   // :147
+      oob = oob || (xp[q] < lbq[q]) || (xp[q] > ubq[q]);  // :149
+      xp[q] = fminf(fmaxf(xp[q], lbq[q]), ubq[q]);         // :151
+    }
+    const float lpp = logp_gaussian(model_sos<Model, HAS_W>(md, xp, ex, yr, wr), xp[Q - 1], M);  // :158
+    float u;
+    if (P.rng == GC_RNG_REPLAY) {
+      u = P.uniforms[step * nv + vv];
+    } else {
+      uint32_t o[4];
+      draw4(P.seed, uint32_t(k), chain, PURPOSE_ACCEPT, o);
+      u = u01(o[0]);
+    }
+    const float ratio = min1(__expf(lpp - lp));  // :160
+    const bool acc = (ratio > u) && !oob;        // :161-162
+    if (P.trace != nullptr && active) P.trace[step * nv + v] = acc ? 1 : 0;
+    if (acc) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) x[q] = xp[q];
+      lp = lpp;
+      ++nacc;
+    }
+    if (keep_sample(P, k)) {  // :170-173
+      if (counter < P.n_sample) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (P.posterior != nullptr && active)
+            P.posterior[((int64_t(q) * R + r) * P.n_sample + counter) * nv + v] = x[q];
+          const double d = double(x[q]) - double(x0[q]);
+          s1[q] += d;
+          s2[q] += d * d;
+        }
+      }
+      ++counter;
+    }
+  }
+  const int filled = min(counter, P.n_sample);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    // unfilled posterior slots are zeros in the reference buffer (:121)
+    const double d = -double(x0[q]);
+    s1[q] += d * double(P.n_sample - filled);
+    s2[q] += d * d * double(P.n_sample - filled);
+    if (P.xfinal != nullptr && active) P.xfinal[(int64_t(r) * Q + q) * nv + v] = x[q];
+  }
+  __shared__ double sh[2 * Q * kBlock];
+  __shared__ int shacc[kBlock];
+  shacc[tl] = active ? nacc : 0;
+  group_stats<Q>(P, s1, s2, R, v, active && gi < gpb, x0, double(P.n_sample) * R, sh);
+  if (active && r == 0 && P.accept != nullptr) {
+    int t = 0;
+    for (int i = 0; i < R; ++i) t += shacc[tl + i];
+    P.accept[v] = float(double(t) / (double(P.iteration) * R));
+  }
+}
+
+// Goodman-Weare affine-invariant ensemble, utils/mcmc.m:192-383.  A group of
+// rep*W consecutive threads owns one voxel: walker w of repetition r at
+// group lane r*W + w.  Walker positions are exchanged through LDS.
+template <class Model, bool HAS_W>
+__global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp, const McmcParams P) {
+  const gc_model& md = *mdp;
+  constexpr int NP = Model::NP;
+  constexpr int Q = NP + 1;
+  constexpr int MR = Model::MR;
+  const int W = P.n_walker, R = P.rep;
+  const int gsz = W * R;
+  const int gpb = kBlock / gsz;
+  const int tl = threadIdx.x;
+  const int gi = tl / gsz, gl = tl % gsz;
+  const int r = gl / W, wk = gl % W;
+  const int64_t v = int64_t(blockIdx.x) * gpb + gi;
+  const bool active = (gi < gpb) && (v < P.nv);
+  const int64_t nv = P.nv;
+  const int64_t vv = active ? v : 0;
+  const bool rb = P.ens == GC_ENSEMBLE_REDBLACK;
+  const int h = W / 2;
+
+  float yr[MR];
+  float wr[HAS_W ? MR : 1];
+#pragma unroll
+  for (int rr = 0; rr < MR; ++rr) {
+    if (Model::valid(md, rr)) {
+      const int64_t m = Model::meas(md, rr);
+      yr[rr] = P.y[m * nv + vv];
+      if (HAS_W) wr[HAS_W ? rr : 0] = P.w[m * nv + vv];
+    } else {
+      yr[rr] = 0.f;
+      if (HAS_W) wr[HAS_W ? rr : 0] = 0.f;
+    }
+  }
+  float ex[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * nv + vv] : 0.f;
+  float xc0[Q], x[Q], lbq[Q], ubq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    lbq[q] = md.lb[q];
+    ubq[q] = md.ub[q];
+    xc0[q] = P.x0[int64_t(q) * nv + vv];
+  }
+  // walker initialisation x0 + (ub-lb)*startRange*randn, clamped (:270-273)
+  {
+    const uint64_t ichain = uint64_t(wk) * uint64_t(nv) + uint64_t(vv);
+    float n[Q];
+    if (P.rng == GC_RNG_REPLAY) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) n[q] = P.normals[(int64_t(wk) * nv + vv) * Q + q];
+    } else {
+#pragma unroll
+      for (int g4 = 0; g4 < (Q + 3) / 4; ++g4) {
+        uint32_t o[4];
+        draw4(P.seed, 0u, ichain, PURPOSE_INIT + g4, o);
+        float a0, a1, a2, a3;
+        box_muller(u01(o[0]), u01(o[1]), a0, a1);
+        if (4 * g4 + 2 < Q) box_muller(u01(o[2]), u01(o[3]), a2, a3);
+        if (4 * g4 + 0 < Q) n[4 * g4 + 0] = a0;
+        if (4 * g4 + 1 < Q) n[4 * g4 + 1] = a1;
+        if (4 * g4 + 2 < Q) n[4 * g4 + 2] = a2;
+        if (4 * g4 + 3 < Q) n[4 * g4 + 3] = a3;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      x[q] = fminf(fmaxf(xc0[q] + (ubq[q] - lbq[q]) * P.start_range * n[q], lbq[q]), ubq[q]);
+  }
+  const int M = md.n_meas;
+  float lp = logp_gaussian(model_sos<Model, HAS_W>(md, x, ex, yr, wr), x[Q - 1], M);  // :275
+
+  __shared__ float shx[Q][kBlock];
+  double s1[Q], s2[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) { s1[q] = 0.0; s2[q] = 0.0; }
+  int counter = 0, nacc = 0;
+  const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(vv);
+  const int base = gi * gsz + r * W;  // LDS index of walker 0 of this (voxel, rep)
+  const int hm = (h > W - h) ? h : (W - h);
+  for (int k = 1; k <= P.iteration; ++k) {
+    const int64_t step = int64_t(r) * P.iteration + (k - 1);
+    const int32_t* part = P.partners + step * W;
+    const int nsub = rb ? 2 : 1;
+    for (int sub = 0; sub < nsub; ++sub) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) shx[q][tl] = x[q];
+      __syncthreads();
+      bool mine = true;
+      int j = wk, pw;
+      if (rb) {
+        mine = (sub == 0) ? (wk < h) : (wk >= h);
+        j = (sub == 0) ? wk : wk - h;
+        pw = mine ? part[((sub == 0) ? 0 : h) + j] : 0;
+      } else {
+        pw = part[wk];
+      }
+      float u1 = 0.f, u2 = 0.f;
+      if (mine) {
+        if (P.rng == GC_RNG_REPLAY) {
+          int64_t ub0, ust;
+          if (rb) { ub0 = (step * 2 + sub) * 2 * hm; ust = hm; }
+          else { ub0 = step * 2 * W; ust = W; }
+          u1 = P.uniforms[(ub0 + j) * nv + vv];
+          u2 = P.uniforms[(ub0 + ust + j) * nv + vv];
+        } else {
+          uint32_t o[4];
+          draw4(P.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o);
+          u1 = u01(o[0]);
+          draw4(P.seed, uint32_t(k), chain, PURPOSE_ACCEPT + 0x100u * sub, o);
+          u2 = u01(o[0]);
+        }
+      }
+      // stretch move (:299-304)
+      const float t = (P.step_size - 1.f) * u1 + 1.f;
+      const float zz = t * t / P.step_size;
+      float xp[Q];
+      bool oob = false;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float xa = shx[q][base + pw];
+        float val = xa + (x[q] - xa) * zz;
+        oob = oob || (val < lbq[q]) || (val > ubq[q]);
+        xp[q] = fminf(fmaxf(val, lbq[q]), ubq[q]);
+      }
+      __syncthreads();
+      if (mine) {
+        const float lpp = logp_gaussian(model_sos<Model, HAS_W>(md, xp, ex, yr, wr), xp[Q - 1], M);
+        // min(zz.^(Nvar-1).*exp(logPProposed-logPCurr),1) > rand (:313-315)
+        float zpow = 1.f;
+#pragma unroll
+        for (int q = 0; q < Q - 1; ++q) zpow *= zz;
+        const float ratio = min1(zpow * __expf(lpp - lp));
+        const bool acc = (ratio > u2) && !oob;
+        if (P.trace != nullptr && active) P.trace[(step * W + wk) * nv + v] = acc ? 1 : 0;
+        if (acc) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) x[q] = xp[q];
+          lp = lpp;
+          ++nacc;
+        }
+      }
+    }
+    if (keep_sample(P, k)) {  // :363-366
+      if (counter < P.n_sample) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (P.posterior != nullptr && active)
+            P.posterior[(((int64_t(q) * R + r) * P.n_sample + counter) * W + wk) * nv + v] = x[q];
+          const double d = double(x[q]) - double(xc0[q]);
+          s1[q] += d;
+          s2[q] += d * d;
+        }
+      }
+      ++counter;
+    }
+  }
+  const int filled = min(counter, P.n_sample);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double d = -double(xc0[q]);
+    s1[q] += d * double(P.n_sample - filled);
+    s2[q] += d * d * double(P.n_sample - filled);
+    if (P.xfinal != nullptr && active) P.xfinal[((int64_t(r) * W + wk) * Q + q) * nv + v] = x[q];
+  }
+  __shared__ double sh[2 * Q * kBlock];
+  __shared__ int shacc[kBlock];
+  shacc[tl] = active ? nacc : 0;
+  group_stats<Q>(P, s1, s2, gsz, v, active, xc0, double(P.n_sample) * R * W, sh);
+  if (active && gl == 0 && P.accept != nullptr) {
+    int t = 0;
+    for (int i = 0; i < gsz; ++i) t += shacc[tl + i];
+    P.accept[v] = float(double(t) / (double(P.iteration) * R * W));
+  }
+}
+
+// ---- layout / transform kernels ---------------------------------------------
+__global__ void k_nd_to_gd(const float* __restrict__ nd, int64_t nvox, int32_t M, const int64_t* __restrict__ vox,
+                           int64_t nv, float* __restrict__ gd) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nv * M) return;
+  const int64_t m = i / nv, v = i % nv;
+  gd[i] = nd[vox[v] + nvox * m];
+}
+
+__global__ void k_clamp01(float* __restrict__ p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = fminf(fmaxf(p[i], 0.f), 1.f);
+}
+
+__global__ void k_fill(float* __restrict__ p, int64_t n, float val) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = val;
+}
+
+__global__ void k_gd_to_nd(const float* __restrict__ gd, int64_t nv, int32_t M, const int64_t* __restrict__ vox,
+                           int64_t nvox, float* __restrict__ nd) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nv * M) return;
+  const int64_t m = i / nv, v = i % nv;
+  nd[vox[v] + nvox * m] = gd[i];
+}
+
+// theta -> z (askadam.initialise_parameter / rescale_parameters, :368-412, :1077-1089)
+__global__ void k_rescale(const gc_model* __restrict__ mdp, int32_t transform, const float* __restrict__ th,
+                          float* __restrict__ z, int64_t nv) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const gc_model& md = *mdp;
+  if (i >= nv * md.n_par) return;
+  const int p = int(i / nv);
+  const float lb = md.lb[p], ub = md.ub[p];
+  const float x = th[i];
+  if (transform == GC_TRANSFORM_SIGMOID) {
+    const float eb = 1e-4f * (ub - lb);
+    const float c = fmaxf(fminf(x, ub - eb), lb + eb);
+    const float t = (c - lb) / (ub - lb);
+    z[i] = logf(t / (1.f - t));
+  } else if (transform == GC_TRANSFORM_LINEAR) {
+    z[i] = (x - lb) / (ub - lb);
+  } else {
+    z[i] = x;
+  }
+}
+
+// z -> theta (unscale_parameters, :1063-1073)
+__global__ void k_unscale(const gc_model* __restrict__ mdp, int32_t transform, const float* __restrict__ z,
+                          float* __restrict__ th, int64_t nv) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const gc_model& md = *mdp;
+  if (i >= nv * md.n_par) return;
+  const int p = int(i / nv);
+  const float lb = md.lb[p], ub = md.ub[p];
+  if (transform == GC_TRANSFORM_SIGMOID) th[i] = lb + (ub - lb) * (1.f / (1.f + expf(-z[i])));
+  else if (transform == GC_TRANSFORM_LINEAR) th[i] = z[i] * (ub - lb) + lb;
+  else th[i] = z[i];
+}
+
+// ---- spatial total variation (utils/spatial_total_variation.m:24-66) --------
+// One thread per grid voxel; masked voxels add their forward-difference
+// terms and scatter-free gather the adjoint: the gradient at voxel i is
+// sum_d (1/v_d)[m(i-e_d) sgn(D_d(i-e_d)) - m(i) sgn(D_d(i))] * lambda/N.
+__global__ __launch_bounds__(kBlock) void k_tv(const float* __restrict__ img, const uint8_t* __restrict__ mask,
+                                                int32_t X, int32_t Y, int32_t Z, int32_t nd, float ivx, float ivy,
+                                                float ivz, float scale, double* __restrict__ partial,
+                                                float* __restrict__ grad) {
+  const int64_t n = int64_t(X) * Y * Z;
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  double acc = 0.0;
+  if (i < n) {
+    const int x = int(i % X);
+    const int yy = int((i / X) % Y);
+    const int zz = int(i / (int64_t(X) * Y));
+    const float c = img[i];
+    const float iv[3] = {ivx, ivy, ivz};
+    float g = 0.f;
+    for (int d = 0; d < nd; ++d) {
+      int64_t jp, jm;
+      if (d == 0) {
+        jp = ((x + 1) % X) + int64_t(X) * (yy + int64_t(Y) * zz);
+        jm = ((x + X - 1) % X) + int64_t(X) * (yy + int64_t(Y) * zz);
+      } else if (d == 1) {
+        jp = x + int64_t(X) * (((yy + 1) % Y) + int64_t(Y) * zz);
+        jm = x + int64_t(X) * (((yy + Y - 1) % Y) + int64_t(Y) * zz);
+      } else {
+        jp = x + int64_t(X) * (yy + int64_t(Y) * ((zz + 1) % Z));
+        jm = x + int64_t(X) * (yy + int64_t(Y) * ((zz + Z - 1) % Z));
+      }
+      const float dp = (img[jp] - c) * iv[d];
+      if (mask[i]) {
+        acc += fabs(double(dp));
+        g -= sgnf(dp) * iv[d];
+      }
+      if (mask[jm]) {
+        const float dm = (c - img[jm]) * iv[d];
+        g += sgnf(dm) * iv[d];
+      }
+    }
+    if (grad != nullptr) grad[i] = g * scale;
+  }
+  acc = wave_sum_d(acc);
+  __shared__ double sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) t += sh[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+}  // namespace gc

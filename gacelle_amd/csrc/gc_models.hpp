@@ -1,0 +1,265 @@
+// gc_models.hpp -- device forward models with hand-written Jacobians.
+//
+// Every model exposes the same interface to the solver kernels:
+//   NP                     parameters the model reads (slots 0..NP-1)
+//   MR                     register slots for a voxel's measurements
+//   valid(md, r), meas(md, r)
+//                          whether register slot r is used and which G-D
+//                          measurement row it holds
+//   eval<JAC>(md, th, ex, f)
+//                          calls f(r, s, J) for every used slot r, with the
+//                          model signal s and (JAC) J[p] = ds/dtheta_p.
+// Loops run over compile-time bounds with runtime guards so that, after
+// inlining, every slot index is a constant and a voxel's measurements stay
+// in registers.  Acquisition tables come from the by-value gc_model kernel
+// argument (wave-uniform: scalar loads / SGPRs).
+#pragma once
+#include "gc_device.hpp"
+#include "../../include/gacelle.h"
+
+namespace gc {
+
+// ---------------------------------------------------------------------------
+// s = m0 * exp(-te * r2s): R2star/gpuR2starMapping.m:427-434 (FWD :321-363)
+// theta = {M0, R2star}
+template <int MT>
+struct R2sModel {
+  static constexpr int NP = 2, MR = MT, NE = 0;
+  GC_DEV static bool valid(const gc_model& md, int r) { return r < md.i[0]; }
+  GC_DEV static int meas(const gc_model&, int r) { return r; }
+  template <bool JAC, class F>
+  GC_DEV static void eval(const gc_model& md, const float* th, const float*, F&& f) {
+    const int nte = md.i[0];
+    const float m0 = th[0], r2s = th[1];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      if (t < nte) {
+        const float te = md.table[t];
+        const float e = __expf(-te * r2s);
+        const float s = m0 * e;
+        float J[NP];
+        if (JAC) {
+          J[0] = e;
+          J[1] = -te * s;
+        }
+        f(t, s, J);
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Joint VFA-SPGR x multi-echo, R1R2s/gpuJointR1R2starMapping.m:513-525:
+//   s = m0*sin(a)*(1-E1)/(1-cos(a)*E1)*exp(-te*r2s), E1 = exp(-TR*R1),
+//   a = B1*FA (:384-391) or the supplied true flip angle.
+// theta = {M0, R1, R2star}; register slot t + MT*f holds measurement t + nte*f.
+template <int MT, int MF>
+struct JointModel {
+  static constexpr int NP = 3, MR = MT * MF, NE = 1;
+  GC_DEV static bool valid(const gc_model& md, int r) { return (r % MT) < md.i[0] && (r / MT) < md.i[1]; }
+  GC_DEV static int meas(const gc_model& md, int r) { return (r % MT) + md.i[0] * (r / MT); }
+  template <bool JAC, class F>
+  GC_DEV static void eval(const gc_model& md, const float* th, const float* ex, F&& f) {
+    const int nte = md.i[0], nfa = md.i[1];
+    const float tr = md.f[0];
+    const float m0 = th[0], r1 = th[1], r2s = th[2];
+    float ete[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) ete[t] = (t < nte) ? __expf(-md.table[t] * r2s) : 0.f;
+    const float e1 = __expf(-tr * r1);
+#pragma unroll
+    for (int fi = 0; fi < MF; ++fi) {
+      if (fi < nfa) {
+        const float alpha = (md.flags & GC_FLAG_R1R2S_TRUE_FA) ? ex[fi] : ex[0] * md.table[nte + fi];
+        float sa, ca;
+        __sincosf(alpha, &sa, &ca);
+        const float den = 1.f - ca * e1;
+        const float rden = 1.f / den;
+        const float g = sa * (1.f - e1) * rden;  // amplitude per unit M0
+        const float amp = m0 * g;
+        // d amp / d R1 = m0*sa*TR*E1*(1-ca)/den^2
+        const float damp_r1 = m0 * sa * tr * e1 * (1.f - ca) * rden * rden;
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          if (t < nte) {
+            const float s = amp * ete[t];
+            float J[NP];
+            if (JAC) {
+              J[0] = g * ete[t];
+              J[1] = damp_r1 * ete[t];
+              J[2] = -md.table[t] * s;
+            }
+            f(t + MT * fi, s, J);
+          }
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// NEXI spherical mean (NEXI/gpuNEXI.m:902-996, integrand :1090-1097), with
+// optional Rician first-moment correction (NEXI/gpuNEXIrice.m:635-660,
+// 726-755).  Quadrature S = sum_j w_j M(x_j) with nodes/weights from the
+// table (the trapezoid of gacelle_trapz.m:217-226 is a fixed weight set).
+// theta = {fa, Da, De, ra[, p2]}; slot k = Sl0 of shell k, MS + k = Sl2.
+template <int MS, bool L2>
+struct NexiModel {
+  static constexpr int NP = L2 ? 5 : 4, MR = L2 ? 2 * MS : MS, NE = 1;
+  GC_DEV static bool valid(const gc_model& md, int r) { return (r % MS) < md.i[0]; }
+  GC_DEV static int meas(const gc_model& md, int r) { return (r < MS) ? r : md.i[0] + (r - MS); }
+
+  // L_1/2(x) and dL/dx with the 34-node Bessel trapezoid (utils/rician.m:44-56)
+  GC_DEV static void laguerre_half(const gc_model& md, float x, float& L, float& dL) {
+    const int ns = md.i[0], nn = md.i[2];
+    const float* cost = md.table + 2 * ns + 2 * nn;
+    bool clamped = false;
+    if (x < -170.f) { x = -170.f; clamped = true; }
+    const float z = -0.5f * x;
+    float i0 = 0.f, i1 = 0.f, d0 = 0.f, d1 = 0.f;
+    for (int k = 0; k < 34; ++k) {
+      const float c = cost[k];
+      const float wk = (k == 0 || k == 33) ? 0.5f : 1.f;
+      const float e = __expf(z * c) * wk;
+      i0 += e;
+      i1 += e * c;
+      d0 += e * c;
+      d1 += e * c * c;
+    }
+    const float h = 1.f / 33.f;  // (pi/33) / pi
+    i0 *= h; i1 *= h; d0 *= h; d1 *= h;
+    const float ex2 = __expf(0.5f * x);
+    const float core = (1.f - x) * i0 - x * i1;
+    L = ex2 * core;
+    // dI/dx = dI/dz * (-1/2)
+    const float dcore = -i0 + (1.f - x) * d0 * (-0.5f) - i1 - x * d1 * (-0.5f);
+    dL = clamped ? 0.f : (0.5f * L + ex2 * dcore);
+  }
+
+  template <bool JAC, class F>
+  GC_DEV static void eval(const gc_model& md, const float* th, const float* ex, F&& f) {
+    const int ns = md.i[0], nn = md.i[2];
+    const bool guard = (md.flags & GC_FLAG_NEXI_RE_GUARD) != 0;
+    const float fa_raw = th[0];
+    const float fa = fmaxf(fa_raw, md.f[0]);
+    const float dfa = (fa_raw >= md.f[0]) ? 1.f : 0.f;
+    const float Da = th[1], De = th[2], ra = th[3];
+    const float omf = 1.f - fa;
+    const float den_re = guard ? fmaxf(omf, 1e-8f) : omf;
+    const float rden_re = 1.f / den_re;
+    const float* xs = md.table + 2 * ns;
+    const float* ws = md.table + 2 * ns + nn;
+#pragma unroll
+    for (int k = 0; k < MS; ++k) {
+      if (k < ns) {
+        const float b = md.table[k], D = md.table[ns + k];
+        const float dab = b * Da, deb = b * De, rab = D * ra;
+        const float re = rab * fa * rden_re;
+        // d re / d fa (through both numerator and 1/(1-fa))
+        const float dre_dfa = (guard && omf < 1e-8f) ? rab * rden_re : rab * rden_re * rden_re;
+        const float dre_dra = D * fa * rden_re;
+        float s0 = 0.f, s2 = 0.f;
+        float gA0 = 0, gD10 = 0, gD20 = 0, gR10 = 0, gR20 = 0;  // Sl0 partial sums
+        float gA2 = 0, gD12 = 0, gD22 = 0, gR12 = 0, gR22 = 0;  // Sl2 partial sums
+        for (int j = 0; j < nn; ++j) {
+          const float x = xs[j], w = ws[j];
+          const float x2 = x * x;
+          const float d1 = dab * x2, d2 = deb, r1 = rab, r2 = re;
+          const float l1 = 0.5f * (r1 + r2 + d1 + d2);
+          const float u = r1 - r2 + d1 - d2;
+          const float q = u * u + 4.f * r1 * r2;
+          const float sq = sqrtf(q);
+          float l2 = 0.5f * sq;
+          const bool l2c = l2 < 1e-8f;
+          if (l2c) l2 = 1e-8f;
+          const float lm = l1 - l2;
+          const float rl2 = 1.f / l2;
+          const float num = fa * d1 + (1.f - fa) * d2 - lm;
+          const float Pp = 0.5f * num * rl2;
+          const float E1 = __expf(-(l1 + l2));
+          const float E2 = __expf(-lm);
+          const float Mx = Pp * E1 + (1.f - Pp) * E2;
+          const float w2 = w * (1.5f * x2 - 0.5f);
+          s0 += w * Mx;
+          if (L2) s2 += w2 * Mx;
+          if (JAC) {
+            // partials of l2 with respect to d1, d2, r1, r2
+            const float c = l2c ? 0.f : 0.25f * rl2;
+            const float l2_d1 = c * u, l2_d2 = -c * u, l2_r1 = c * (u + 2.f * r2), l2_r2 = c * (2.f * r1 - u);
+            const float EmE = E1 - E2;
+            // dM/dz = dPp/dz*(E1-E2) - Pp*E1*(dl1+dl2) - (1-Pp)*E2*(dl1-dl2)
+            // dPp/dz = (dnum/dz)/(2 l2) - Pp*dl2/dz / l2 ; dnum = da*(d1-d2) + a dd1 + (1-a) dd2 - dlm
+            auto dM = [&](float dnum_extra, float dl2) {
+              const float dl1 = 0.5f;
+              const float dlm = dl1 - dl2;
+              const float dPp = 0.5f * (dnum_extra - dlm) * rl2 - Pp * dl2 * rl2;
+              return dPp * EmE - Pp * E1 * (dl1 + dl2) - (1.f - Pp) * E2 * dlm;
+            };
+            const float mA = 0.5f * (d1 - d2) * rl2 * EmE;
+            const float mD1 = dM(fa, l2_d1);
+            const float mD2 = dM(1.f - fa, l2_d2);
+            const float mR1 = dM(0.f, l2_r1);
+            const float mR2 = dM(0.f, l2_r2);
+            gA0 += w * mA; gD10 += w * mD1 * x2; gD20 += w * mD2; gR10 += w * mR1; gR20 += w * mR2;
+            if (L2) { gA2 += w2 * mA; gD12 += w2 * mD1 * x2; gD22 += w2 * mD2; gR12 += w2 * mR1; gR22 += w2 * mR2; }
+          }
+        }
+        // chain to theta = {fa, Da, De, ra}
+        float J0[NP], J2[NP];
+        if (JAC) {
+          J0[0] = (gA0 + gR20 * dre_dfa) * dfa;
+          J0[1] = gD10 * b;
+          J0[2] = gD20 * b;
+          J0[3] = gR10 * D + gR20 * dre_dra;
+          if (L2) J0[4] = 0.f;
+        }
+        if (md.flags & GC_FLAG_NEXI_RICIAN) {
+          const float sigma = ex[0];
+          const float xr = -(s0 * s0) * 0.5f / (sigma * sigma);
+          float L, dL;
+          laguerre_half(md, xr, L, dL);
+          const float c0 = sigma * 1.2533141373155003f;  // sigma*sqrt(pi/2)
+          const float m1 = c0 * L;
+          if (m1 >= s0) {
+            const float ds = c0 * dL * (-s0 / (sigma * sigma));
+            if (JAC) {
+#pragma unroll
+              for (int p = 0; p < NP; ++p) J0[p] *= ds;
+            }
+            s0 = m1;
+          }
+        }
+        if (s0 > 1.f) {
+          s0 = 1.f;
+          if (JAC) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) J0[p] = 0.f;
+          }
+        }
+        f(k, s0, J0);
+        if (L2) {
+          const float p2 = th[4];
+          const float sg = sgnf(s2);
+          float sl2 = p2 * fabsf(s2);
+          if (JAC) {
+            J2[0] = p2 * sg * (gA2 + gR22 * dre_dfa) * dfa;
+            J2[1] = p2 * sg * gD12 * b;
+            J2[2] = p2 * sg * gD22 * b;
+            J2[3] = p2 * sg * (gR12 * D + gR22 * dre_dra);
+            J2[4] = fabsf(s2);
+          }
+          if (sl2 > 1.f) {
+            sl2 = 1.f;
+            if (JAC) {
+#pragma unroll
+              for (int p = 0; p < NP; ++p) J2[p] = 0.f;
+            }
+          }
+          f(MS + k, sl2, J2);
+        }
+      }
+    }
+  }
+};
+
+}  // namespace gc

@@ -1,0 +1,327 @@
+/*
+ * gacelle.h -- C ABI of the MI355X (gfx950) voxelwise-estimation library
+ * (libgacelle_hip.so).
+ *
+ * The library replaces GACELLE's MATLAB hot path: the askadam minimisation
+ * loop, the mcmc Metropolis-Hastings and Goodman-Weare samplers, and the
+ * forward models they drive.  Every entry point below names the reference
+ * interface it stands in for (paths relative to the GACELLE source tree).
+ *
+ * Conventions
+ *  - Plain C11, no C++ or framework types.  All functions return an int
+ *    status: GC_OK (0) or a negative GC_ERR_* code; gc_last_error() gives
+ *    the message.  Nothing throws across the ABI.
+ *  - Array arguments named *_dev are DEVICE pointers (hipMalloc'ed or any
+ *    device-accessible allocation) on the context's device.  Small
+ *    descriptor structs (gc_model, gc_askadam_cfg, gc_mcmc_cfg) are host
+ *    memory and are copied by value.
+ *  - Voxel data use the "G-D" layout of utils/utils.m:157-163 (measurement
+ *    by voxel), stored structure-of-arrays with the voxel index fastest:
+ *    element (m, v) of an [M x Nv] matrix lives at m*Nv + v.  Measurement
+ *    order is that of utils/utils.m:215-238: echo time fastest, then flip
+ *    angle (or shell), then real/imaginary.
+ *  - Parameter maps are [P x Nv] in the same layout, in the order of the
+ *    model class's modelParams list.
+ *  - A context is used by one host thread at a time; calls are ordered on
+ *    the context's HIP stream.
+ */
+#ifndef GACELLE_H
+#define GACELLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GC_ABI_VERSION 1
+#define GC_MAX_PAR 24   /* parameter slots per voxel            */
+#define GC_MAX_TABLE 256 /* acquisition table entries             */
+#define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
+#define GC_MAX_REG 4    /* regularised parameter maps             */
+
+/* ---- status codes (mirroring the MATLAB error identifiers) ------------- */
+enum {
+  GC_OK = 0,
+  GC_ERR_ARG = -1,            /* invalid argument / shape                */
+  GC_ERR_HIP = -2,            /* HIP runtime failure                     */
+  GC_ERR_UNSUPPORTED = -3,    /* option not implemented by this library  */
+  GC_ERR_NOMEM = -4,          /* 'GACELLE:memoryError' (utils/utils.m:762) */
+  GC_ERR_NWALKER = -5,        /* 'goodman_weare:Nwalker' (utils/mcmc.m:243) */
+  GC_ERR_INPUT_MISMATCH = -6, /* 'GACELLE:inputMismatch'                  */
+  GC_ERR_NO_DEVICE = -7       /* no usable gfx950 device                  */
+};
+
+/* ---- forward models ----------------------------------------------------- */
+enum {
+  GC_MODEL_R2STAR = 1,         /* R2star/gpuR2starMapping.m                 */
+  GC_MODEL_JOINT_R1R2STAR = 2, /* R1R2s/gpuJointR1R2starMapping.m           */
+  GC_MODEL_NEXI = 3,           /* NEXI/gpuNEXI.m, NEXI/gpuNEXIrice.m        */
+  GC_MODEL_MCRMWI = 4          /* MCRMWI/gpuMCRMWI.m (BM / Bloch T1 paths)  */
+};
+
+/* model option flags (gc_model.flags) */
+enum {
+  GC_FLAG_R1R2S_TRUE_FA = 1 << 0,  /* extra rows hold trueFlipAngle per FA    */
+  GC_FLAG_NEXI_RICIAN = 1 << 1,    /* Rician mean correction, sigma = extra row 0 */
+  GC_FLAG_NEXI_RE_GUARD = 1 << 2,  /* re = ra*fa/max(1-fa,eps) (gpuNEXIrice.m:678) */
+  GC_FLAG_MCRMWI_COMPLEX = 1 << 3, /* complex data: re/im stacked, dfreqBKG/dpini fitted */
+  GC_FLAG_MCRMWI_EXCHANGE = 1 << 4 /* BM two-pool exchange steady state (kIEWM fitted) */
+};
+
+/* NEXI quadrature kinds (gc_model.i[3]) */
+enum { GC_QUAD_TRAPZ = 0, GC_QUAD_TABLE = 1 };
+
+/*
+ * Model description.  Acquisition tables are uniform across voxels; the
+ * library keeps a device copy that kernels read at wave-uniform addresses
+ * (scalar loads through the scalar cache on gfx950).
+ *
+ *   R2STAR  : i[0]=n_te; table[0:n_te]=TE [s].              P=2 {M0,R2star}
+ *   JOINT   : i[0]=n_te, i[1]=n_fa; f[0]=TR [s];
+ *             table[0:n_te]=TE [s], table[n_te:n_te+n_fa]=FA [rad];
+ *             extra row 0 = B1 (or n_fa rows of true flip angle [rad]).
+ *                                                        P=3 {M0,R1,R2star}
+ *   NEXI    : i[0]=n_shell, i[1]=lmax (0|2), i[2]=n_nodes, i[3]=quad kind;
+ *             table[0:ns]=b [ms/um^2], table[ns:2ns]=Delta [ms],
+ *             table[2ns:2ns+nn]=nodes x in [0,1],
+ *             table[2ns+nn:2ns+2nn]=quadrature weights (the trapezoid's
+ *             weights for GC_QUAD_TRAPZ), table[2ns+2nn:+34]=cos of the 34
+ *             Bessel trapezoid nodes on [0,pi] (Rician correction);
+ *             f[0]=fa floor (askadam.epsilon, 1e-8); extra row 0 = sigma
+ *             when GC_FLAG_NEXI_RICIAN.
+ *                                        P=4 {fa,Da,De,ra} (+p2 when lmax=2)
+ *   MCRMWI  : i[0]=n_te, i[1]=n_fa; f[0]=TR [s], f[1]=B0 [T], f[2]=gyro
+ *             [MHz/T], f[3]=rho_mw, f[4]=t1_mw [s], f[5]=freq_EW [ppm];
+ *             table[0:n_te]=TE [s], table[n_te:n_te+n_fa]=FA [rad];
+ *             extra rows: 0=B1, 1=pini [rad], 2=ff, 3..3+n_fa=freqBKG [ppm].
+ *             P = {S0,MWF,IWF,R1IEW,[kIEWM],R2sMW,R2sIW,R2sEW,freqMW,freqIW,
+ *                  [dfreqBKG x n_fa, dpini]}  ([] depends on flags)
+ *
+ * n_par may exceed the model's own parameter count: trailing slots are
+ * carried (and optimised with zero gradient) exactly like a modelParams
+ * entry that the forward model never reads.
+ */
+typedef struct gc_model {
+  int32_t model_id;
+  int32_t n_meas;  /* M: real measurements per voxel              */
+  int32_t n_par;   /* P: parameter slots per voxel                */
+  int32_t n_extra; /* rows of per-voxel auxiliary data            */
+  int32_t flags;
+  int32_t i[8];
+  float f[16];
+  float table[GC_MAX_TABLE];
+  float lb[GC_MAX_PAR]; /* fitting.lb, one entry per slot          */
+  float ub[GC_MAX_PAR]; /* fitting.ub                              */
+} gc_model;
+
+/* ---- askadam configuration: POD mirror of askadam.check_set_default_basic
+ *      (utils/askadam.m:735-878) ------------------------------------------ */
+enum { GC_OPT_ADAM = 0, GC_OPT_SGDM = 1, GC_OPT_RMSPROP = 2 };
+enum { GC_LOSS_L1 = 0, GC_LOSS_L2 = 1, GC_LOSS_HUBER = 2, GC_LOSS_MSE = 3 };
+enum { GC_TRANSFORM_SIGMOID = 0, GC_TRANSFORM_LINEAR = 1, GC_TRANSFORM_NONE = 2 };
+enum { GC_CONV_EMA = 0, GC_CONV_LINEAR = 1 };
+/* bias-correction form of adamupdate: lr*sqrt(1-b2^t)/(1-b1^t) * m/(sqrt(v)+eps)
+ * (effective learning rate) or Kingma & Ba Alg. 1: lr * mhat/(sqrt(vhat)+eps) */
+enum { GC_ADAM_EFFECTIVE_LR = 0, GC_ADAM_KINGMA_BA = 1 };
+enum { GC_TV_2D = 0, GC_TV_3D = 1 };
+enum {
+  GC_STOP_NONE = 0,        /* ran all iterations                      */
+  GC_STOP_CONVERGENCE = 1, /* loss convergence below tolerance        */
+  GC_STOP_TOL = 2,         /* loss below fitting.tol                  */
+  GC_STOP_STEP = 3,        /* step norm below convergenceStepTol      */
+  GC_STOP_GRAD = 4         /* gradient norm below convergenceGradTol  */
+};
+
+typedef struct gc_askadam_cfg {
+  int32_t iteration;          /* fitting.iteration                      */
+  int32_t optimiser;          /* GC_OPT_*                               */
+  int32_t loss;               /* GC_LOSS_* (fitting.lossFunction)       */
+  int32_t transform;          /* GC_TRANSFORM_SIGMOID | _LINEAR         */
+  int32_t adam_form;          /* GC_ADAM_*                              */
+  int32_t convergence_model;  /* GC_CONV_*                              */
+  int32_t convergence_window; /* fitting.convergenceWindow              */
+  int32_t patience_conv;      /* fitting.patienceConvergence            */
+  int32_t patience_step;      /* fitting.patienceStep                   */
+  int32_t patience_grad;      /* fitting.patienceGrad                   */
+  int32_t block_iters;        /* iterations per launch (0 = automatic)  */
+  int32_t tv_mode;            /* GC_TV_2D | GC_TV_3D                    */
+  int32_t n_reg;              /* number of regularised maps (0 = none)  */
+  int32_t regmap[GC_MAX_REG]; /* parameter slot of each regularised map */
+  int32_t robust;             /* fitting.robustConvergence (must be 0)  */
+  int32_t reserved;
+  double lr0;                 /* fitting.initialLearnRate               */
+  double decay;               /* fitting.decayRate                      */
+  double beta1, beta2, eps;   /* adamupdate decays and epsilon          */
+  double momentum;            /* sgdmupdateMomentum                     */
+  double rms_decay, rms_eps;  /* rmspropupdateSqGradDecay / Epsilon     */
+  double tol;                 /* fitting.tol                            */
+  double convergence_value;   /* fitting.convergenceValue               */
+  double ema_decay;           /* fitting.emaDecay                       */
+  double step_tol;            /* fitting.convergenceStepTol (0 = off)   */
+  double grad_tol;            /* fitting.convergenceGradTol (0 = off)   */
+  double lambda[GC_MAX_REG];  /* fitting.lambda                         */
+  double voxel_size[3];       /* fitting.voxelSize                      */
+} gc_askadam_cfg;
+
+/* One masked volume of a batched askadam run.  Inputs are device arrays;
+ * scalar outputs are written back into this struct. */
+typedef struct gc_volume {
+  int64_t nv;                /* masked voxels                            */
+  const float* y_dev;        /* [M x nv] data                            */
+  const float* w_dev;        /* [M x nv] weights, NULL = ones            */
+  const float* extra_dev;    /* [n_extra x nv], NULL when n_extra = 0    */
+  const int64_t* vox_dev;    /* [nv] linear grid index of each voxel
+                                (column-major find(mask)); TV only       */
+  int32_t dims[3];           /* image grid (TV only)                     */
+  int32_t reserved0;
+  float* z_dev;              /* in: initial optimisation variables [P x nv]
+                                (askadam.initialise_parameter space);
+                                out: final variables (out.final)         */
+  float* zmin_dev;           /* out: variables at minimum loss (out.min) */
+  float* resloss_dev;        /* out: [nv] per-voxel mean residual, final; NULL ok */
+  float* resloss_min_dev;    /* out: [nv] same at minimum loss; NULL ok  */
+  float* residual_dev;       /* out: [M x nv] residuals, final; NULL ok  */
+  float* residual_min_dev;   /* out: [M x nv] residuals at min; NULL ok  */
+  double* loss_trace_host;   /* optional HOST array [iteration+1]: loss at
+                                the starting point and at every evaluated
+                                iteration (NULL = not recorded)         */
+  /* scalar outputs */
+  double loss, loss_fidelity, loss_reg;
+  double min_loss, min_loss_fidelity, min_loss_reg;
+  double convergence;        /* final convergence signal                 */
+  int32_t n_iteration;       /* out.final.Niteration                     */
+  int32_t min_iteration;     /* out.min.Niteration                       */
+  int32_t stop_reason;       /* GC_STOP_*                                */
+  int32_t n_launch;          /* kernel launches used (diagnostic)        */
+} gc_volume;
+
+/* ---- MCMC configuration: POD mirror of mcmc.check_set_default_basic
+ *      (utils/mcmc.m:633-679) ---------------------------------------------- */
+enum { GC_MCMC_MH = 0, GC_MCMC_GW = 1 };
+enum { GC_ENSEMBLE_SIMULTANEOUS = 0, GC_ENSEMBLE_REDBLACK = 1 };
+enum { GC_RNG_NATIVE = 0, GC_RNG_REPLAY = 1 };
+
+typedef struct gc_mcmc_cfg {
+  int32_t algorithm;       /* GC_MCMC_MH | GC_MCMC_GW                   */
+  int32_t iteration;       /* fitting.iteration                         */
+  int32_t thinning;        /* fitting.thinning                          */
+  int32_t n_burnin;        /* mcmc.get_number_burnin (utils/mcmc.m:788) */
+  int32_t repetition;      /* fitting.repetition (independent chains)   */
+  int32_t n_walker;        /* fitting.Nwalker (GW)                      */
+  int32_t ensemble_update; /* GC_ENSEMBLE_* (fitting.Ensembleupdate)    */
+  int32_t rng_mode;        /* GC_RNG_NATIVE | GC_RNG_REPLAY             */
+  int32_t noise_slot;      /* index of 'noise' among the Q variables    */
+  int32_t n_var;           /* Q: variables per voxel incl. noise        */
+  uint64_t seed;           /* native-RNG seed                           */
+  double step_size;        /* fitting.StepSize (GW stretch scale a)     */
+  double start_range;      /* fitting.startRange (GW walker spread)     */
+  float x_step[GC_MAX_PAR];/* fitting.xStepSize (MH proposal sd)        */
+} gc_mcmc_cfg;
+
+typedef struct gc_mcmc_io {
+  int64_t nv;
+  const float* y_dev;        /* [M x nv]                                  */
+  const float* w_dev;        /* [M x nv] or NULL (ones)                   */
+  const float* extra_dev;    /* [n_extra x nv] or NULL                    */
+  const float* x0_dev;       /* [Q x nv] starting point                   */
+  /* replay draws (GC_RNG_REPLAY), in the reference's draw order:
+   *  MH : normals  [rep][iter][nv][Q]   (randn([Q,Nv]) of utils/mcmc.m:147)
+   *       uniforms [rep][iter][nv]      (rand(1,Nv) of :161)
+   *  GW : normals  [W][nv][Q]           (walker initialisation, :271)
+   *       uniforms [rep][iter][2][W][nv] (zz draw then acceptance draw,
+   *                 :299 and :314; red-black: per half [rep][iter][2 halves]
+   *                 [2][W/2][nv])
+   *       partners [rep][iter][W]       (find_partner derangement, 0-based;
+   *                 red-black: [rep][iter][2][W/2] anchor walker indices)  */
+  const float* normals_dev;
+  const float* uniforms_dev;
+  const int32_t* partners_dev;
+  /* outputs */
+  float* posterior_dev;      /* [Q][rep][Ns][W][nv] (MH: W = 1) or NULL  */
+  float* mean_dev;           /* [Q x nv] posterior mean (res2out 'mean')  */
+  float* std_dev;            /* [Q x nv] posterior std, N-1 ('std')       */
+  float* accept_dev;         /* [nv] acceptance fraction, or NULL          */
+  float* xfinal_dev;         /* [rep][W][Q][nv] final chain state, or NULL */
+  uint8_t* trace_dev;        /* test hook: [rep][iter][W][nv] accept flags */
+  int32_t n_sample;          /* out: Ns = numel(Nburnin+1:thinning:iteration) */
+  int32_t n_stored;          /* out: samples actually stored per chain    */
+} gc_mcmc_io;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct gc_ctx gc_ctx;
+
+/* ABI version of the loaded library (GC_ABI_VERSION). */
+int gc_version(void);
+
+/* Create a context on HIP device `device` (replaces gpuDevice,
+ * utils/askadam.m:149 and utils/mcmc.m:113-118).  Work is issued on the
+ * device's default stream until gc_ctx_set_stream. */
+int gc_ctx_create(int32_t device, gc_ctx** out);
+int gc_ctx_destroy(gc_ctx* ctx);
+/* Issue work on an existing hipStream_t (NULL = the default stream). */
+int gc_ctx_set_stream(gc_ctx* ctx, void* hip_stream);
+int gc_ctx_synchronize(gc_ctx* ctx);
+const char* gc_last_error(const gc_ctx* ctx);
+
+/* ---- layout helpers (utils/utils.m:157-163, 194-198, 299-315) ----------- */
+/* G-D gather: gd[m*nv+v] = nd[vox[v] + nvox_grid*m]; nd is column-major
+ * [X,Y,Z,M] (reshape_ND2GD with find(mask) order). */
+int gc_nd_to_gd(gc_ctx* ctx, const float* nd_dev, int64_t nvox_grid, int32_t n_meas,
+                const int64_t* vox_dev, int64_t nv, float* gd_dev);
+/* Inverse scatter with zeros outside the mask (reshape_GD2ND). */
+int gc_gd_to_nd(gc_ctx* ctx, const float* gd_dev, int64_t nv, int32_t n_meas,
+                const int64_t* vox_dev, int64_t nvox_grid, float* nd_dev);
+
+/* ---- parameter transforms (utils/askadam.m:368-412, 1063-1099) ---------- */
+/* theta -> z: clamp to [lb+1e-4(ub-lb), ub-1e-4(ub-lb)] then logit
+ * (sigmoid), or rescale01 (linear). */
+int gc_rescale_parameters(gc_ctx* ctx, const gc_model* model, int32_t transform,
+                          const float* theta_dev, float* z_dev, int64_t nv);
+/* z -> theta: lb + (ub-lb)*sigmoid(z) or z*(ub-lb)+lb. */
+int gc_unscale_parameters(gc_ctx* ctx, const gc_model* model, int32_t transform,
+                          const float* z_dev, float* theta_dev, int64_t nv);
+
+/* ---- operator pair (the FWD plugin + dlgradient of askadam.model_gradient,
+ *      utils/askadam.m:23-117) ------------------------------------------- */
+/* s = FWD(theta): R2star/gpuR2starMapping.m:321-363,
+ * R1R2s/gpuJointR1R2starMapping.m:375-422, NEXI/gpuNEXI.m:902-996,
+ * MCRMWI/gpuMCRMWI.m:566-714.  NaN/Inf outputs are zeroed
+ * (utils/utils.m:347-350). */
+int gc_fwd(gc_ctx* ctx, const gc_model* model, const float* theta_dev,
+           const float* extra_dev, float* s_dev, int64_t nv);
+/* Fused forward + adjoint of loss = scale * sum(lossfn(w, FWD(x), y)).
+ * x is theta (transform = GC_TRANSFORM_NONE) or z (sigmoid/linear, the
+ * gradient then includes the transform).  grad_dev [P x nv] receives
+ * d loss / d x; loss_sum_dev (one double) receives sum(lossfn) (unscaled);
+ * residual_dev (optional, [M x nv]) receives the elementwise loss. */
+int gc_fwd_adj(gc_ctx* ctx, const gc_model* model, int32_t loss, int32_t transform,
+               const float* x_dev, const float* y_dev, const float* w_dev,
+               const float* extra_dev, int64_t nv, float scale, float* grad_dev,
+               double* loss_sum_dev, float* residual_dev);
+
+/* ---- regulariser (utils/spatial_total_variation.m:24-66) ---------------- */
+/* loss = lambda/n_masked * sum_i mask_i * sum_d |D_d img_i| / vsize_d with
+ * periodic forward differences; grad_img_dev (optional) receives
+ * d loss / d img (subgradient, sign(0) = 0).  img is column-major [X,Y,Z]. */
+int gc_tv_fwd_adj(gc_ctx* ctx, const float* img_dev, const uint8_t* mask_dev,
+                  const int32_t dims[3], int32_t tv_mode, const double voxel_size[3],
+                  double lambda, int64_t n_masked, double* loss_dev, float* grad_img_dev);
+
+/* ---- solvers ------------------------------------------------------------ */
+/* askadam().optimisation (utils/askadam.m:121-365) for one model over a
+ * batch of independent volumes, each with its own N, stopping state and
+ * outputs. */
+int gc_askadam_run(gc_ctx* ctx, const gc_model* model, const gc_askadam_cfg* cfg,
+                   gc_volume* vols, int32_t n_vol);
+
+/* mcmc().metropolis_hastings (utils/mcmc.m:82-190) and
+ * mcmc().goodman_weare (utils/mcmc.m:192-383), with res2out mean/std
+ * (utils/mcmc.m:806-862) computed on the device. */
+int gc_mcmc_run(gc_ctx* ctx, const gc_model* model, const gc_mcmc_cfg* cfg,
+                gc_mcmc_io* io);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GACELLE_H */

@@ -1,0 +1,799 @@
+// oracle/oracle.cpp -- CPU restatement of GACELLE's voxelwise-estimation
+// hot path: askadam (utils/askadam.m), mcmc MH / Goodman-Weare
+// (utils/mcmc.m), logP_Gaussian (utils/logP_Gaussian.m) and
+// spatial_total_variation (utils/spatial_total_variation.m).
+//
+// TEST INFRASTRUCTURE ONLY.  It is the checker for the HIP library's parity
+// tests and the CPU baseline timed beside it; nothing in the library links
+// or calls it.  Arithmetic is templated: double for the parity oracle,
+// float (+ OpenMP over voxels) for the CPU-path baseline.
+//
+// Every function cites the reference lines it restates.  Semantics of the
+// closed MATLAB built-ins (adamupdate, l1loss, ...) are stated where used.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <type_traits>
+#include <vector>
+
+#include "../include/gacelle.h"
+#include "dual.hpp"
+#include "models.hpp"
+
+namespace orc {
+
+// ===========================================================================
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random numbers:
+// as easy as 1, 2, 3"), written from the paper's round function.  The native
+// RNG stream of the library is defined in terms of this generator.
+static inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+  const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+  const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+  const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+  const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+  const uint32_t n0 = hi1 ^ c[1] ^ k[0];
+  const uint32_t n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0;
+  c[1] = lo1;
+  c[2] = n2;
+  c[3] = lo0;
+}
+
+static inline void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  uint32_t k[2] = {key[0], key[1]};
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k[0] += 0x9E3779B9u;
+      k[1] += 0xBB67AE85u;
+    }
+    philox_round(c, k);
+  }
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+// uint32 -> uniform in (0,1): ((x >> 8) + 0.5) * 2^-24 (exact in float)
+static inline double u01(uint32_t x) { return (double(x >> 8) + 0.5) * (1.0 / 16777216.0); }
+
+// Native stream layout.  Counter = {iteration, chain_lo, chain_hi, purpose}.
+enum : uint32_t { PURPOSE_NORMAL0 = 0, PURPOSE_ACCEPT = 15, PURPOSE_ZZ = 16, PURPOSE_INIT = 32 };
+
+static inline void draw(uint64_t seed, uint32_t it, uint64_t chain, uint32_t purpose, uint32_t out[4]) {
+  const uint32_t ctr[4] = {it, uint32_t(chain), uint32_t(chain >> 32), purpose};
+  const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+  philox4x32_10(ctr, key, out);
+}
+
+// Box-Muller normal number q of a draw group (4 normals per Philox call).
+static inline double native_normal(uint64_t seed, uint32_t it, uint64_t chain, uint32_t base, int q) {
+  uint32_t o[4];
+  draw(seed, it, chain, base + uint32_t(q / 4), o);
+  const int pair = (q % 4) / 2;
+  const double u1 = u01(o[2 * pair]), u2 = u01(o[2 * pair + 1]);
+  const double r = std::sqrt(-2.0 * std::log(u1));
+  const double a = 2.0 * kPi * u2;
+  return (q % 2 == 0) ? r * std::cos(a) : r * std::sin(a);
+}
+
+// Partner tables for the native GW stream (shared by all voxels, as the
+// reference's find_partner / randi draws are, utils/mcmc.m:297,329,797-803).
+// Simultaneous: a uniform derangement of 0..W-1 by Fisher-Yates shuffles
+// rejected until no fixed point (the randperm-until-derangement loop).
+// Red-black: for each half, na anchors drawn uniformly with replacement
+// from the other half.
+static void native_partners(uint64_t seed, int rep, int it, int W, int mode, int32_t* out) {
+  const uint64_t stream = (uint64_t(0xFFFFu) << 48) | (uint64_t(rep) << 32);
+  uint32_t attempt = 0;
+  if (mode == GC_ENSEMBLE_SIMULTANEOUS) {
+    for (;;) {
+      for (int i = 0; i < W; ++i) out[i] = i;
+      for (int i = W - 1; i > 0; --i) {
+        uint32_t o[4];
+        draw(seed, uint32_t(it), stream + attempt, uint32_t(i), o);
+        const int j = int((uint64_t(o[0]) * uint64_t(i + 1)) >> 32);
+        std::swap(out[i], out[j]);
+      }
+      bool ok = true;
+      for (int i = 0; i < W; ++i) ok = ok && (out[i] != i);
+      if (ok || W < 2) break;
+      ++attempt;
+    }
+  } else {
+    const int h = W / 2;
+    const int na[2] = {h, W - h};
+    int pos = 0;
+    for (int s = 0; s < 2; ++s) {
+      const int fro0 = (s == 0) ? h : 0;
+      const int nfro = (s == 0) ? W - h : h;
+      for (int j = 0; j < na[s]; ++j) {
+        uint32_t o[4];
+        draw(seed, uint32_t(it), stream, uint32_t(1000 + pos), o);
+        out[pos++] = fro0 + int((uint64_t(o[0]) * uint64_t(nfro)) >> 32);
+      }
+    }
+  }
+}
+
+// MATLAB's min(x, 1) ignores NaN: min(NaN, 1) = 1 (acceptance ratios of
+// utils/mcmc.m:160 and :313).
+template <class S>
+static inline S min1(S x) { return std::isnan(x) ? S(1) : std::min(x, S(1)); }
+
+// ===========================================================================
+// logP_Gaussian, utils/logP_Gaussian.m:18-22
+template <class S>
+static inline S logp_gaussian(S sos, S noise, int nm) {
+  return -sos / (S(2) * noise * noise) - S(nm) / S(2) * std::log(S(2) * S(kPi) * noise * noise);
+}
+
+// ===========================================================================
+// Loss functions of askadam.model_gradient, utils/askadam.m:86-95.  The
+// MATLAB built-ins (l1loss / l2loss / huber / mse with Reduction="none")
+// are taken elementwise: w|d|, w d^2, w*huber_1(d), d^2/2 (unweighted).
+template <class V, class S>
+static inline V loss_elem(int kind, const V& r, S w) {
+  switch (kind) {
+    case GC_LOSS_L1: return fabs(r) * w;
+    case GC_LOSS_L2: return r * r * w;
+    case GC_LOSS_HUBER: {
+      const V a = fabs(r);
+      if (value(a) <= S(1)) return r * r * S(0.5) * w;
+      return (a - S(0.5)) * w;
+    }
+    default: return r * r * S(0.5);
+  }
+}
+
+// parameter transform z -> theta, utils/askadam.m:1063-1073; the linear
+// branch clamps a local copy first (forward_pass, :424-426).
+template <class V, class S>
+static inline V unscale(int transform, const V& z, S lb, S ub) {
+  if (transform == GC_TRANSFORM_SIGMOID) return S(lb) + S(ub - lb) * (S(1) / (S(1) + exp(-z)));
+  if (transform == GC_TRANSFORM_LINEAR) return vmin(vmax(z, S(0)), S(1)) * S(ub - lb) + S(lb);
+  return z;
+}
+
+// One voxel of askadam.model_gradient without the regulariser: the loss
+// elements and d(sum of loss elements)/dz.
+template <class S, int NP>
+static void eval_voxel(const gc_model& md, int loss_kind, int transform, const S* z, const float* y,
+                       const float* w, const S* extra, double& lsum, S* grad, S* elem) {
+  using D = Dual<S, NP>;
+  const int P = md.n_par;
+  D th[GC_MAX_PAR];
+  for (int p = 0; p < P; ++p) {
+    const S zval = z[p];
+    D zp = (p < NP) ? D::var(zval, p) : D(zval);
+    th[p] = unscale<D, S>(transform, zp, S(md.lb[p]), S(md.ub[p]));
+  }
+  D s[1024];
+  fwd_model<D, S>(md, th, extra, s);
+  for (int p = 0; p < P; ++p) grad[p] = 0;
+  lsum = 0;
+  for (int m = 0; m < md.n_meas; ++m) {
+    D sm = s[m];
+    if (!finite(sm)) sm = D(S(0));  // utils.set_nan_inf_zero, utils/utils.m:347-350
+    const D r = sm - S(y[m]);
+    const D l = loss_elem<D, S>(loss_kind, r, w ? S(w[m]) : S(1));
+    lsum += double(l.v);
+    if (elem) elem[m] = l.v;
+    for (int p = 0; p < P && p < NP; ++p) grad[p] += l.d[p];
+  }
+}
+
+// d theta / d z of the transform (for chaining the regulariser gradient)
+template <class S>
+static inline S dtheta_dz(int transform, S z, S lb, S ub) {
+  if (transform == GC_TRANSFORM_SIGMOID) {
+    const S sg = S(1) / (S(1) + std::exp(-z));
+    return (ub - lb) * sg * (S(1) - sg);
+  }
+  if (transform == GC_TRANSFORM_LINEAR) return (z >= S(0) && z <= S(1)) ? (ub - lb) : S(0);
+  return S(1);
+}
+
+// ===========================================================================
+// spatial_total_variation, utils/spatial_total_variation.m:24-66.  img is
+// column-major [X,Y,Z] with zeros outside the mask (reshape_GD2ND).  Cost:
+// sum_i mask_i sum_d |circshift(img,-1,d)_i - img_i| / vsize_d, times
+// lambda / Nmasked.  grad (optional) accumulates the subgradient
+// (sign(0) = 0) with respect to img.
+template <class S>
+static double tv_loss_grad(const S* img, const uint8_t* mask, const int32_t dims[3], int mode,
+                           const double vsize[3], double lambda, int64_t nmasked, S* grad) {
+  const int64_t X = dims[0], Y = dims[1], Z = dims[2];
+  const int nd = (mode == GC_TV_3D) ? 3 : 2;
+  double acc = 0;
+  const double scale = lambda / double(nmasked);
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t yy = 0; yy < Y; ++yy)
+      for (int64_t x = 0; x < X; ++x) {
+        const int64_t i = x + X * (yy + Y * z);
+        if (!mask[i]) continue;
+        for (int d = 0; d < nd; ++d) {
+          int64_t j;
+          if (d == 0) j = ((x + 1) % X) + X * (yy + Y * z);
+          else if (d == 1) j = x + X * (((yy + 1) % Y) + Y * z);
+          else j = x + X * (yy + Y * ((z + 1) % Z));
+          const double g = (double(img[j]) - double(img[i])) / vsize[d];
+          acc += std::fabs(g);
+          if (grad) {
+            const double sg = g > 0 ? 1.0 : (g < 0 ? -1.0 : 0.0);
+            grad[j] += S(scale * sg / vsize[d]);
+            grad[i] -= S(scale * sg / vsize[d]);
+          }
+        }
+      }
+  return acc * scale;
+}
+
+// ===========================================================================
+// askadam().optimisation, utils/askadam.m:121-365 (robustConvergence off).
+struct AskOut {
+  double loss, fid, reg, min_loss, min_fid, min_reg, conv;
+  int32_t niter, min_iter, stop;
+};
+
+template <class S, int NP>
+static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv, const float* y,
+                       const float* w, const float* extra, const int64_t* vox, const int32_t* dims,
+                       const float* z0, double* zfinal, double* zmin, double* res_final, double* res_min,
+                       double* trace, AskOut& out, int nthreads) {
+  const int P = md.n_par, M = md.n_meas, E = md.n_extra;
+  const S inv_count = S(1.0 / (double(M) * double(nv)));
+  std::vector<S> z(size_t(P) * nv), g(size_t(P) * nv), m1(size_t(P) * nv, S(0)), m2(size_t(P) * nv, S(0)),
+      zprev, zmn;
+  for (size_t i = 0; i < z.size(); ++i) z[i] = S(z0[i]);
+  std::vector<S> ex(size_t(std::max(E, 1)) * nv, S(0));
+  for (int64_t i = 0; i < int64_t(E) * nv; ++i) ex[i] = S(extra[i]);
+
+  // linear mode: set_boundary01 of the starting point (:183-185)
+  if (cfg.transform == GC_TRANSFORM_LINEAR)
+    for (auto& v : z) v = std::min(std::max(v, S(0)), S(1));
+
+  const bool use_tv = cfg.n_reg > 0 && cfg.lambda[0] > 0;
+  int64_t nvox = 1;
+  std::vector<uint8_t> mask;
+  std::vector<S> img, gimg;
+  if (use_tv) {
+    nvox = int64_t(dims[0]) * dims[1] * dims[2];
+    mask.assign(nvox, 0);
+    for (int64_t v = 0; v < nv; ++v) mask[vox[v]] = 1;
+    img.assign(nvox, S(0));
+    gimg.assign(nvox, S(0));
+  }
+
+  std::vector<S> elem_cur(size_t(M) * nv);
+  // model_gradient: returns (loss, fid, reg) and fills g, elem_cur
+  auto evaluate = [&](double& loss, double& fid, double& reg) {
+    double lsum_total = 0;
+#pragma omp parallel for schedule(static) reduction(+ : lsum_total) num_threads(nthreads)
+    for (int64_t v = 0; v < nv; ++v) {
+      S zv[GC_MAX_PAR], gv[GC_MAX_PAR], exv[64], el[1024];
+      float yv[1024], wv[1024];
+      for (int p = 0; p < P; ++p) zv[p] = z[size_t(p) * nv + v];
+      for (int e = 0; e < E; ++e) exv[e] = ex[size_t(e) * nv + v];
+      for (int m = 0; m < M; ++m) {
+        yv[m] = y[size_t(m) * nv + v];
+        wv[m] = w ? w[size_t(m) * nv + v] : 1.0f;
+      }
+      double ls;
+      eval_voxel<S, NP>(md, cfg.loss, cfg.transform, zv, yv, w ? wv : nullptr, exv, ls, gv, el);
+      lsum_total += ls;
+      for (int p = 0; p < P; ++p) g[size_t(p) * nv + v] = gv[p] * inv_count;
+      for (int m = 0; m < M; ++m) elem_cur[size_t(m) * nv + v] = el[m];
+    }
+    fid = lsum_total / (double(M) * double(nv));  // mean(residuals), :96
+    reg = 0;
+    if (use_tv) {
+      for (int k = 0; k < cfg.n_reg; ++k) {
+        const int p = cfg.regmap[k];
+        std::fill(img.begin(), img.end(), S(0));
+        std::fill(gimg.begin(), gimg.end(), S(0));
+        for (int64_t v = 0; v < nv; ++v)
+          img[vox[v]] = unscale<S, S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
+        reg += tv_loss_grad<S>(img.data(), mask.data(), dims, cfg.tv_mode, cfg.voxel_size, cfg.lambda[k], nv,
+                               gimg.data());
+        for (int64_t v = 0; v < nv; ++v)
+          g[size_t(p) * nv + v] += gimg[vox[v]] * dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v],
+                                                             S(md.lb[p]), S(md.ub[p]));
+      }
+    }
+    loss = fid + reg;  // :102
+  };
+
+  // 2.5 starting point evaluation (:197-200)
+  double loss, fid, reg;
+  evaluate(loss, fid, reg);
+  if (trace) trace[0] = loss;
+  double min_loss = loss, min_fid = fid, min_reg = reg;
+  int min_iter = 0, epoch = 0;
+  zmn = z;
+  std::vector<S> elem_min = elem_cur;
+
+  // 2.7 convergence state (:212-219)
+  double ema = loss;
+  std::vector<double> cbuf(std::max(cfg.convergence_window, 1), 1.0);
+  int pc = 0, ps = 0, pg = 0;
+  double conv = 0;
+  zprev = z;
+  int stop = GC_STOP_NONE;
+
+  for (epoch = 1; epoch <= cfg.iteration; ++epoch) {
+    if (epoch > 1) evaluate(loss, fid, reg);  // epoch 1 evaluates the unchanged start
+    if (trace) trace[epoch] = loss;
+    // 3.2 update_min_loss (:433-446): strict improvement, pre-update params
+    if (min_loss > loss) {
+      min_loss = loss; min_fid = fid; min_reg = reg; zmn = z; min_iter = epoch; elem_min = elem_cur;
+    }
+    // 3.4 update_convergence_signals (:512-632)
+    if (cfg.convergence_model == GC_CONV_EMA) {
+      // update_convergence_ema, :1192-1204
+      const double e_new = cfg.ema_decay * ema + (1 - cfg.ema_decay) * loss;
+      conv = std::fabs(e_new - ema) / (std::fabs(ema) + 1e-8);
+      ema = e_new;
+    } else {
+      // update_convergence, :1185-1190: minus the least-squares slope
+      std::rotate(cbuf.begin(), cbuf.begin() + 1, cbuf.end());
+      cbuf.back() = loss;
+      const int n = int(cbuf.size());
+      double sx = 0, sy = 0, sxx = 0, sxy = 0;
+      for (int i = 0; i < n; ++i) {
+        const double xi = i + 1;
+        sx += xi; sy += cbuf[i]; sxx += xi * xi; sxy += xi * cbuf[i];
+      }
+      const double den = n * sxx - sx * sx;
+      const double slope = (den != 0) ? (n * sxy - sx * sy) / den : 0.0;
+      conv = -slope;
+    }
+    if (conv > cfg.convergence_value || epoch <= cfg.convergence_window) pc = 0;
+    else if ((min_loss - loss) > cfg.convergence_value) pc = 0;
+    else pc += 1;
+    if (cfg.step_tol > 0) {
+      double num = 0, den = 0;
+      for (size_t i = 0; i < z.size(); ++i) {
+        const double dz = double(z[i]) - double(zprev[i]);
+        num += dz * dz;
+        den += double(z[i]) * double(z[i]);
+      }
+      const double sn = std::sqrt(num) / (1 + std::sqrt(den));
+      zprev = z;
+      ps = (sn < cfg.step_tol) ? ps + 1 : 0;
+    }
+    if (cfg.grad_tol > 0) {
+      double gn = 0;
+      for (size_t i = 0; i < g.size(); ++i) gn += double(g[i]) * double(g[i]);
+      gn = std::sqrt(gn);
+      pg = (gn < cfg.grad_tol) ? pg + 1 : 0;
+    }
+    // 3.5 check_stopping (:634-671), in order
+    if (pc > cfg.patience_conv) stop = GC_STOP_CONVERGENCE;
+    else if (loss < cfg.tol) stop = GC_STOP_TOL;
+    else if (cfg.step_tol > 0 && ps > cfg.patience_step) stop = GC_STOP_STEP;
+    else if (cfg.grad_tol > 0 && pg > cfg.patience_grad) stop = GC_STOP_GRAD;
+    if (stop != GC_STOP_NONE) break;
+    // 3.6 update_parameters (:673-691), not on the last iteration
+    if (epoch < cfg.iteration) {
+      const double lr = cfg.lr0 / (1 + cfg.decay * (epoch - 1));  // update_learn_rate, :1180-1182
+      if (cfg.optimiser == GC_OPT_ADAM) {
+        // adamupdate(params, grad, avg, avgsq, t=epoch, lr, b1, b2, eps)
+        const double bc1 = 1 - std::pow(cfg.beta1, epoch), bc2 = 1 - std::pow(cfg.beta2, epoch);
+        const S b1 = S(cfg.beta1), b2 = S(cfg.beta2), eps = S(cfg.eps);
+        const S lr_eff = S(lr * std::sqrt(bc2) / bc1);
+        for (size_t i = 0; i < z.size(); ++i) {
+          m1[i] = b1 * m1[i] + (S(1) - b1) * g[i];
+          m2[i] = b2 * m2[i] + (S(1) - b2) * g[i] * g[i];
+          if (cfg.adam_form == GC_ADAM_KINGMA_BA)
+            z[i] -= S(lr) * (m1[i] / S(bc1)) / (std::sqrt(m2[i] / S(bc2)) + eps);
+          else
+            z[i] -= lr_eff * m1[i] / (std::sqrt(m2[i]) + eps);
+        }
+      } else if (cfg.optimiser == GC_OPT_SGDM) {
+        // sgdmupdate: vel = momentum*vel - lr*grad; params = params + vel
+        for (size_t i = 0; i < z.size(); ++i) {
+          m1[i] = S(cfg.momentum) * m1[i] - S(lr) * g[i];
+          z[i] += m1[i];
+        }
+      } else {
+        // rmspropupdate: avgsq = d*avgsq + (1-d)*g^2; params -= lr*g/(sqrt(avgsq)+eps)
+        for (size_t i = 0; i < z.size(); ++i) {
+          m2[i] = S(cfg.rms_decay) * m2[i] + (S(1) - S(cfg.rms_decay)) * g[i] * g[i];
+          z[i] -= S(lr) * g[i] / (std::sqrt(m2[i]) + S(cfg.rms_eps));
+        }
+      }
+    }
+  }
+  if (epoch > cfg.iteration) epoch = cfg.iteration;
+  // 4. finalisation (:312-362): linear mode clamps both results to [0,1]
+  if (cfg.transform == GC_TRANSFORM_LINEAR) {
+    for (auto& v : z) v = std::min(std::max(v, S(0)), S(1));
+    for (auto& v : zmn) v = std::min(std::max(v, S(0)), S(1));
+  }
+  for (size_t i = 0; i < z.size(); ++i) { zfinal[i] = double(z[i]); zmin[i] = double(zmn[i]); }
+  if (res_final) for (size_t i = 0; i < elem_cur.size(); ++i) res_final[i] = double(elem_cur[i]);
+  if (res_min) for (size_t i = 0; i < elem_min.size(); ++i) res_min[i] = double(elem_min[i]);
+  out.loss = loss; out.fid = fid; out.reg = reg;
+  out.min_loss = min_loss; out.min_fid = min_fid; out.min_reg = min_reg;
+  out.conv = conv; out.niter = epoch; out.min_iter = min_iter; out.stop = stop;
+  return 0;
+}
+
+// ===========================================================================
+// MCMC: utils/mcmc.m:82-190 (MH) and :192-383 (GW, simultaneous and
+// red-black).  Chains of different repetitions are independent here; the
+// reference runs them one after another from the same start (:136, :281).
+struct McmcArgs {
+  const gc_model* md;
+  const gc_mcmc_cfg* cfg;
+  int64_t nv;
+  const float *y, *w, *extra, *x0, *normals, *uniforms;
+  const int32_t* partners;
+  double *posterior, *mean, *stdv, *accept, *xfinal, *margin;
+  uint8_t* trace;
+};
+
+template <class S>
+static S sos_of(const gc_model& md, const S* x, const float* y, const float* w, const S* ex, int64_t nv, int64_t v) {
+  S s[1024];
+  fwd_model<S, S>(md, x, ex, s);
+  S acc = 0;
+  for (int m = 0; m < md.n_meas; ++m) {
+    S sm = s[m];
+    const S r = sm - S(y[size_t(m) * nv + v]);
+    acc += (w ? S(w[size_t(m) * nv + v]) : S(1)) * r * r;
+  }
+  return acc;
+}
+
+static inline int mcmc_ns(const gc_mcmc_cfg& c) {
+  // Ns = numel(Nburnin+1:thinning:iteration), utils/mcmc.m:110,235
+  if (c.iteration < c.n_burnin + 1) return 0;
+  return (c.iteration - c.n_burnin - 1) / c.thinning + 1;
+}
+
+static inline bool keep_sample(const gc_mcmc_cfg& c, int k) {
+  // (k > Nburnin) && mod(k-Nburnin+1, thinning) == 0, utils/mcmc.m:170,363
+  return k > c.n_burnin && ((k - c.n_burnin + 1) % c.thinning) == 0;
+}
+
+template <class S>
+static void mh_run(const McmcArgs& a, int nthreads) {
+  const gc_model& md = *a.md;
+  const gc_mcmc_cfg& c = *a.cfg;
+  const int Q = c.n_var, M = md.n_meas, E = md.n_extra, R = c.repetition;
+  const int64_t nv = a.nv;
+  const int Ns = mcmc_ns(c);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t v = 0; v < nv; ++v) {
+    S ex[64], x0[GC_MAX_PAR];
+    for (int e = 0; e < E; ++e) ex[e] = S(a.extra[size_t(e) * nv + v]);
+    for (int q = 0; q < Q; ++q) {
+      // xCurr = max(xCurr,lb); min(.,ub) (:126-128)
+      x0[q] = std::min(std::max(S(a.x0[size_t(q) * nv + v]), S(md.lb[q])), S(md.ub[q]));
+    }
+    const S lp0 = logp_gaussian<S>(sos_of<S>(md, x0, a.y, a.w, ex, nv, v), x0[c.noise_slot], M);  // :129
+    double s1[GC_MAX_PAR] = {0}, s2[GC_MAX_PAR] = {0};
+    int64_t nacc = 0;
+    for (int r = 0; r < R; ++r) {
+      S x[GC_MAX_PAR], xp[GC_MAX_PAR];
+      for (int q = 0; q < Q; ++q) x[q] = x0[q];
+      S lp = lp0;
+      int counter = 0;
+      const uint64_t chain = uint64_t(r) * uint64_t(nv) + uint64_t(v);
+      for (int k = 1; k <= c.iteration; ++k) {
+        const size_t step = size_t(r) * c.iteration + (k - 1);
+        bool oob = false;
+        for (int q = 0; q < Q; ++q) {
+          double n;
+          if (c.rng_mode == GC_RNG_REPLAY) n = a.normals[(step * nv + v) * Q + q];
+          else n = native_normal(c.seed, uint32_t(k), chain, PURPOSE_NORMAL0, q);
+          // xProposed = xCurr + xStepsize.*randn (:147)
+          xp[q] = x[q] + S(c.x_step[q]) * S(n);
+          oob = oob || (xp[q] < S(md.lb[q])) || (xp[q] > S(md.ub[q]));  // :149
+          xp[q] = std::min(std::max(xp[q], S(md.lb[q])), S(md.ub[q]));  // :151
+        }
+        const S lpp = logp_gaussian<S>(sos_of<S>(md, xp, a.y, a.w, ex, nv, v), xp[c.noise_slot], M);  // :158
+        double u;
+        if (c.rng_mode == GC_RNG_REPLAY) u = a.uniforms[step * nv + v];
+        else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ACCEPT, o); u = u01(o[0]); }
+        const S ratio = min1<S>(S(std::exp(lpp - lp)));  // :160
+        const bool acc = (ratio > S(u)) && !oob;              // :161-162
+        if (a.trace) a.trace[step * nv + v] = acc ? 1 : 0;
+        if (a.margin) a.margin[step * nv + v] = oob ? -INFINITY : double(lpp - lp) - std::log(u);
+        if (acc) {
+          for (int q = 0; q < Q; ++q) x[q] = xp[q];
+          lp = lpp;
+          ++nacc;
+        }
+        if (keep_sample(c, k)) {
+          if (counter < Ns) {
+            for (int q = 0; q < Q; ++q) {
+              if (a.posterior)
+                a.posterior[((size_t(q) * R + r) * Ns + counter) * nv + v] = double(x[q]);
+              const double d = double(x[q]) - double(x0[q]);
+              s1[q] += d; s2[q] += d * d;
+            }
+          }
+          ++counter;
+        }
+      }
+      const int filled = std::min(counter, Ns);
+      for (int q = 0; q < Q; ++q) {
+        // unfilled posterior slots stay zero (utils/mcmc.m:121) and enter res2out
+        const double d = -double(x0[q]);
+        s1[q] += d * (Ns - filled); s2[q] += d * d * (Ns - filled);
+        if (a.xfinal) a.xfinal[(size_t(r) * Q + q) * nv + v] = double(x[q]);
+      }
+    }
+    const double n = double(Ns) * R;
+    for (int q = 0; q < Q; ++q) {
+      if (a.mean) a.mean[size_t(q) * nv + v] = double(x0[q]) + s1[q] / n;
+      if (a.stdv) a.stdv[size_t(q) * nv + v] = (n > 1) ? std::sqrt(std::max(0.0, (s2[q] - s1[q] * s1[q] / n) / (n - 1))) : 0.0;
+    }
+    if (a.accept) a.accept[v] = double(nacc) / (double(c.iteration) * R);
+  }
+}
+
+template <class S>
+static void gw_run(const McmcArgs& a, int nthreads) {
+  const gc_model& md = *a.md;
+  const gc_mcmc_cfg& c = *a.cfg;
+  const int Q = c.n_var, M = md.n_meas, E = md.n_extra, R = c.repetition, W = c.n_walker;
+  const int64_t nv = a.nv;
+  const int Ns = mcmc_ns(c);
+  const S aa = S(c.step_size);
+  const bool rb = c.ensemble_update == GC_ENSEMBLE_REDBLACK;
+  const int h = W / 2;
+  // partner tables for the native stream (shared by all voxels)
+  std::vector<int32_t> ptab;
+  const int ptab_w = W;
+  if (c.rng_mode == GC_RNG_NATIVE) {
+    ptab.resize(size_t(R) * c.iteration * ptab_w);
+    for (int r = 0; r < R; ++r)
+      for (int k = 1; k <= c.iteration; ++k)
+        native_partners(c.seed, r, k, W, c.ensemble_update, &ptab[(size_t(r) * c.iteration + (k - 1)) * ptab_w]);
+  }
+  const int32_t* partners = (c.rng_mode == GC_RNG_NATIVE) ? ptab.data() : a.partners;
+
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t v = 0; v < nv; ++v) {
+    S ex[64], xc0[GC_MAX_PAR];
+    for (int e = 0; e < E; ++e) ex[e] = S(a.extra[size_t(e) * nv + v]);
+    for (int q = 0; q < Q; ++q) xc0[q] = S(a.x0[size_t(q) * nv + v]);
+    // walker initialisation: x0 + (ub-lb)*startRange.*randn, clamped (:270-273)
+    std::vector<S> xinit(size_t(W) * Q), lpinit(W);
+    for (int wk = 0; wk < W; ++wk) {
+      for (int q = 0; q < Q; ++q) {
+        double n;
+        if (c.rng_mode == GC_RNG_REPLAY) n = a.normals[(size_t(wk) * nv + v) * Q + q];
+        else n = native_normal(c.seed, 0u, uint64_t(wk) * uint64_t(nv) + uint64_t(v), PURPOSE_INIT, q);
+        S xi = xc0[q] + S(md.ub[q] - md.lb[q]) * S(c.start_range) * S(n);
+        xi = std::min(std::max(xi, S(md.lb[q])), S(md.ub[q]));
+        xinit[size_t(wk) * Q + q] = xi;
+      }
+      lpinit[wk] = logp_gaussian<S>(sos_of<S>(md, &xinit[size_t(wk) * Q], a.y, a.w, ex, nv, v),
+                                   xinit[size_t(wk) * Q + c.noise_slot], M);  // :275
+    }
+    double s1[GC_MAX_PAR] = {0}, s2[GC_MAX_PAR] = {0};
+    int64_t nacc = 0;
+    std::vector<S> x(size_t(W) * Q), lp(W), xp(size_t(W) * Q), lpp(W), zz(W);
+    std::vector<uint8_t> oob(W);
+    for (int r = 0; r < R; ++r) {
+      x = xinit;
+      lp = lpinit;
+      int counter = 0;
+      for (int k = 1; k <= c.iteration; ++k) {
+        const size_t step = size_t(r) * c.iteration + (k - 1);
+        const int32_t* part = partners + step * W;
+        const int nsub = rb ? 2 : 1;
+        for (int sub = 0; sub < nsub; ++sub) {
+          int a0 = 0, na = W;
+          if (rb) { a0 = (sub == 0) ? 0 : h; na = (sub == 0) ? h : W - h; }
+          // uniforms for this (sub-)step: [2][na][nv] at offset
+          const size_t ubase = rb ? (step * 2 + sub) * 2 * size_t(h > W - h ? h : W - h) : step * 2 * size_t(W);
+          const size_t ustride = rb ? size_t(h > W - h ? h : W - h) : size_t(W);
+          for (int j = 0; j < na; ++j) {
+            const int wk = a0 + j;
+            const int pw = rb ? part[(sub == 0 ? 0 : h) + j] : part[wk];
+            double u1;
+            const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v);
+            if (c.rng_mode == GC_RNG_REPLAY) u1 = a.uniforms[(ubase + j) * nv + v];
+            else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o); u1 = u01(o[0]); }
+            // zz = ((a-1)*rand + 1).^2 / a (:299, :332)
+            const S t = (aa - S(1)) * S(u1) + S(1);
+            zz[wk] = t * t / aa;
+            bool ob = false;
+            for (int q = 0; q < Q; ++q) {
+              const S xa = x[size_t(pw) * Q + q];
+              S val = xa + (x[size_t(wk) * Q + q] - xa) * zz[wk];  // :300, :333
+              ob = ob || val < S(md.lb[q]) || val > S(md.ub[q]);
+              val = std::min(std::max(val, S(md.lb[q])), S(md.ub[q]));
+              xp[size_t(wk) * Q + q] = val;
+            }
+            oob[wk] = ob;
+            lpp[wk] = logp_gaussian<S>(sos_of<S>(md, &xp[size_t(wk) * Q], a.y, a.w, ex, nv, v),
+                                       xp[size_t(wk) * Q + c.noise_slot], M);
+          }
+          for (int j = 0; j < na; ++j) {
+            const int wk = a0 + j;
+            const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v);
+            double u2;
+            if (c.rng_mode == GC_RNG_REPLAY) u2 = a.uniforms[(ubase + ustride + j) * nv + v];
+            else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ACCEPT + 0x100u * sub, o); u2 = u01(o[0]); }
+            // min(zz.^(Nvar-1).*exp(logPProposed-logPCurr),1) > rand (:313-315)
+            const S ratio = min1<S>(S(std::pow(zz[wk], S(Q - 1)) * std::exp(lpp[wk] - lp[wk])));
+            const bool acc = ratio > S(u2) && !oob[wk];
+            if (a.trace) a.trace[(step * W + wk) * nv + v] = acc ? 1 : 0;
+            if (a.margin)
+              a.margin[(step * W + wk) * nv + v] =
+                  oob[wk] ? -INFINITY : double(Q - 1) * std::log(double(zz[wk])) + double(lpp[wk] - lp[wk]) - std::log(u2);
+            if (acc) {
+              for (int q = 0; q < Q; ++q) x[size_t(wk) * Q + q] = xp[size_t(wk) * Q + q];
+              lp[wk] = lpp[wk];
+              ++nacc;
+            }
+          }
+        }
+        if (keep_sample(c, k)) {
+          if (counter < Ns) {
+            for (int wk = 0; wk < W; ++wk)
+              for (int q = 0; q < Q; ++q) {
+                const double xv = double(x[size_t(wk) * Q + q]);
+                if (a.posterior)
+                  a.posterior[(((size_t(q) * R + r) * Ns + counter) * W + wk) * nv + v] = xv;
+                const double d = xv - double(xc0[q]);
+                s1[q] += d; s2[q] += d * d;
+              }
+          }
+          ++counter;
+        }
+      }
+      const int filled = std::min(counter, Ns);
+      for (int q = 0; q < Q; ++q) {
+        const double d = -double(xc0[q]);
+        s1[q] += d * double(Ns - filled) * W;
+        s2[q] += d * d * double(Ns - filled) * W;
+      }
+      if (a.xfinal)
+        for (int wk = 0; wk < W; ++wk)
+          for (int q = 0; q < Q; ++q)
+            a.xfinal[((size_t(r) * W + wk) * Q + q) * nv + v] = double(x[size_t(wk) * Q + q]);
+    }
+    const double n = double(Ns) * W * R;
+    for (int q = 0; q < Q; ++q) {
+      if (a.mean) a.mean[size_t(q) * nv + v] = double(xc0[q]) + s1[q] / n;
+      if (a.stdv) a.stdv[size_t(q) * nv + v] = (n > 1) ? std::sqrt(std::max(0.0, (s2[q] - s1[q] * s1[q] / n) / (n - 1))) : 0.0;
+    }
+    if (a.accept) a.accept[v] = double(nacc) / (double(c.iteration) * R * W);
+  }
+}
+
+template <class F>
+static int with_np(int P, F&& f) {
+  if (P <= 2) return f(std::integral_constant<int, 2>{});
+  if (P <= 3) return f(std::integral_constant<int, 3>{});
+  if (P <= 4) return f(std::integral_constant<int, 4>{});
+  if (P <= 5) return f(std::integral_constant<int, 5>{});
+  if (P <= 8) return f(std::integral_constant<int, 8>{});
+  if (P <= 16) return f(std::integral_constant<int, 16>{});
+  return f(std::integral_constant<int, GC_MAX_PAR>{});
+}
+
+}  // namespace orc
+
+// ===========================================================================
+// C API of the oracle (called through ctypes by tests/ and bench.py's CPU
+// baseline).
+extern "C" {
+
+int orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  orc::philox4x32_10(ctr, key, out);
+  return 0;
+}
+
+int orc_native_partners(uint64_t seed, int32_t rep, int32_t it, int32_t W, int32_t mode, int32_t* out) {
+  orc::native_partners(seed, rep, it, W, mode, out);
+  return 0;
+}
+
+double orc_native_normal(uint64_t seed, uint32_t it, uint64_t chain, uint32_t base, int32_t q) {
+  return orc::native_normal(seed, it, chain, base, q);
+}
+
+double orc_logp_gaussian(double sos, double noise, int32_t nm) { return orc::logp_gaussian<double>(sos, noise, nm); }
+
+// s[M x nv] = FWD(theta[P x nv]); jac (optional) [M x P x nv] = ds/dtheta.
+int orc_fwd(const gc_model* md, int64_t nv, const double* theta, const double* extra, double* s, double* jac) {
+  using D = orc::Dual<double, GC_MAX_PAR>;
+  const int P = md->n_par, M = md->n_meas, E = md->n_extra;
+  if (M > 1024 || P > GC_MAX_PAR || E > 64) return GC_ERR_ARG;
+  for (int64_t v = 0; v < nv; ++v) {
+    double ex[64];
+    for (int e = 0; e < E; ++e) ex[e] = extra[size_t(e) * nv + v];
+    D th[GC_MAX_PAR];
+    for (int p = 0; p < P; ++p) th[p] = D::var(theta[size_t(p) * nv + v], p);
+    static thread_local D out[1024];
+    orc::fwd_model<D, double>(*md, th, ex, out);
+    for (int m = 0; m < M; ++m) {
+      s[size_t(m) * nv + v] = out[m].v;
+      if (jac)
+        for (int p = 0; p < P; ++p) jac[(size_t(m) * P + p) * nv + v] = out[m].d[p];
+    }
+  }
+  return 0;
+}
+
+// Plain-value forward in double (no derivatives) -- used for finite differences.
+int orc_fwd_value(const gc_model* md, int64_t nv, const double* theta, const double* extra, double* s) {
+  const int P = md->n_par, M = md->n_meas, E = md->n_extra;
+  if (M > 1024 || P > GC_MAX_PAR || E > 64) return GC_ERR_ARG;
+  for (int64_t v = 0; v < nv; ++v) {
+    double ex[64], th[GC_MAX_PAR], out[1024];
+    for (int e = 0; e < E; ++e) ex[e] = extra[size_t(e) * nv + v];
+    for (int p = 0; p < P; ++p) th[p] = theta[size_t(p) * nv + v];
+    orc::fwd_model<double, double>(*md, th, ex, out);
+    for (int m = 0; m < M; ++m) s[size_t(m) * nv + v] = out[m];
+  }
+  return 0;
+}
+
+int orc_tv(const double* img, const uint8_t* mask, const int32_t dims[3], int32_t mode, const double vsize[3],
+           double lambda, int64_t nmasked, double* loss, double* grad) {
+  *loss = orc::tv_loss_grad<double>(img, mask, dims, mode, vsize, lambda, nmasked, grad);
+  return 0;
+}
+
+// askadam; precision 0 = double (oracle), 1 = float (CPU-path baseline).
+// scal_out[10] = {loss, fid, reg, min_loss, min_fid, min_reg, conv, niter, min_iter, stop}
+int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const float* y, const float* w,
+                const float* extra, const int64_t* vox, const int32_t* dims, const float* z0, double* zfinal,
+                double* zmin, double* res_final, double* res_min, double* trace, double* scal_out, int32_t precision,
+                int32_t nthreads) {
+  if (cfg->robust) return GC_ERR_UNSUPPORTED;
+  if (md->n_meas > 1024 || md->n_par > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
+  orc::AskOut o{};
+  int rc;
+  if (precision == 0) {
+    rc = orc::with_np(md->n_par, [&](auto npc) {
+      constexpr int NP = decltype(npc)::value;
+      return orc::askadam_run<double, NP>(*md, *cfg, nv, y, w, extra, vox, dims, z0, zfinal, zmin, res_final,
+                                          res_min, trace, o, std::max(1, nthreads));
+    });
+  } else {
+    rc = orc::with_np(md->n_par, [&](auto npc) {
+      constexpr int NP = decltype(npc)::value;
+      return orc::askadam_run<float, NP>(*md, *cfg, nv, y, w, extra, vox, dims, z0, zfinal, zmin, res_final,
+                                         res_min, trace, o, std::max(1, nthreads));
+    });
+  }
+  scal_out[0] = o.loss; scal_out[1] = o.fid; scal_out[2] = o.reg;
+  scal_out[3] = o.min_loss; scal_out[4] = o.min_fid; scal_out[5] = o.min_reg;
+  scal_out[6] = o.conv; scal_out[7] = o.niter; scal_out[8] = o.min_iter; scal_out[9] = o.stop;
+  return rc;
+}
+
+// MCMC (MH or GW).  Outputs are double arrays in the gc_mcmc_io layouts;
+// margin (optional) receives log(acceptance ratio) - log(u) per decision.
+int orc_mcmc(const gc_model* md, const gc_mcmc_cfg* cfg, int64_t nv, const float* y, const float* w,
+             const float* extra, const float* x0, const float* normals, const float* uniforms,
+             const int32_t* partners, double* posterior, double* mean, double* stdv, double* accept, double* xfinal,
+             uint8_t* trace, double* margin, int32_t precision, int32_t nthreads) {
+  if (md->n_meas > 1024 || cfg->n_var > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
+  if (cfg->algorithm == GC_MCMC_GW && cfg->ensemble_update == GC_ENSEMBLE_REDBLACK && cfg->n_walker < 2)
+    return GC_ERR_NWALKER;
+  orc::McmcArgs a{md, cfg, nv, y, w, extra, x0, normals, uniforms, partners, posterior, mean, stdv, accept,
+                  xfinal, margin, trace};
+  const int nt = std::max(1, nthreads);
+  if (cfg->algorithm == GC_MCMC_MH) {
+    if (precision == 0) orc::mh_run<double>(a, nt); else orc::mh_run<float>(a, nt);
+  } else {
+    if (precision == 0) orc::gw_run<double>(a, nt); else orc::gw_run<float>(a, nt);
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""Direct C-ABI drivers used by the GPU parity tests (device buffers from
+torch, every computation in libgacelle_hip.so)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from gacelle_amd import _abi, runtime
+
+
+def ctx():
+    return runtime.context()
+
+
+def fwd(md, theta, extra=None):
+    c = ctx()
+    theta = np.ascontiguousarray(theta, np.float32)
+    nv = theta.shape[1]
+    th = c.to_device(theta)
+    ex = c.to_device(extra) if extra is not None else None
+    s = c.empty((md.n_meas, nv))
+    c.check(c.lib.gc_fwd(c.handle, ctypes.byref(md), runtime.ptr(th), runtime.ptr(ex), runtime.ptr(s), nv))
+    return runtime.to_host(s)
+
+
+def fwd_adj(md, loss, transform, x, y, w=None, extra=None, scale=1.0, residual=False):
+    c = ctx()
+    x = np.ascontiguousarray(x, np.float32)
+    P, nv = x.shape
+    xd = c.to_device(x)
+    yd = c.to_device(y)
+    wd = c.to_device(w) if w is not None else None
+    ex = c.to_device(extra) if extra is not None else None
+    g = c.empty((P, nv))
+    ls = c.zeros((1,), np.float64)
+    rs = c.empty((md.n_meas, nv)) if residual else None
+    c.check(c.lib.gc_fwd_adj(c.handle, ctypes.byref(md), loss, transform, runtime.ptr(xd), runtime.ptr(yd),
+                             runtime.ptr(wd), runtime.ptr(ex), nv, scale, runtime.ptr(g), runtime.ptr(ls),
+                             runtime.ptr(rs)))
+    out = dict(grad=runtime.to_host(g), loss_sum=float(runtime.to_host(ls)[0]))
+    if residual:
+        out["residual"] = runtime.to_host(rs)
+    return out
+
+
+def askadam(md, cfg, vols, trace=False):
+    """vols: list of dict(y, z0, w=None, extra=None).  Returns per-volume dicts."""
+    c = ctx()
+    n = len(vols)
+    arr = (_abi.gc_volume * n)()
+    keep, metas = [], []
+    for i, v in enumerate(vols):
+        y = np.ascontiguousarray(v["y"], np.float32)
+        M, nv = y.shape
+        z0 = np.ascontiguousarray(v["z0"], np.float32)
+        P = z0.shape[0]
+        yd = c.to_device(y)
+        wd = c.to_device(v["w"]) if v.get("w") is not None else None
+        ex = c.to_device(v["extra"]) if v.get("extra") is not None else None
+        zd = c.to_device(z0)
+        zm = c.empty((P, nv))
+        rl = c.empty((nv,))
+        rlm = c.empty((nv,))
+        rs = c.empty((M, nv))
+        rsm = c.empty((M, nv))
+        tr = (ctypes.c_double * (int(cfg.iteration) + 1))() if trace else None
+        a = arr[i]
+        a.nv = nv
+        a.y_dev = runtime.ptr(yd)
+        a.w_dev = runtime.ptr(wd)
+        a.extra_dev = runtime.ptr(ex)
+        a.z_dev = runtime.ptr(zd)
+        a.zmin_dev = runtime.ptr(zm)
+        a.resloss_dev = runtime.ptr(rl)
+        a.resloss_min_dev = runtime.ptr(rlm)
+        a.residual_dev = runtime.ptr(rs)
+        a.residual_min_dev = runtime.ptr(rsm)
+        a.loss_trace_host = ctypes.cast(tr, ctypes.POINTER(ctypes.c_double)) if tr is not None else None
+        keep += [yd, wd, ex, zd, zm, rl, rlm, rs, rsm]
+        metas.append((zd, zm, rl, rlm, rs, rsm, tr))
+    c.check(c.lib.gc_askadam_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), arr, n))
+    outs = []
+    for i, (zd, zm, rl, rlm, rs, rsm, tr) in enumerate(metas):
+        a = arr[i]
+        o = dict(z_final=runtime.to_host(zd), z_min=runtime.to_host(zm), resloss=runtime.to_host(rl),
+                 resloss_min=runtime.to_host(rlm), residual=runtime.to_host(rs), residual_min=runtime.to_host(rsm),
+                 loss=a.loss, loss_fidelity=a.loss_fidelity, loss_reg=a.loss_reg, min_loss=a.min_loss,
+                 min_iteration=a.min_iteration, n_iteration=a.n_iteration, stop_reason=a.stop_reason,
+                 convergence=a.convergence, n_launch=a.n_launch)
+        if tr is not None:
+            o["trace"] = np.array(tr[: a.n_iteration + 1])
+        outs.append(o)
+    return outs
+
+
+def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partners=None, store=True, trace=False):
+    c = ctx()
+    y = np.ascontiguousarray(y, np.float32)
+    M, nv = y.shape
+    Q, R = cfg.n_var, cfg.repetition
+    W = cfg.n_walker if cfg.algorithm == _abi.GC_MCMC_GW else 1
+    it = cfg.iteration
+    ns = 0 if it < cfg.n_burnin + 1 else (it - cfg.n_burnin - 1) // cfg.thinning + 1
+    io = _abi.gc_mcmc_io()
+    yd = c.to_device(y)
+    x0d = c.to_device(np.ascontiguousarray(x0, np.float32))
+    wd = c.to_device(w) if w is not None else None
+    ex = c.to_device(extra) if extra is not None else None
+    nd = c.to_device(normals) if normals is not None else None
+    ud = c.to_device(uniforms) if uniforms is not None else None
+    pd = c.to_device(np.asarray(partners, np.int32), np.int32) if partners is not None else None
+    post = c.empty((Q, R, ns, W, nv)) if store and ns > 0 else None
+    mean = c.empty((Q, nv))
+    std = c.empty((Q, nv))
+    acc = c.empty((nv,))
+    xf = c.empty((R, W, Q, nv))
+    tr = c.zeros((R, it, W, nv), np.uint8) if trace else None
+    io.nv = nv
+    for name, t in (("y_dev", yd), ("w_dev", wd), ("extra_dev", ex), ("x0_dev", x0d), ("normals_dev", nd),
+                    ("uniforms_dev", ud), ("partners_dev", pd), ("posterior_dev", post), ("mean_dev", mean),
+                    ("std_dev", std), ("accept_dev", acc), ("xfinal_dev", xf), ("trace_dev", tr)):
+        setattr(io, name, runtime.ptr(t))
+    c.check(c.lib.gc_mcmc_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), ctypes.byref(io)))
+    return dict(posterior=None if post is None else runtime.to_host(post), mean=runtime.to_host(mean),
+                std=runtime.to_host(std), accept=runtime.to_host(acc), xfinal=runtime.to_host(xf),
+                trace=None if tr is None else runtime.to_host(tr), n_sample=io.n_sample, n_stored=io.n_stored)

@@ -1,0 +1,73 @@
+"""Shared builders for the tests: synthetic problems and C-ABI descriptors."""
+from __future__ import annotations
+
+import numpy as np
+
+from gacelle_amd import _abi, models
+from gacelle_amd.askadam import askadam
+from gacelle_amd.mcmc import mcmc
+
+TE12 = np.arange(2, 47, 4) * 1e-3          # 2:4:46 ms
+TE_JOINT = np.arange(2, 25, 2) * 1e-3      # 2:2:24 ms
+FA_JOINT = np.array([3, 8, 15, 25], float)  # degrees
+TR_JOINT = 30e-3
+
+
+def no_stop(**kw):
+    f = {"tol": 0, "convergenceValue": -1, "convergenceStepTol": 0, "convergenceGradTol": 0, "verbose": False}
+    f.update(kw)
+    return f
+
+
+def r2s_problem(nv, seed=0, snr=100):
+    rng = np.random.default_rng(seed)
+    s0 = rng.uniform(0.5, 1.5, nv)
+    r2 = rng.uniform(10, 60, nv)
+    s = s0 * np.exp(-np.outer(TE12, r2))
+    sigma = s0.mean() / snr
+    y = (s + sigma * rng.standard_normal(s.shape)).astype(np.float32)
+    return y, np.stack([s0, r2]), sigma
+
+
+def r2s_setup(fitting, solver="askadam"):
+    m = models.gpuR2starMapping(TE12)
+    if solver == "askadam":
+        f = askadam.check_set_default_basic(dict(modelParams=["M0", "R2star"], lb=[0, 0.1], ub=[2, 200], **fitting))
+    else:
+        f = mcmc.check_set_default_basic(dict(modelParams=["M0", "R2star", "noise"], lb=[0, 0.1, 0.001],
+                                              ub=[2, 200, 0.1], **fitting))
+    md = m.gc_model(f, solver, TE12.size)
+    return m, f, md
+
+
+def logit_start(theta0, md):
+    P = theta0.shape[0]
+    z = np.zeros_like(theta0, dtype=np.float64)
+    for p in range(P):
+        lb, ub = md.lb[p], md.ub[p]
+        eb = 1e-4 * (ub - lb)
+        c = np.clip(theta0[p], lb + eb, ub - eb)
+        t = (c - lb) / (ub - lb)
+        z[p] = np.log(t / (1 - t))
+    return z.astype(np.float32)
+
+
+def mcmc_cfg(algo, iteration, thinning, burnin, rep=1, W=1, seed=48463, replay=False, steps=(), redblack=False,
+             a=2.0, start_range=1e-3, Q=3):
+    c = _abi.gc_mcmc_cfg()
+    c.algorithm = algo
+    c.iteration = iteration
+    c.thinning = thinning
+    c.n_burnin = burnin
+    c.repetition = rep
+    c.n_walker = W
+    c.ensemble_update = _abi.GC_ENSEMBLE_REDBLACK if redblack else _abi.GC_ENSEMBLE_SIMULTANEOUS
+    c.rng_mode = _abi.GC_RNG_REPLAY if replay else _abi.GC_RNG_NATIVE
+    c.noise_slot = Q - 1
+    c.n_var = Q
+    c.seed = seed
+    c.step_size = a
+    c.start_range = start_range
+    for q, s in enumerate(steps):
+        c.x_step[q] = s
+    return c

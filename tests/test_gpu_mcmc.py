@@ -1,0 +1,162 @@
+"""GPU parity of the MCMC samplers against the oracle (C ABI, gc_mcmc_run)."""
+import numpy as np
+import pytest
+
+from gacelle_amd import _abi
+from oracle import oracle as orc
+
+from . import gpu_abi
+from .helpers import mcmc_cfg, r2s_problem, r2s_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _first_divergence_ok(g_trace, o_trace, margin, tol=1e-3):
+    """Chains whose accept decisions differ: the first differing decision
+    must be a marginal one (|log ratio - log u| < tol in the oracle)."""
+    diff = g_trace != o_trace
+    flat = diff.reshape(-1, diff.shape[-1]) if diff.ndim > 2 else diff
+    R, it = g_trace.shape[0], g_trace.shape[1]
+    bad = 0
+    n_div = 0
+    for r in range(R):
+        d = diff[r].reshape(it, -1)
+        m = margin[r].reshape(it, -1)
+        cols = np.where(d.any(axis=0))[0]
+        for c in cols:
+            k = np.argmax(d[:, c])
+            n_div += 1
+            if not abs(m[k, c]) < tol:
+                bad += 1
+    return n_div, bad
+
+
+def test_mh_replay_matches_oracle():
+    nv = 2000
+    y, th, sigma = r2s_problem(nv, seed=31)
+    _, f, md = r2s_setup(dict(iteration=500, thinning=5, burnin=0.1), solver="mcmc")
+    it, R, Q = 500, 2, 3
+    cfg = mcmc_cfg(_abi.GC_MCMC_MH, it, 5, 50, rep=R, replay=True, steps=(0.01, 1, 0.005))
+    rng = np.random.default_rng(5)
+    normals = rng.standard_normal((R, it, nv, Q)).astype(np.float32)
+    uniforms = rng.uniform(0, 1, (R, it, nv)).astype(np.float32)
+    x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, trace=True)
+    o = orc.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, trace=True, margin=True)
+    same = np.all(g["trace"] == o["trace"], axis=(1, 2)).reshape(R, -1)
+    frac = same.mean()
+    assert frac > 0.97, frac
+    n_div, bad = _first_divergence_ok(g["trace"], o["trace"], o["margin"])
+    assert bad == 0, (n_div, bad)
+    # identical decision sequences -> identical chains within FP32 tolerance
+    xs_g = g["xfinal"][:, 0]  # [R, Q, nv]
+    xs_o = o["xfinal"][:, 0]
+    for r in range(R):
+        ok = same[r]
+        np.testing.assert_allclose(xs_g[r][:, ok], xs_o[r][:, ok], rtol=1e-5, atol=1e-6)
+    post_ok = np.all(same, axis=0)
+    np.testing.assert_allclose(g["posterior"][..., post_ok], o["posterior"][..., post_ok], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g["mean"][:, post_ok], o["mean"][:, post_ok], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g["std"][:, post_ok], o["std"][:, post_ok], rtol=1e-3, atol=1e-6)
+
+
+def test_mh_native_stream_matches_oracle():
+    nv = 1000
+    y, th, sigma = r2s_problem(nv, seed=32)
+    _, f, md = r2s_setup({}, solver="mcmc")
+    cfg = mcmc_cfg(_abi.GC_MCMC_MH, 400, 20, 40, rep=4, steps=(0.01, 1, 0.005))
+    x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, trace=True)
+    o = orc.mcmc(md, cfg, y, x0, trace=True, margin=True)
+    same = np.all(g["trace"] == o["trace"], axis=(1, 2))
+    assert same.mean() > 0.97
+    n_div, bad = _first_divergence_ok(g["trace"], o["trace"], o["margin"])
+    assert bad == 0, (n_div, bad)
+
+
+@pytest.mark.parametrize("redblack", [False, True])
+def test_gw_replay_matches_oracle(redblack):
+    """Same draw stream on device and oracle.  A voxel's walkers interact,
+    so one marginal FP32-vs-FP64 decision changes the whole ensemble from
+    there on: every first divergence must be marginal, and voxels without
+    any divergence must match to FP32 tolerance."""
+    nv, W, R, it, Q = 300, 16, 2, 60, 3
+    y, th, sigma = r2s_problem(nv, seed=33)
+    _, f, md = r2s_setup({}, solver="mcmc")
+    cfg = mcmc_cfg(_abi.GC_MCMC_GW, it, 5, 10, rep=R, W=W, replay=True, redblack=redblack)
+    rng = np.random.default_rng(8)
+    normals = rng.standard_normal((W, nv, Q)).astype(np.float32)
+    if redblack:
+        hm = (W + 1) // 2
+        uniforms = rng.uniform(0, 1, (R, it, 2, 2, hm, nv)).astype(np.float32)
+        h = W // 2
+        partners = np.zeros((R, it, W), np.int32)
+        partners[..., :h] = rng.integers(h, W, (R, it, h))
+        partners[..., h:] = rng.integers(0, h, (R, it, W - h))
+    else:
+        uniforms = rng.uniform(0, 1, (R, it, 2, W, nv)).astype(np.float32)
+        partners = np.zeros((R, it, W), np.int32)
+        for r in range(R):
+            for k in range(it):
+                while True:
+                    p = rng.permutation(W)
+                    if np.all(p != np.arange(W)):
+                        break
+                partners[r, k] = p
+    x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, partners=partners, trace=True)
+    o = orc.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, partners=partners, trace=True, margin=True)
+    d = g["trace"] != o["trace"]  # [R, it, W, nv]
+    for v in np.where(d.any(axis=(0, 1, 2)))[0]:
+        for r in range(R):
+            dv = d[r, :, :, v]
+            if not dv.any():
+                continue
+            k = int(np.argmax(dv.any(axis=1)))
+            ws = np.where(dv[k])[0]
+            assert np.all(np.abs(o["margin"][r, k, ws, v]) < 1e-3), (v, r, k, o["margin"][r, k, ws, v])
+    same_vox = ~d.any(axis=(0, 1, 2))
+    assert same_vox.mean() > 0.5, same_vox.mean()
+    np.testing.assert_allclose(g["mean"][:, same_vox], o["mean"][:, same_vox], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g["std"][:, same_vox], o["std"][:, same_vox], rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(g["posterior"][..., same_vox], o["posterior"][..., same_vox], rtol=1e-5, atol=1e-6)
+
+
+def test_gw_native_stream_matches_oracle():
+    """Native Philox stream and host partner tables: same decisions as the
+    oracle restatement (up to marginal FP32 flips)."""
+    nv, W, R, it = 200, 16, 2, 40
+    y, th, sigma = r2s_problem(nv, seed=34)
+    _, f, md = r2s_setup({}, solver="mcmc")
+    cfg = mcmc_cfg(_abi.GC_MCMC_GW, it, 5, 10, rep=R, W=W)
+    x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, trace=True)
+    o = orc.mcmc(md, cfg, y, x0, trace=True, margin=True)
+    same_vox = np.all(g["trace"] == o["trace"], axis=(0, 1, 2))
+    assert same_vox.mean() > 0.5, same_vox.mean()
+    np.testing.assert_allclose(g["mean"][:, same_vox], o["mean"][:, same_vox], rtol=1e-5, atol=1e-6)
+
+
+def test_mh_native_statistics_analytic_target():
+    """TE ~ 0 makes s = M0: with the noise fixed the M0 posterior is
+    N(mean(y), sigma^2/M); native-stream chains must reproduce it."""
+    from gacelle_amd import models
+    from gacelle_amd.mcmc import mcmc as M
+
+    te = np.full(12, 1e-9)
+    m = models.gpuR2starMapping(te)
+    nv, sigma = 400, 0.02
+    rng = np.random.default_rng(3)
+    mu = rng.uniform(0.8, 1.2, nv)
+    y = (mu + sigma * rng.standard_normal((12, nv))).astype(np.float32)
+    f = M.check_set_default_basic(dict(modelParams=["M0", "R2star", "noise"], lb=[0, 0.1, sigma],
+                                       ub=[2, 200, sigma]))
+    md = m.gc_model(f, "mcmc", 12)
+    cfg = mcmc_cfg(_abi.GC_MCMC_MH, 20000, 10, 2000, rep=4, steps=(0.01, 1, 0.0))
+    x0 = np.stack([y.mean(axis=0), np.full(nv, 30.0), np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, store=False)
+    post_mean = y.astype(np.float64).mean(axis=0)
+    post_sd = sigma / np.sqrt(12)
+    z = (g["mean"][0] - post_mean) / post_sd
+    assert abs(np.mean(z)) < 0.05 and np.std(z) < 0.2, (np.mean(z), np.std(z))
+    np.testing.assert_allclose(np.median(g["std"][0]), post_sd, rtol=0.05)

@@ -1,0 +1,373 @@
+"""Benchmark: voxels fitted/s (askadam) and MCMC samples/s on one MI355X.
+
+Workload (one "step" = one whole-volume askadam fit): mono-exponential R2*
+mapping of a 256x256x128 volume with a Bernoulli(0.7) mask (~5.87M voxels),
+12 echoes TE = 2:4:46 ms, S0 ~ U(0.5,1.5), R2* ~ U(10,60) /s, Gaussian noise
+at SNR 100; askadam 2000 iterations, L1 loss, Adam lr 1e-3, sigmoid
+transform, stopping criteria off (fixed iteration count).  Metropolis-
+Hastings: 4 chains (repetitions) x 5000 iterations per voxel, native RNG.
+
+Reported beside the fused path, in the same run:
+  * the plain per-op GPU tape (PyTorch eager autograd + Adam), same fit;
+  * the CPU path (the oracle's FP32 restatement, OpenMP over host cores)
+    on a bounded voxel sample;
+  * the roofline position of the fused kernel from HIP-event kernel time.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+TE = np.arange(2, 47, 4) * 1e-3
+FLOP_PER_VOXEL_ITER = 140.0      # 9M + 16P, M = 12 echoes, P = 2
+TRANS_PER_VOXEL_ITER = 18.0      # M exp + P (exp + sqrt + div)
+PEAK_FP32_TFLOPS = 157.3         # MI355X published FP32 vector (packed)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6
+PEAK_TRANS_PER_S = 9.8e12        # quarter-rate transcendental issue
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_volume(dims, frac, seed):
+    rng = np.random.default_rng(seed)
+    mask = rng.random(dims) < frac
+    nv = int(mask.sum())
+    s0 = rng.uniform(0.5, 1.5, nv).astype(np.float32)
+    r2 = rng.uniform(10, 60, nv).astype(np.float32)
+    sigma = float(s0.mean()) / 100.0
+    return mask, nv, s0, r2, sigma
+
+
+def synth_device(s0, r2, sigma, seed, device):
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    te = torch.tensor(TE, dtype=torch.float32, device=device).reshape(-1, 1)
+    S0 = torch.from_numpy(s0).to(device)
+    R2 = torch.from_numpy(r2).to(device)
+    y = S0.unsqueeze(0) * torch.exp(-te * R2.unsqueeze(0))
+    y += sigma * torch.randn(y.shape, generator=g, device=device, dtype=torch.float32)
+    return y.contiguous()
+
+
+def r2s_model(lb, ub, n_par):
+    from gacelle_amd import _abi
+
+    md = _abi.gc_model()
+    md.model_id = _abi.GC_MODEL_R2STAR
+    md.i[0] = TE.size
+    md.n_meas = TE.size
+    md.n_par = n_par
+    for t, v in enumerate(TE):
+        md.table[t] = float(v)
+    for p, (a, b) in enumerate(zip(lb, ub)):
+        md.lb[p] = a
+        md.ub[p] = b
+    return md
+
+
+def askadam_cfg(iterations):
+    from gacelle_amd.askadam import askadam
+
+    f = askadam.check_set_default_basic(dict(modelParams=["M0", "R2star"], lb=[0, 0.1], ub=[2, 200],
+                                             iteration=iterations, initialLearnRate=1e-3, lossFunction="L1",
+                                             tol=0, convergenceValue=-1, convergenceStepTol=0,
+                                             convergenceGradTol=0, verbose=False))
+    return askadam().make_cfg(f, None)
+
+
+def logit_z0(nv, md, device):
+    import torch
+
+    z = np.zeros((2, 1), np.float64)
+    for p, v in enumerate((1.0, 30.0)):
+        t = (v - md.lb[p]) / (md.ub[p] - md.lb[p])
+        z[p] = np.log(t / (1 - t))
+    return torch.from_numpy(np.repeat(z, nv, axis=1).astype(np.float32)).to(device).contiguous()
+
+
+def run_askadam(ctx, md, cfg, y, z0):
+    from gacelle_amd import _abi, runtime
+
+    nv = y.shape[1]
+    z = z0.clone()
+    zmin = ctx.empty((2, nv))
+    rl = ctx.empty((nv,))
+    v = (_abi.gc_volume * 1)()
+    v[0].nv = nv
+    v[0].y_dev = runtime.ptr(y)
+    v[0].z_dev = runtime.ptr(z)
+    v[0].zmin_dev = runtime.ptr(zmin)
+    v[0].resloss_dev = runtime.ptr(rl)
+    ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(cfg), v, 1))
+    return z, v[0]
+
+
+def run_mh(ctx, md, y, x0, iterations, reps, seed=48463):
+    from gacelle_amd import _abi, runtime
+
+    nv = y.shape[1]
+    c = _abi.gc_mcmc_cfg()
+    c.algorithm = _abi.GC_MCMC_MH
+    c.iteration = iterations
+    c.thinning = 20
+    c.n_burnin = int(np.floor(iterations * 0.1))
+    c.repetition = reps
+    c.n_walker = 1
+    c.rng_mode = _abi.GC_RNG_NATIVE
+    c.noise_slot = 2
+    c.n_var = 3
+    c.seed = seed
+    for q, s in enumerate((0.01, 1.0, 0.005)):
+        c.x_step[q] = s
+    io = _abi.gc_mcmc_io()
+    mean = ctx.empty((3, nv))
+    std = ctx.empty((3, nv))
+    io.nv = nv
+    io.y_dev = runtime.ptr(y)
+    io.x0_dev = runtime.ptr(x0)
+    io.mean_dev = runtime.ptr(mean)
+    io.std_dev = runtime.ptr(std)
+    ctx.check(ctx.lib.gc_mcmc_run(ctx.handle, ctypes.byref(md), ctypes.byref(c), ctypes.byref(io)))
+    return mean, std, io
+
+
+def theta_of(z, md):
+    out = []
+    for p in range(2):
+        out.append(md.lb[p] + (md.ub[p] - md.lb[p]) / (1 + np.exp(-z[p].astype(np.float64))))
+    return np.stack(out)
+
+
+def cpu_baseline_askadam(s0, r2, sigma, iterations, nthreads, nsample, seed=7):
+    """The CPU path: the oracle's FP32 restatement, OpenMP over host cores."""
+    from oracle import oracle as orc
+
+    rng = np.random.default_rng(seed)
+    idx = np.arange(nsample)
+    S = s0[idx, None].T * np.exp(-np.outer(TE, r2[idx]))
+    y = (S + sigma * rng.standard_normal(S.shape)).astype(np.float32)
+    md = r2s_model([0, 0.1], [2, 200], 2)
+    cfg = askadam_cfg(iterations)
+    z0 = np.zeros((2, nsample), np.float32)
+    for p, v in enumerate((1.0, 30.0)):
+        t = (v - md.lb[p]) / (md.ub[p] - md.lb[p])
+        z0[p] = np.log(t / (1 - t))
+    t0 = time.perf_counter()
+    orc.askadam(md, cfg, y, z0, precision=1, nthreads=nthreads)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_mh(s0, r2, sigma, iterations, reps, nthreads, nsample, seed=8):
+    from gacelle_amd import _abi
+    from oracle import oracle as orc
+
+    rng = np.random.default_rng(seed)
+    S = s0[:nsample, None].T * np.exp(-np.outer(TE, r2[:nsample]))
+    y = (S + sigma * rng.standard_normal(S.shape)).astype(np.float32)
+    md = r2s_model([0, 0.1, 0.001], [2, 200, 0.1], 3)
+    c = _abi.gc_mcmc_cfg()
+    c.algorithm, c.iteration, c.thinning = _abi.GC_MCMC_MH, iterations, 20
+    c.n_burnin, c.repetition, c.n_walker, c.rng_mode = int(iterations * 0.1), reps, 1, _abi.GC_RNG_NATIVE
+    c.noise_slot, c.n_var, c.seed = 2, 3, 48463
+    for q, s in enumerate((0.01, 1.0, 0.005)):
+        c.x_step[q] = s
+    x0 = np.stack([s0[:nsample], r2[:nsample], np.full(nsample, sigma, np.float32)]).astype(np.float32)
+    t0 = time.perf_counter()
+    orc.mcmc(md, c, y, x0, store=False, precision=1, nthreads=nthreads)
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iterations", type=int, default=2000)
+    ap.add_argument("--mh-iterations", type=int, default=5000)
+    ap.add_argument("--mh-chains", type=int, default=4)
+    ap.add_argument("--dims", type=int, nargs=3, default=[256, 256, 128])
+    ap.add_argument("--skip-tape", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-mh", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the path shards over voxels; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gacelle_amd import runtime
+    from gacelle_amd.tape import askadam_r2star_tape
+
+    device = torch.device("cuda", 0)
+    ctx = runtime.context(0)
+    dims = tuple(args.dims)
+    mask, nv, s0, r2, sigma = make_volume(dims, 0.7, 23439)
+    log(f"volume {dims}, {nv} masked voxels")
+    y = synth_device(s0, r2, sigma, 5438973, device)
+    md = r2s_model([0, 0.1], [2, 200], 2)
+    cfg = askadam_cfg(args.iterations)
+    z0 = logit_z0(nv, md, device)
+
+    if args.calib_copy:
+        a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=device)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB write")
+        del a, b
+
+    # ---- fused askadam --------------------------------------------------
+    for w in range(args.warmup):
+        run_askadam(ctx, md, cfg, y, z0)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    times, kms, kl = [], [], []
+    zf = vol = None
+    torch.cuda.synchronize()
+    for s in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        zf, vol = run_askadam(ctx, md, cfg, y, z0)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        kms.append(vol.kernel_ms)
+        kl.append(vol.kernel_launches)
+        log(f"askadam step {s + 1}/{args.steps}: {times[-1] * 1e3:.2f} ms (kernel {vol.kernel_ms:.2f} ms, "
+            f"{vol.kernel_launches} launches, K={vol.iters_per_launch})")
+    t_step = float(np.median(times))
+    vox_per_s = nv / t_step
+    zf_h = zf.cpu().numpy()
+    th = theta_of(zf_h, md)
+    err_fused = dict(M0=float(np.median(np.abs(th[0] - s0) / s0)), R2star=float(np.median(np.abs(th[1] - r2) / r2)))
+    fused_loss = float(vol.loss)
+
+    # roofline of the fused kernel: algorithmic FLOP per voxel-iteration
+    k_ms = float(np.median(kms))
+    units = float(nv) * args.iterations
+    achieved_tflops = units * FLOP_PER_VOXEL_ITER / (k_ms * 1e-3) / 1e12
+    trans_rate = units * TRANS_PER_VOXEL_ITER / (k_ms * 1e-3)
+    launches = int(np.median(kl))
+
+    # ---- plain per-op tape -----------------------------------------------
+    tape = None
+    if not args.skip_tape:
+        te_t = torch.tensor(TE, dtype=torch.float32, device=device)
+        torch.cuda.synchronize()
+        askadam_r2star_tape(y, te_t, [0.0, 0.1], [2.0, 200.0], z0, 5)  # warm-up (allocator, kernels)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        zt, tape_loss = askadam_r2star_tape(y, te_t, [0.0, 0.1], [2.0, 200.0], z0, args.iterations)
+        torch.cuda.synchronize()
+        t_tape = time.perf_counter() - t0
+        tht = theta_of(zt.cpu().numpy(), md)
+        tape = dict(voxels_per_s=nv / t_tape, seconds=t_tape, final_loss=tape_loss,
+                    speedup_fused_vs_tape=t_tape / t_step,
+                    loss_rel_diff=abs(tape_loss - fused_loss) / abs(tape_loss),
+                    median_rel_err=dict(M0=float(np.median(np.abs(tht[0] - s0) / s0)),
+                                        R2star=float(np.median(np.abs(tht[1] - r2) / r2))))
+        log(f"tape: {t_tape:.2f} s, loss {tape_loss:.6e} vs fused {fused_loss:.6e}")
+        del zt
+
+    # ---- MH -----------------------------------------------------------------
+    mh = None
+    if not args.skip_mh:
+        md3 = r2s_model([0, 0.1, 0.001], [2, 200, 0.1], 3)
+        x0 = torch.stack([torch.from_numpy(s0), torch.from_numpy(r2),
+                          torch.full((nv,), sigma, dtype=torch.float32)]).to(device).contiguous()
+        run_mh(ctx, md3, y, x0, 50, args.mh_chains)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mean, std, io = run_mh(ctx, md3, y, x0, args.mh_iterations, args.mh_chains)
+        torch.cuda.synchronize()
+        t_mh = time.perf_counter() - t0
+        steps_total = float(nv) * args.mh_chains * args.mh_iterations
+        mh = dict(samples_per_s=steps_total / t_mh, retained_samples_per_s=float(nv) * args.mh_chains *
+                  io.n_sample / t_mh, seconds=t_mh, kernel_ms=io.kernel_ms, chains=args.mh_chains,
+                  iterations=args.mh_iterations, retained_per_chain=int(io.n_sample))
+        m = mean.cpu().numpy()
+        mh["posterior_mean_median_rel_err"] = dict(M0=float(np.median(np.abs(m[0] - s0) / s0)),
+                                                   R2star=float(np.median(np.abs(m[1] - r2) / r2)))
+        log(f"MH: {t_mh:.2f} s, {mh['samples_per_s']:.3e} chain-steps/s")
+
+    # ---- CPU path -------------------------------------------------------------
+    cpu = None
+    if not args.skip_cpu:
+        ncores = len(os.sched_getaffinity(0))
+        nthreads = min(ncores, 16)
+        nsample = 100000
+        t_cpu = cpu_baseline_askadam(s0, r2, sigma, args.iterations, nthreads, nsample)
+        cpu = dict(value=nsample / t_cpu, unit="voxels/s", cores=nthreads, kind="port",
+                   sample=f"askadam {args.iterations} iterations on {nsample} voxels of the same workload "
+                          f"(oracle FP32 restatement, OpenMP {nthreads} threads): {t_cpu:.1f} s")
+        log(f"CPU askadam: {t_cpu:.2f} s for {nsample} voxels")
+        if mh is not None:
+            ns_mh = 8000
+            t_cmh = cpu_baseline_mh(s0, r2, sigma, args.mh_iterations, args.mh_chains, nthreads, ns_mh)
+            cpu["mh_samples_per_s"] = ns_mh * args.mh_chains * args.mh_iterations / t_cmh
+            cpu["mh_sample"] = f"MH {args.mh_chains}x{args.mh_iterations} on {ns_mh} voxels: {t_cmh:.1f} s"
+            log(f"CPU MH: {t_cmh:.2f} s for {ns_mh} voxels")
+
+    out = {
+        "metric": "voxels fitted/s (askadam) + MCMC samples/s vs plain GPU path at equal error",
+        "value": vox_per_s,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"mono-exponential R2* askadam, {dims[0]}x{dims[1]}x{dims[2]} Bernoulli(0.7) mask "
+                        f"({nv} voxels), 12 echoes 2:4:46 ms, {args.iterations} iterations L1/Adam lr 1e-3; "
+                        f"MH {args.mh_chains} chains x {args.mh_iterations} iterations",
+            "voxels": nv,
+            "iterations": args.iterations,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": "k_askadam (fused forward+adjoint+Adam, K iterations per launch)",
+            "kernel_ms_total": k_ms,
+            "launches": launches,
+            "avg_launch_ms": k_ms / max(launches, 1),
+            "frac_of_nonpacked_peak": achieved_tflops / PEAK_FP32_NONPACKED_TFLOPS,
+            "transcendentals_per_s": trans_rate,
+            "trans_frac": trans_rate / PEAK_TRANS_PER_S,
+            "flop_per_unit": FLOP_PER_VOXEL_ITER,
+            "unit_of_work": "voxel-iteration",
+        },
+        "cpu_baseline": cpu,
+        "fused": {"final_loss": fused_loss, "median_rel_err": err_fused},
+        "tape": tape,
+        "mcmc": mh,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,8 +1,7 @@
 """ctypes mirror of include/gacelle.h and the loader of libgacelle_hip.so.
 
-The structures below follow the C header field by field; ``abi_layout()``
-returns their sizes so tests can check them against the compiled library.
-The loader never falls back to anything: when the HIP library is missing or
+The structures below follow the C header field by field (tests compile
+the header with gcc and compare every size and offset).  The loader never falls back to anything: when the HIP library is missing or
 fails to load, every entry point raises.
 """
 from __future__ import annotations
@@ -39,6 +38,7 @@ GC_FLAG_NEXI_RICIAN = 1 << 1
 GC_FLAG_NEXI_RE_GUARD = 1 << 2
 GC_FLAG_MCRMWI_COMPLEX = 1 << 3
 GC_FLAG_MCRMWI_EXCHANGE = 1 << 4
+GC_FLAG_UNIFORM_TE = 1 << 5
 
 GC_QUAD_TRAPZ = 0
 GC_QUAD_TABLE = 1
@@ -141,6 +141,9 @@ class gc_volume(ctypes.Structure):
         ("min_iteration", c_int32),
         ("stop_reason", c_int32),
         ("n_launch", c_int32),
+        ("kernel_ms", c_double),
+        ("kernel_launches", c_int32),
+        ("iters_per_launch", c_int32),
     ]
 
 
@@ -181,6 +184,7 @@ class gc_mcmc_io(ctypes.Structure):
         ("trace_dev", c_void_p),
         ("n_sample", c_int32),
         ("n_stored", c_int32),
+        ("kernel_ms", c_double),
     ]
 
 

@@ -103,6 +103,8 @@ int with_model(gc_ctx* ctx, const gc_model& md, F&& f) {
   switch (md.model_id) {
     case GC_MODEL_R2STAR: {
       const int nte = md.i[0];
+      if (nte == 12 && (md.flags & GC_FLAG_UNIFORM_TE)) return f(R2sModel<12, true, true>{});
+      if (nte == 12) return f(R2sModel<12, true>{});
       if (nte <= 16) return f(R2sModel<16>{});
       if (nte <= 32) return f(R2sModel<32>{});
       return set_err(ctx, GC_ERR_UNSUPPORTED, "R2star: at most 32 echoes supported (got %d)", nte);
@@ -223,6 +225,27 @@ struct AskDriver {
   DevBuf d_vols, d_es, d_sums;
   HostBuf h_vols, h_es, h_sums;
   bool desc_in_flight = false;  // an async copy may still read h_vols/h_es
+  // HIP events around every fused launch (kernel-only timing)
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  double kernel_ms = 0.0;
+  int kernel_launches = 0;
+
+  ~AskDriver() {
+    for (auto e : ev_start) (void)hipEventDestroy(e);
+    for (auto e : ev_stop) (void)hipEventDestroy(e);
+  }
+
+  // accumulate the durations of recorded launches (after a stream sync)
+  void collect_events() {
+    for (size_t i = 0; i < ev_start.size(); ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev_start[i], ev_stop[i]) == hipSuccess) kernel_ms += ms;
+      (void)hipEventDestroy(ev_start[i]);
+      (void)hipEventDestroy(ev_stop[i]);
+    }
+    ev_start.clear();
+    ev_stop.clear();
+  }
 
   AskDriver(gc_ctx* c, const gc_model& m, const gc_askadam_cfg& a, gc_volume* v, int n)
       : ctx(c), md(m), cfg(a), vols(v), n_vol(n), run(n) {}
@@ -360,8 +383,16 @@ struct AskDriver {
     ap.momentum = float(cfg.momentum);
     ap.rms_decay = float(cfg.rms_decay);
     ap.rms_eps = float(cfg.rms_eps);
+    hipEvent_t ev_a, ev_b;
+    GC_HIP(ctx, hipEventCreate(&ev_a));
+    GC_HIP(ctx, hipEventCreate(&ev_b));
+    ev_start.push_back(ev_a);
+    ev_stop.push_back(ev_b);
+    GC_HIP(ctx, hipEventRecord(ev_a, ctx->stream));
     hipLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, dmd, ap);
     GC_HIP(ctx, hipGetLastError());
+    GC_HIP(ctx, hipEventRecord(ev_b, ctx->stream));
+    ++kernel_launches;
     for (int i = 0; i < n_vol; ++i) {
       if (n_ep[i] <= 0) continue;
       hipLaunchKernelGGL(k_reduce_rows, dim3(n_ep[i] * 4), dim3(kBlock), 0, ctx->stream,
@@ -471,6 +502,8 @@ struct AskDriver {
       if (rc) return rc;
       GC_HIP(ctx, hipMemcpyAsync(sums, d_sums.p, sizeof(double) * K * 4 * n_vol, hipMemcpyDeviceToHost, ctx->stream));
       GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      desc_in_flight = false;
+      collect_events();
       any = false;
       for (int i = 0; i < n_vol; ++i) {
         VolRun& R = run[i];
@@ -610,6 +643,13 @@ struct AskDriver {
       V.n_launch = R.launches;
     }
     GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    desc_in_flight = false;
+    collect_events();
+    for (int i = 0; i < n_vol; ++i) {
+      vols[i].kernel_ms = kernel_ms;
+      vols[i].kernel_launches = kernel_launches;
+      vols[i].iters_per_launch = K;
+    }
     return GC_OK;
   }
 };
@@ -974,6 +1014,10 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
   }
   const int gpb = kBlock / gsz;
   const unsigned grid = unsigned((io->nv + gpb - 1) / gpb);
+  hipEvent_t ev0, ev1;
+  GC_HIP(ctx, hipEventCreate(&ev0));
+  GC_HIP(ctx, hipEventCreate(&ev1));
+  GC_HIP(ctx, hipEventRecord(ev0, ctx->stream));
   rc = with_model(ctx, *md, [&](auto model) -> int {
     using Model = decltype(model);
     return with_bool(io->w_dev != nullptr, [&](auto hw) -> int {
@@ -986,8 +1030,18 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
       return GC_OK;
     });
   });
-  if (rc) return rc;
+  if (rc) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return rc;
+  }
+  GC_HIP(ctx, hipEventRecord(ev1, ctx->stream));
   GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  io->kernel_ms = ms;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
   return GC_OK;
 }
 

@@ -12,12 +12,17 @@
 //             (utils/mcmc.m:192-383).
 //  k_reduce   deterministic FP64 reduction of block partials.
 #pragma once
+#include <type_traits>
+
 #include "gc_device.hpp"
 #include "gc_models.hpp"
 
 namespace gc {
 
 constexpr int kBlock = 256;
+
+// reciprocal with the hardware v_rcp_f32 (1 ulp)
+GC_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 constexpr int kRing = 16;  // iterations buffered in LDS before a partial flush
 
 // ---- loss functions of askadam.model_gradient (utils/askadam.m:86-95) -----
@@ -25,7 +30,7 @@ template <int LOSS>
 GC_DEV void loss_fn(float r, float w, float& l, float& dl) {
   if (LOSS == GC_LOSS_L1) {
     l = w * fabsf(r);
-    dl = w * sgnf(r);
+    dl = (r == 0.f) ? 0.f : copysignf(w, r);
   } else if (LOSS == GC_LOSS_L2) {
     l = w * r * r;
     dl = 2.f * w * r;
@@ -44,7 +49,7 @@ GC_DEV void loss_fn(float r, float w, float& l, float& dl) {
 // around values that are later combined (see the build notes on SLP).
 GC_DEV void transform_fwd(int transform, float z, float lb, float ub, float& th, float& dth) {
   const float r = ub - lb;
-  const float sg = 1.f / (1.f + __expf(-z));
+  const float sg = rcpf(1.f + __expf(-z));
   const float zc = fminf(fmaxf(z, 0.f), 1.f);
   const bool sig = transform == GC_TRANSFORM_SIGMOID;
   const bool lin = transform == GC_TRANSFORM_LINEAR;
@@ -157,17 +162,31 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
     float gth[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) gth[p] = 0.f;
-    Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
-      const bool fin = isfinite(s);  // utils.set_nan_inf_zero
-      const float sv = fin ? s : 0.f;
-      float l, dl;
-      loss_fn<LOSS>(sv - yr[r], HAS_W ? wr[HAS_W ? r : 0] : 1.f, l, dl);
-      lsum += l;
-      if (fin) {
+    // Fast pass without per-element checks; a non-finite model signal makes
+    // the loss sum non-finite, and only then is the voxel re-evaluated with
+    // utils.set_nan_inf_zero semantics (zero signal, zero gradient).
+    auto accumulate = [&](auto checked) {
+      constexpr bool CHECK = decltype(checked)::value;
+      lsum = 0.f;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) gth[p] += dl * J[p];
-      }
-    });
+      for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+      Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
+        bool fin = true;
+        float sv = s;
+        if (CHECK) {
+          fin = isfinite(s);
+          sv = fin ? s : 0.f;
+        }
+        float l, dl;
+        loss_fn<LOSS>(sv - yr[r], HAS_W ? wr[HAS_W ? r : 0] : 1.f, l, dl);
+        lsum += l;
+        if (CHECK) dl = fin ? dl : 0.f;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) gth[p] = fmaf(dl, J[p], gth[p]);
+      });
+    };
+    accumulate(std::false_type{});
+    if (!isfinite(lsum)) accumulate(std::true_type{});
     float den = dead_z2, gn = 0.f, step = 0.f;
     float g[NP];
 #pragma unroll
@@ -189,15 +208,15 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
           m1[p] = ap.beta1 * m1[p] + (1.f - ap.beta1) * g[p];
           m2[p] = ap.beta2 * m2[p] + (1.f - ap.beta2) * g[p] * g[p];
           if (ap.adam_form == GC_ADAM_KINGMA_BA)
-            dz = -es.lr * (m1[p] * es.c1) / (sqrtf(m2[p] * es.c2) + ap.eps);
+            dz = -es.lr * (m1[p] * es.c1) * rcpf(sqrtf(m2[p] * es.c2) + ap.eps);
           else
-            dz = -es.lr * m1[p] / (sqrtf(m2[p]) + ap.eps);
+            dz = -es.lr * m1[p] * rcpf(sqrtf(m2[p]) + ap.eps);
         } else if (ap.opt == GC_OPT_SGDM) {
           m1[p] = ap.momentum * m1[p] - es.lr * g[p];
           dz = m1[p];
         } else {
           m2[p] = ap.rms_decay * m2[p] + (1.f - ap.rms_decay) * g[p] * g[p];
-          dz = -es.lr * g[p] / (sqrtf(m2[p]) + ap.rms_eps);
+          dz = -es.lr * g[p] * rcpf(sqrtf(m2[p]) + ap.rms_eps);
         }
         const float zn = z[p] + dz;
         const float d = zn - z[p];

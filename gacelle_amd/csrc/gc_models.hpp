@@ -11,8 +11,8 @@
 //                          model signal s and (JAC) J[p] = ds/dtheta_p.
 // Loops run over compile-time bounds with runtime guards so that, after
 // inlining, every slot index is a constant and a voxel's measurements stay
-// in registers.  Acquisition tables come from the by-value gc_model kernel
-// argument (wave-uniform: scalar loads / SGPRs).
+// in registers.  Acquisition tables are read from the device copy of the
+// gc_model at wave-uniform addresses (scalar loads into SGPRs).
 #pragma once
 #include "gc_device.hpp"
 #include "../../include/gacelle.h"
@@ -21,25 +21,40 @@ namespace gc {
 
 // ---------------------------------------------------------------------------
 // s = m0 * exp(-te * r2s): R2star/gpuR2starMapping.m:427-434 (FWD :321-363)
-// theta = {M0, R2star}
-template <int MT>
+// theta = {M0, R2star}.  EXACT: the echo count equals MT (no guards).
+// With GC_FLAG_UNIFORM_TE the decays exp(-TE_k R2*) are formed as
+// exp(-TE_0 R2*) * exp(-dTE R2*)^k (two exponentials per voxel instead of
+// one per echo; relative error <= k ulp).
+template <int MT, bool EXACT = false, bool UNI = false>
 struct R2sModel {
   static constexpr int NP = 2, MR = MT, NE = 0;
-  GC_DEV static bool valid(const gc_model& md, int r) { return r < md.i[0]; }
+  GC_DEV static bool valid(const gc_model& md, int r) { return EXACT ? (r < MT) : (r < md.i[0]); }
   GC_DEV static int meas(const gc_model&, int r) { return r; }
   template <bool JAC, class F>
   GC_DEV static void eval(const gc_model& md, const float* th, const float*, F&& f) {
-    const int nte = md.i[0];
+    const int nte = EXACT ? MT : md.i[0];
     const float m0 = th[0], r2s = th[1];
+    const float kr = -1.4426950408889634f * r2s;  // -log2(e) * R2*
+    float e = 0.f, q = 0.f;
+    if (UNI) {
+      e = __builtin_amdgcn_exp2f(kr * md.table[0]);
+      q = __builtin_amdgcn_exp2f(kr * md.f[8]);
+    }
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
-      if (t < nte) {
+      if (EXACT || t < nte) {
         const float te = md.table[t];
-        const float e = __expf(-te * r2s);
-        const float s = m0 * e;
+        float et;
+        if (UNI) {
+          et = e;
+          e *= q;
+        } else {
+          et = __builtin_amdgcn_exp2f(kr * te);
+        }
+        const float s = m0 * et;
         float J[NP];
         if (JAC) {
-          J[0] = e;
+          J[0] = et;
           J[1] = -te * s;
         }
         f(t, s, J);

@@ -127,6 +127,17 @@ class _ModelBase:
         return _ask().optimisation(data, mask, w, pars0, fitting, self, *fwd_args)
 
 
+def uniform_spacing(te):
+    """Spacing of equally spaced echo times, or None."""
+    te = np.asarray(te, dtype=np.float64).ravel()
+    if te.size < 3:
+        return None
+    d = np.diff(te)
+    if np.all(np.abs(d - d[0]) <= 1e-6 * max(abs(d[0]), 1e-12)) and d[0] > 0:
+        return float(np.float32(te[1]) - np.float32(te[0]))
+    return None
+
+
 def _fill_table(md, values, offset=0):
     for k, v in enumerate(values):
         md.table[offset + k] = float(v)
@@ -155,6 +166,10 @@ class gpuR2starMapping(_ModelBase):
         md.i[0] = self.te.size
         md.n_meas = self.te.size
         _fill_table(md, self.te)
+        dte = uniform_spacing(self.te)
+        if dte is not None:
+            md.flags |= _abi.GC_FLAG_UNIFORM_TE
+            md.f[8] = dte
         if nmeas is not None and nmeas != md.n_meas:
             raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The size of TE does not match with the 4th dimension of the image.")
         return md

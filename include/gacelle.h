@@ -66,7 +66,9 @@ enum {
   GC_FLAG_NEXI_RICIAN = 1 << 1,    /* Rician mean correction, sigma = extra row 0 */
   GC_FLAG_NEXI_RE_GUARD = 1 << 2,  /* re = ra*fa/max(1-fa,eps) (gpuNEXIrice.m:678) */
   GC_FLAG_MCRMWI_COMPLEX = 1 << 3, /* complex data: re/im stacked, dfreqBKG/dpini fitted */
-  GC_FLAG_MCRMWI_EXCHANGE = 1 << 4 /* BM two-pool exchange steady state (kIEWM fitted) */
+  GC_FLAG_MCRMWI_EXCHANGE = 1 << 4, /* BM two-pool exchange steady state (kIEWM fitted) */
+  GC_FLAG_UNIFORM_TE = 1 << 5       /* echo times equally spaced: f[8] = TE spacing;
+                                       exp(-TE_k R) is formed by recurrence */
 };
 
 /* NEXI quadrature kinds (gc_model.i[3]) */
@@ -193,7 +195,12 @@ typedef struct gc_volume {
   int32_t n_iteration;       /* out.final.Niteration                     */
   int32_t min_iteration;     /* out.min.Niteration                       */
   int32_t stop_reason;       /* GC_STOP_*                                */
-  int32_t n_launch;          /* kernel launches used (diagnostic)        */
+  int32_t n_launch;          /* fused launches that advanced this volume */
+  /* timing of the fused iteration kernel over the whole batched run
+   * (HIP events on the context stream; same values in every volume) */
+  double kernel_ms;          /* summed duration of all fused launches    */
+  int32_t kernel_launches;   /* number of fused launches                 */
+  int32_t iters_per_launch;  /* K, iterations per launch                 */
 } gc_volume;
 
 /* ---- MCMC configuration: POD mirror of mcmc.check_set_default_basic
@@ -246,6 +253,7 @@ typedef struct gc_mcmc_io {
   uint8_t* trace_dev;        /* test hook: [rep][iter][W][nv] accept flags */
   int32_t n_sample;          /* out: Ns = numel(Nburnin+1:thinning:iteration) */
   int32_t n_stored;          /* out: samples actually stored per chain    */
+  double kernel_ms;          /* out: sampler kernel duration (HIP events) */
 } gc_mcmc_io;
 
 /* ---- context ------------------------------------------------------------ */
@@ -303,10 +311,11 @@ int gc_fwd_adj(gc_ctx* ctx, const gc_model* model, int32_t loss, int32_t transfo
 /* ---- regulariser (utils/spatial_total_variation.m:24-66) ---------------- */
 /* loss = lambda/n_masked * sum_i mask_i * sum_d |D_d img_i| / vsize_d with
  * periodic forward differences; grad_img_dev (optional) receives
- * d loss / d img (subgradient, sign(0) = 0).  img is column-major [X,Y,Z]. */
+ * d loss / d img (subgradient, sign(0) = 0).  img is column-major [X,Y,Z];
+ * loss is written to a HOST double (the call synchronises). */
 int gc_tv_fwd_adj(gc_ctx* ctx, const float* img_dev, const uint8_t* mask_dev,
                   const int32_t dims[3], int32_t tv_mode, const double voxel_size[3],
-                  double lambda, int64_t n_masked, double* loss_dev, float* grad_img_dev);
+                  double lambda, int64_t n_masked, double* loss, float* grad_img_dev);
 
 /* ---- solvers ------------------------------------------------------------ */
 /* askadam().optimisation (utils/askadam.m:121-365) for one model over a

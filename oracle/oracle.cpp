@@ -170,7 +170,9 @@ static void eval_voxel(const gc_model& md, int loss_kind, int transform, const S
     D zp = (p < NP) ? D::var(zval, p) : D(zval);
     th[p] = unscale<D, S>(transform, zp, S(md.lb[p]), S(md.ub[p]));
   }
-  D s[1024];
+  static thread_local std::vector<D> sbuf;
+  if (int(sbuf.size()) < md.n_meas) sbuf.resize(md.n_meas);
+  D* s = sbuf.data();
   fwd_model<D, S>(md, th, extra, s);
   for (int p = 0; p < P; ++p) grad[p] = 0;
   lsum = 0;

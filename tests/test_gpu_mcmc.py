@@ -78,12 +78,14 @@ def test_mh_native_stream_matches_oracle():
 def test_gw_replay_matches_oracle(redblack):
     """Same draw stream on device and oracle.  A voxel's walkers interact,
     so one marginal FP32-vs-FP64 decision changes the whole ensemble from
-    there on: every first divergence must be marginal, and voxels without
-    any divergence must match to FP32 tolerance."""
-    nv, W, R, it, Q = 300, 16, 2, 60, 3
+    there on, and stretch moves amplify rounding differences over many
+    iterations: over a 20-iteration horizon every first divergence must be
+    marginal, and voxels without any divergence must match to FP32
+    tolerance.  (Long horizons are compared statistically.)"""
+    nv, W, R, it, Q = 300, 16, 2, 20, 3
     y, th, sigma = r2s_problem(nv, seed=33)
     _, f, md = r2s_setup({}, solver="mcmc")
-    cfg = mcmc_cfg(_abi.GC_MCMC_GW, it, 5, 10, rep=R, W=W, replay=True, redblack=redblack)
+    cfg = mcmc_cfg(_abi.GC_MCMC_GW, it, 2, 4, rep=R, W=W, replay=True, redblack=redblack)
     rng = np.random.default_rng(8)
     normals = rng.standard_normal((W, nv, Q)).astype(np.float32)
     if redblack:
@@ -135,6 +137,31 @@ def test_gw_native_stream_matches_oracle():
     same_vox = np.all(g["trace"] == o["trace"], axis=(0, 1, 2))
     assert same_vox.mean() > 0.5, same_vox.mean()
     np.testing.assert_allclose(g["mean"][:, same_vox], o["mean"][:, same_vox], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("algo", ["mh", "gw"])
+def test_long_run_posterior_statistics_match_oracle(algo):
+    """Long native-stream runs: device and FP64 oracle posteriors agree
+    statistically (per-voxel mean differences are within the Monte-Carlo
+    error and unbiased; standard deviations agree)."""
+    nv = 200
+    y, th, sigma = r2s_problem(nv, seed=35)
+    _, f, md = r2s_setup({}, solver="mcmc")
+    if algo == "mh":
+        cfg = mcmc_cfg(_abi.GC_MCMC_MH, 4000, 10, 400, rep=4, steps=(0.01, 1, 0.005))
+        n_eff = 4 * 360 / 10.0
+    else:
+        cfg = mcmc_cfg(_abi.GC_MCMC_GW, 1000, 10, 100, rep=1, W=16)
+        n_eff = 16 * 90 / 5.0
+    x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, store=False)
+    o = orc.mcmc(md, cfg, y, x0, store=False)
+    for q in range(2):
+        se = np.sqrt(g["std"][q] ** 2 + o["std"][q] ** 2) / np.sqrt(n_eff)
+        z = (g["mean"][q] - o["mean"][q]) / se
+        assert abs(np.median(z)) < 0.5 and np.mean(np.abs(z) < 4) > 0.95, (q, np.median(z), np.mean(np.abs(z) < 4))
+        ratio = g["std"][q] / o["std"][q]
+        assert 0.8 < np.median(ratio) < 1.25, (q, np.median(ratio))
 
 
 def test_mh_native_statistics_analytic_target():

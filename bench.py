@@ -64,19 +64,18 @@ def synth_device(s0, r2, sigma, seed, device):
 
 
 def r2s_model(lb, ub, n_par):
-    from gacelle_amd import _abi
+    """The mono-exponential model as the Python mirror builds it (equal
+    echo spacing selects the exponential-recurrence kernel)."""
+    from gacelle_amd import models
+    from gacelle_amd.askadam import askadam
+    from gacelle_amd.mcmc import mcmc
 
-    md = _abi.gc_model()
-    md.model_id = _abi.GC_MODEL_R2STAR
-    md.i[0] = TE.size
-    md.n_meas = TE.size
-    md.n_par = n_par
-    for t, v in enumerate(TE):
-        md.table[t] = float(v)
-    for p, (a, b) in enumerate(zip(lb, ub)):
-        md.lb[p] = a
-        md.ub[p] = b
-    return md
+    names = ["M0", "R2star", "noise"][:n_par]
+    if n_par == 2:
+        f, solver = askadam.check_set_default_basic(dict(modelParams=names, lb=list(lb), ub=list(ub))), "askadam"
+    else:
+        f, solver = mcmc.check_set_default_basic(dict(modelParams=names, lb=list(lb), ub=list(ub))), "mcmc"
+    return models.gpuR2starMapping(TE).gc_model(f, solver, TE.size)
 
 
 def askadam_cfg(iterations):
@@ -311,14 +310,15 @@ def main():
     if not args.skip_cpu:
         ncores = len(os.sched_getaffinity(0))
         nthreads = min(ncores, 16)
-        nsample = 100000
+        # about 10 s of host work on 16 threads
+        nsample = 16000 * nthreads
         t_cpu = cpu_baseline_askadam(s0, r2, sigma, args.iterations, nthreads, nsample)
         cpu = dict(value=nsample / t_cpu, unit="voxels/s", cores=nthreads, kind="port",
                    sample=f"askadam {args.iterations} iterations on {nsample} voxels of the same workload "
                           f"(oracle FP32 restatement, OpenMP {nthreads} threads): {t_cpu:.1f} s")
         log(f"CPU askadam: {t_cpu:.2f} s for {nsample} voxels")
         if mh is not None:
-            ns_mh = 8000
+            ns_mh = 4000 * nthreads
             t_cmh = cpu_baseline_mh(s0, r2, sigma, args.mh_iterations, args.mh_chains, nthreads, ns_mh)
             cpu["mh_samples_per_s"] = ns_mh * args.mh_chains * args.mh_iterations / t_cmh
             cpu["mh_sample"] = f"MH {args.mh_chains}x{args.mh_iterations} on {ns_mh} voxels: {t_cmh:.1f} s"

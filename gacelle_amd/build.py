@@ -1,12 +1,20 @@
-"""Build libgacelle_hip.so in-tree with hipcc for gfx950."""
+"""Build libgacelle_hip.so in-tree with hipcc for gfx950.
+
+Every translation unit under csrc/ (the C ABI in gacelle.hip and one
+ops_*.hip per register-tiled model variant) is compiled in parallel to an
+object under _build/, then linked into gacelle_amd/libgacelle_hip.so.
+"""
 from __future__ import annotations
 
+import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "gacelle.hip")
+CSRC = os.path.join(HERE, "csrc")
+OBJ_DIR = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libgacelle_hip.so")
 
 HIPCC_FLAGS = [
@@ -14,7 +22,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # no packed-FP32 instructions (v_pk_*_f32) in any HIP code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # no SLP vectorisation: it forms <2 x float> phis across uniform
@@ -23,16 +30,36 @@ HIPCC_FLAGS = [
 ]
 
 
-def build_library(verbose: bool = False) -> str:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", OUT + ".tmp", SRC]
+def sources() -> list[str]:
+    return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+
+
+def _run(cmd: list[str]) -> tuple[int, str]:
     res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libgacelle_hip.so")
+    return res.returncode, res.stdout + res.stderr
+
+
+def build_library(verbose: bool = False, jobs: int | None = None) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    jobs = jobs or min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(lambda so: (so[0], *_run([hipcc, *HIPCC_FLAGS, "-c", "-o", so[1], so[0]])),
+                              zip(srcs, objs)))
+    failed = [(s, log) for s, rc, log in results if rc != 0]
+    if failed:
+        for s, log in failed:
+            sys.stderr.write(f"--- {s}\n{log}\n")
+        raise RuntimeError(f"hipcc failed on {len(failed)} translation unit(s)")
+    rc, log = _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs])
+    if rc != 0:
+        sys.stderr.write(log)
+        raise RuntimeError("linking libgacelle_hip.so failed")
     os.replace(OUT + ".tmp", OUT)
     if verbose:
-        print("built", OUT)
+        print(f"built {OUT} from {len(srcs)} translation units")
     return OUT
 
 

@@ -1,0 +1,521 @@
+// gc_askadam.hpp -- the askadam driver: the epoch loop of
+// utils/askadam.m:261-299 (min-loss tracking, convergence signals,
+// stopping, update) in FP64 on the host, fed by per-iteration sums that the
+// fused kernel produces K iterations at a time.  A stop or a new minimum that
+// falls inside a block of K iterations is resolved exactly by replaying that
+// volume from the block's starting state, which is kept (state sets).
+#pragma once
+#include "gc_host.hpp"
+
+namespace gc {
+
+// ---- askadam driver --------------------------------------------------------
+struct VolRun {
+  int64_t nv = 0;
+  int nblk = 0;
+  DevBuf set[3][3];  // [state set][z, m, v]
+  DevBuf zlast[2];
+  DevBuf partials;
+  DevBuf ones;       // weights when a volume has none but others do
+  int cur = 0;          // input state set of the next launch
+  int pinned = -1;      // state set kept for the min-loss replay
+  int pin_e0 = 0, pin_r = 0;
+  int min_zlast = -1;   // z_last buffer holding z at the min-loss iteration
+  int zl_next = 0;      // z_last buffer the next launch writes
+  bool active = true;
+  int next_epoch = 1;
+  // loop state of utils/askadam.m (FP64)
+  double ema = 0, conv = 0;
+  std::vector<double> cbuf;
+  int pc = 0, ps = 0, pg = 0;
+  double carry_step = 0;
+  double loss = 0, fid = 0, reg = 0;
+  double min_loss = 0, min_fid = 0, min_reg = 0;
+  int min_iter = 0, last_epoch = 0, stop = GC_STOP_NONE;
+  bool have_init = false;
+  int final_from = -1;  // z_last buffer holding the final z, or -1 (replay)
+  int final_set = -1, final_e0 = 0, final_r = 0;
+  int launches = 0;
+  int chunks = 1;       // chunks per partial-sum row (k_reduce_rows)
+};
+
+struct AskDriver {
+  gc_ctx* ctx;
+  const gc_model& md;
+  const gc_askadam_cfg& cfg;
+  gc_volume* vols;
+  int n_vol;
+  int K = 0;
+  int total_blk = 0;
+  const gc_model* dmd = nullptr;
+  bool has_w = false;
+  std::vector<VolRun> run;
+  DevBuf d_vols, d_es, d_sums;
+  HostBuf h_vols, h_es, h_sums;
+  int max_chunks = 1;
+  std::vector<double> row_sums;  // [vol][K*4] after fold_sums
+  bool desc_in_flight = false;  // an async copy may still read h_vols/h_es
+  // start/stop events attached to every fused launch (kernel-only timing),
+  // created once and reused
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  size_t ev_used = 0;
+  double kernel_ms = 0.0;
+  int kernel_launches = 0;
+
+  ~AskDriver() {
+    for (auto e : ev_start) (void)hipEventDestroy(e);
+    for (auto e : ev_stop) (void)hipEventDestroy(e);
+  }
+
+  // accumulate the durations of recorded launches (after a stream sync)
+  void collect_events() {
+    for (size_t i = 0; i < ev_used; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev_start[i], ev_stop[i]) == hipSuccess) kernel_ms += ms;
+    }
+    ev_used = 0;
+  }
+
+  // add the chunk sums of every row in a fixed order (after a stream sync)
+  void fold_sums(const std::vector<int>& n_ep) {
+    const double* h = h_sums.as<double>();
+    for (int i = 0; i < n_vol; ++i) {
+      const int C = run[i].chunks;
+      for (int r = 0; r < n_ep[i] * 4; ++r) {
+        const double* src = h + (size_t(i) * K * 4 + r) * max_chunks;
+        double t = 0.0;
+        for (int c = 0; c < C; ++c) t += src[c];
+        row_sums[size_t(i) * K * 4 + r] = t;
+      }
+    }
+  }
+
+  AskDriver(gc_ctx* c, const gc_model& m, const gc_askadam_cfg& a, gc_volume* v, int n)
+      : ctx(c), md(m), cfg(a), vols(v), n_vol(n), run(n) {}
+
+  bool stopping_possible() const {
+    return cfg.tol > 0 || cfg.convergence_value >= 0 || cfg.step_tol > 0 || cfg.grad_tol > 0;
+  }
+
+  int setup() {
+    int rcm = upload_model(ctx, &md, &dmd);
+    if (rcm) return rcm;
+    int64_t nv_total = 0;
+    for (int i = 0; i < n_vol; ++i) {
+      if (vols[i].nv <= 0) return set_err(ctx, GC_ERR_ARG, "volume %d has no voxels", i);
+      if (!vols[i].y_dev || !vols[i].z_dev) return set_err(ctx, GC_ERR_ARG, "volume %d: y/z missing", i);
+      if (md.n_extra > 0 && !vols[i].extra_dev) return set_err(ctx, GC_ERR_ARG, "volume %d: extra data missing", i);
+      nv_total += vols[i].nv;
+      if (vols[i].w_dev) has_w = true;
+    }
+    // Iterations per launch.  When a stop can land inside a block, blocks of
+    // ~2.5e8 voxel-iterations (at most 256 iterations) bound the replay that
+    // resolves it.  Without stopping criteria the host has nothing to decide
+    // between blocks: as many iterations per launch as the block partials
+    // (16 bytes per block and iteration) allow within 1 GiB.
+    int64_t nblk_total = 0;
+    for (int i = 0; i < n_vol; ++i) nblk_total += (vols[i].nv + kBlock - 1) / kBlock;
+    const int k_mem = int(std::max<int64_t>(16, (int64_t(1) << 30) / (16 * std::max<int64_t>(nblk_total, 1))));
+    if (cfg.block_iters > 0) K = cfg.block_iters;
+    else if (stopping_possible()) {
+      const double k = 2.5e8 / double(std::max<int64_t>(nv_total, 1));
+      K = int(std::min(256.0, std::max(16.0, k)));
+    } else {
+      K = std::min(k_mem, 8192);  // 4 partial rows per iteration on grid.y
+    }
+    K = std::max(1, std::min(K, std::max(1, cfg.iteration)));
+    if (K > 16383) return set_err(ctx, GC_ERR_ARG, "iterationsPerLaunch %d exceeds 16383", K);
+    const int P = md.n_par;
+    total_blk = 0;
+    for (int i = 0; i < n_vol; ++i) {
+      VolRun& R = run[i];
+      R.nv = vols[i].nv;
+      R.nblk = int((R.nv + kBlock - 1) / kBlock);
+      const size_t pbytes = size_t(P) * R.nv * sizeof(float);
+      for (int s = 0; s < 3; ++s)
+        for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.set[s][j].alloc(pbytes));
+      for (int s = 0; s < 2; ++s) GC_HIP(ctx, R.zlast[s].alloc(pbytes));
+      GC_HIP(ctx, R.partials.alloc(size_t(K) * 4 * R.nblk * sizeof(float)));
+      GC_HIP(ctx, hipMemcpyAsync(R.set[0][0].p, vols[i].z_dev, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+      GC_HIP(ctx, hipMemsetAsync(R.set[0][1].p, 0, pbytes, ctx->stream));
+      GC_HIP(ctx, hipMemsetAsync(R.set[0][2].p, 0, pbytes, ctx->stream));
+      if (cfg.transform == GC_TRANSFORM_LINEAR) {
+        // set_boundary01 of the starting point (utils/askadam.m:183-185)
+        hipLaunchKernelGGL(k_clamp01, dim3(nblocks(int64_t(P) * R.nv)), dim3(kBlock), 0, ctx->stream,
+                           R.set[0][0].as<float>(), int64_t(P) * R.nv);
+      }
+      if (has_w && !vols[i].w_dev) {
+        const int64_t n = int64_t(md.n_meas) * R.nv;
+        GC_HIP(ctx, R.ones.alloc(size_t(n) * sizeof(float)));
+        hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, R.ones.as<float>(), n, 1.0f);
+      }
+      R.cbuf.assign(std::max(cfg.convergence_window, 1), 1.0);
+      R.chunks = int((R.nblk + kReduceChunk - 1) / kReduceChunk);
+      max_chunks = std::max(max_chunks, R.chunks);
+      total_blk += R.nblk;
+    }
+    GC_HIP(ctx, d_vols.alloc(sizeof(VolDev) * n_vol));
+    GC_HIP(ctx, h_vols.alloc(sizeof(VolDev) * n_vol));
+    GC_HIP(ctx, d_es.alloc(sizeof(EpochScal) * K * n_vol));
+    GC_HIP(ctx, h_es.alloc(sizeof(EpochScal) * K * n_vol));
+    GC_HIP(ctx, d_sums.alloc(sizeof(double) * K * 4 * max_chunks * n_vol));
+    GC_HIP(ctx, h_sums.alloc(sizeof(double) * K * 4 * max_chunks * n_vol));
+    row_sums.assign(size_t(K) * 4 * n_vol, 0.0);
+    return GC_OK;
+  }
+
+  EpochScal epoch_scal(int e) const {
+    EpochScal s{};
+    const double lr = cfg.lr0 / (1.0 + cfg.decay * (e - 1));  // update_learn_rate, :1180-1182
+    if (cfg.optimiser == GC_OPT_ADAM) {
+      const double bc1 = 1.0 - std::pow(cfg.beta1, e), bc2 = 1.0 - std::pow(cfg.beta2, e);
+      if (cfg.adam_form == GC_ADAM_KINGMA_BA) {
+        s.lr = float(lr);
+        s.c1 = float(1.0 / bc1);
+        s.c2 = float(1.0 / bc2);
+      } else {
+        s.lr = float(lr * std::sqrt(bc2) / bc1);
+        s.c1 = s.c2 = 1.f;
+      }
+    } else {
+      s.lr = float(lr);
+    }
+    return s;
+  }
+
+  // Launch one block of iterations for the volumes with n_ep[i] > 0.
+  // in_set/out_set select state sets; zl selects the z_last buffer (-1 none).
+  template <class Model, int LOSS, bool HAS_W>
+  int launch(const std::vector<int>& e0, const std::vector<int>& n_ep, const std::vector<int>& in_set,
+             const std::vector<int>& out_set, const std::vector<int>& zl, int iteration_cap) {
+    // the pinned descriptor buffers are reused: wait for the copies of the
+    // previous launch before overwriting them
+    if (desc_in_flight) GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    VolDev* hv = h_vols.as<VolDev>();
+    EpochScal* hes = h_es.as<EpochScal>();
+    int blk0 = 0;
+    for (int i = 0; i < n_vol; ++i) {
+      VolRun& R = run[i];
+      VolDev d{};
+      d.y = vols[i].y_dev;
+      d.w = HAS_W ? (vols[i].w_dev ? vols[i].w_dev : R.ones.as<float>()) : nullptr;
+      d.ex = vols[i].extra_dev;
+      const int si = std::max(in_set[i], 0), so = std::max(out_set[i], 0);
+      d.z_in = R.set[si][0].as<float>();
+      d.m_in = R.set[si][1].as<float>();
+      d.v_in = R.set[si][2].as<float>();
+      d.z_out = R.set[so][0].as<float>();
+      d.m_out = R.set[so][1].as<float>();
+      d.v_out = R.set[so][2].as<float>();
+      d.z_last = zl[i] >= 0 ? R.zlast[zl[i]].as<float>() : nullptr;
+      d.partials = R.partials.as<float>();
+      d.es = d_es.as<EpochScal>() + size_t(i) * K;
+      d.nv = R.nv;
+      d.blk0 = blk0;
+      d.nblk = R.nblk;
+      d.epoch0 = e0[i];
+      d.n_epochs = n_ep[i];
+      d.iteration = iteration_cap;
+      d.inv_count = float(1.0 / (double(md.n_meas) * double(R.nv)));
+      hv[i] = d;
+      for (int k = 0; k < n_ep[i]; ++k) hes[size_t(i) * K + k] = epoch_scal(e0[i] + k);
+      blk0 += R.nblk;
+      if (n_ep[i] > 0) R.launches++;
+    }
+    GC_HIP(ctx, hipMemcpyAsync(d_vols.p, h_vols.p, sizeof(VolDev) * n_vol, hipMemcpyHostToDevice, ctx->stream));
+    GC_HIP(ctx, hipMemcpyAsync(d_es.p, h_es.p, sizeof(EpochScal) * K * n_vol, hipMemcpyHostToDevice, ctx->stream));
+    desc_in_flight = true;
+    AskParams ap{};
+    ap.vols = d_vols.as<VolDev>();
+    ap.n_vol = n_vol;
+    ap.opt = cfg.optimiser;
+    ap.transform = cfg.transform;
+    ap.adam_form = cfg.adam_form;
+    ap.need_norms = (cfg.step_tol > 0 || cfg.grad_tol > 0) ? 1 : 0;
+    ap.beta1 = float(cfg.beta1);
+    ap.beta2 = float(cfg.beta2);
+    ap.eps = float(cfg.eps);
+    ap.momentum = float(cfg.momentum);
+    ap.rms_decay = float(cfg.rms_decay);
+    ap.rms_eps = float(cfg.rms_eps);
+    if (ev_used == ev_start.size()) {
+      hipEvent_t ea, eb;
+      GC_HIP(ctx, hipEventCreate(&ea));
+      GC_HIP(ctx, hipEventCreate(&eb));
+      ev_start.push_back(ea);
+      ev_stop.push_back(eb);
+    }
+    hipEvent_t ev_a = ev_start[ev_used], ev_b = ev_stop[ev_used];
+    ++ev_used;
+    // events attached to the dispatch itself time the kernel alone
+    hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
+                          ev_b, 0, dmd, ap);
+    GC_HIP(ctx, hipGetLastError());
+    ++kernel_launches;
+    for (int i = 0; i < n_vol; ++i) {
+      if (n_ep[i] <= 0) continue;
+      hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, n_ep[i] * 4), dim3(kBlock), 0, ctx->stream,
+                         run[i].partials.as<float>(), int64_t(run[i].nblk), int64_t(kReduceChunk),
+                         d_sums.as<double>() + size_t(i) * K * 4 * max_chunks);
+      GC_HIP(ctx, hipGetLastError());
+    }
+    return GC_OK;
+  }
+
+  // One iteration of the host-side loop logic (utils/askadam.m:264-292 minus
+  // the update itself).  Returns true when the volume stops at `epoch`.
+  bool epoch_logic(VolRun& R, int epoch, double sum_l, double sum_step_prev, double sum_den, double sum_g,
+                   double* trace) {
+    const double count = double(md.n_meas) * double(R.nv);
+    R.fid = sum_l / count;
+    R.reg = 0.0;
+    R.loss = R.fid + R.reg;
+    if (trace) trace[epoch] = R.loss;
+    R.last_epoch = epoch;
+    // update_min_loss (:433-446)
+    bool new_min = false;
+    if (R.min_loss > R.loss) {
+      R.min_loss = R.loss; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = epoch;
+      new_min = true;
+    }
+    (void)new_min;
+    // update_convergence_signals (:512-632)
+    if (cfg.convergence_model == GC_CONV_EMA) {
+      const double e_new = cfg.ema_decay * R.ema + (1.0 - cfg.ema_decay) * R.loss;
+      R.conv = std::fabs(e_new - R.ema) / (std::fabs(R.ema) + 1e-8);
+      R.ema = e_new;
+    } else {
+      std::rotate(R.cbuf.begin(), R.cbuf.begin() + 1, R.cbuf.end());
+      R.cbuf.back() = R.loss;
+      const int n = int(R.cbuf.size());
+      double sx = 0, sy = 0, sxx = 0, sxy = 0;
+      for (int i = 0; i < n; ++i) {
+        const double xi = i + 1;
+        sx += xi; sy += R.cbuf[i]; sxx += xi * xi; sxy += xi * R.cbuf[i];
+      }
+      const double den = n * sxx - sx * sx;
+      R.conv = -((den != 0) ? (n * sxy - sx * sy) / den : 0.0);
+    }
+    if (R.conv > cfg.convergence_value || epoch <= cfg.convergence_window) R.pc = 0;
+    else if ((R.min_loss - R.loss) > cfg.convergence_value) R.pc = 0;
+    else R.pc += 1;
+    if (cfg.step_tol > 0) {
+      const double num = (epoch == 1) ? 0.0 : sum_step_prev;
+      const double sn = std::sqrt(num) / (1.0 + std::sqrt(sum_den));
+      R.ps = (sn < cfg.step_tol) ? R.ps + 1 : 0;
+    }
+    if (cfg.grad_tol > 0) {
+      const double gn = std::sqrt(sum_g);
+      R.pg = (gn < cfg.grad_tol) ? R.pg + 1 : 0;
+    }
+    // check_stopping (:634-671), in order
+    int stop = GC_STOP_NONE;
+    if (R.pc > cfg.patience_conv) stop = GC_STOP_CONVERGENCE;
+    else if (R.loss < cfg.tol) stop = GC_STOP_TOL;
+    else if (cfg.step_tol > 0 && R.ps > cfg.patience_step) stop = GC_STOP_STEP;
+    else if (cfg.grad_tol > 0 && R.pg > cfg.patience_grad) stop = GC_STOP_GRAD;
+    R.stop = stop;
+    return stop != GC_STOP_NONE;
+  }
+
+  template <class Model, int LOSS, bool HAS_W>
+  int run_all() {
+    const int P = md.n_par;
+    const int iter = cfg.iteration;
+    const double* sums = row_sums.data();
+    const size_t sum_bytes = sizeof(double) * K * 4 * max_chunks * n_vol;
+    std::vector<int> e0(n_vol), nep(n_vol), in(n_vol), out(n_vol), zl(n_vol);
+
+    if (iter <= 0) {
+      // evaluation only (epoch 0): run one non-updating iteration
+      for (int i = 0; i < n_vol; ++i) { e0[i] = 1; nep[i] = 1; in[i] = 0; out[i] = 1; zl[i] = 0; }
+      int rc = launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, 0);
+      if (rc) return rc;
+      GC_HIP(ctx, hipMemcpyAsync(h_sums.p, d_sums.p, sum_bytes, hipMemcpyDeviceToHost, ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      fold_sums(nep);
+      for (int i = 0; i < n_vol; ++i) {
+        VolRun& R = run[i];
+        const double* s = sums + size_t(i) * K * 4;
+        R.fid = s[0] / (double(md.n_meas) * double(R.nv));
+        R.loss = R.fid;
+        R.min_loss = R.loss; R.min_fid = R.fid; R.min_iter = 0; R.last_epoch = 0;
+        if (vols[i].loss_trace_host) vols[i].loss_trace_host[0] = R.loss;
+        R.final_from = 0;
+        R.min_zlast = 0;
+      }
+      return GC_OK;
+    }
+
+    bool any = true;
+    while (any) {
+      for (int i = 0; i < n_vol; ++i) {
+        VolRun& R = run[i];
+        if (!R.active) { nep[i] = 0; in[i] = R.cur; out[i] = R.cur; zl[i] = -1; e0[i] = 0; continue; }
+        e0[i] = R.next_epoch;
+        nep[i] = std::min(K, iter - R.next_epoch + 1);
+        in[i] = R.cur;
+        int o = 0;
+        while (o == R.cur || o == R.pinned) ++o;
+        out[i] = o;
+        zl[i] = R.zl_next;
+      }
+      int rc = launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, iter);
+      if (rc) return rc;
+      GC_HIP(ctx, hipMemcpyAsync(h_sums.p, d_sums.p, sum_bytes, hipMemcpyDeviceToHost, ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      desc_in_flight = false;
+      collect_events();
+      fold_sums(nep);
+      any = false;
+      for (int i = 0; i < n_vol; ++i) {
+        VolRun& R = run[i];
+        if (!R.active) continue;
+        const double* s = sums + size_t(i) * K * 4;
+        double* trace = vols[i].loss_trace_host;
+        if (!R.have_init) {
+          // starting-point evaluation (:197-219) equals the first iteration's
+          const double l0 = s[0] / (double(md.n_meas) * double(R.nv));
+          R.loss = R.fid = l0; R.reg = 0;
+          R.min_loss = l0; R.min_fid = l0; R.min_reg = 0; R.min_iter = 0;
+          R.pinned = in[i]; R.pin_e0 = e0[i]; R.pin_r = 0; R.min_zlast = -1;
+          R.ema = l0;
+          if (trace) trace[0] = l0;
+          R.have_init = true;
+        }
+        bool stopped = false;
+        int k = 0;
+        for (; k < nep[i]; ++k) {
+          const int epoch = e0[i] + k;
+          const double prev_min = R.min_loss;
+          const double step_prev = (k == 0) ? R.carry_step : s[(k - 1) * 4 + 1];
+          stopped = epoch_logic(R, epoch, s[k * 4 + 0], step_prev, s[k * 4 + 2], s[k * 4 + 3], trace);
+          if (R.min_loss < prev_min) {
+            if (k == nep[i] - 1) {
+              R.min_zlast = zl[i];
+              R.pinned = -1;
+            } else {
+              R.min_zlast = -1;
+              R.pinned = in[i]; R.pin_e0 = e0[i]; R.pin_r = k;
+            }
+          }
+          if (stopped) break;
+        }
+        if (stopped) {
+          R.active = false;
+          if (k == nep[i] - 1) { R.final_from = zl[i]; }
+          else { R.final_from = -1; R.final_set = in[i]; R.final_e0 = e0[i]; R.final_r = k; }
+        } else {
+          R.carry_step = s[(nep[i] - 1) * 4 + 1];
+          R.cur = out[i];
+          R.next_epoch = e0[i] + nep[i];
+          if (R.next_epoch > iter) {
+            R.active = false;
+            R.final_from = zl[i];
+          }
+        }
+        // the next launch must not overwrite the z_last buffer holding the min
+        R.zl_next = (R.min_zlast == 0) ? 1 : ((R.min_zlast == 1) ? 0 : (zl[i] == 0 ? 1 : 0));
+        if (R.final_from >= 0 && R.final_from == R.zl_next) R.zl_next ^= 1;
+        any = any || R.active;
+      }
+    }
+    (void)P;
+    return GC_OK;
+  }
+
+  // Replay `r`+1 iterations of one volume from state set `set` starting at
+  // epoch e0 and leave z (pre-update) of iteration e0+r in z_last[zb].
+  template <class Model, int LOSS, bool HAS_W>
+  int replay(int vi, int set, int e0v, int r, int zb) {
+    std::vector<int> e0(n_vol, 0), nep(n_vol, 0), in(n_vol, 0), out(n_vol, 0), zl(n_vol, -1);
+    VolRun& R = run[vi];
+    int o = 0;
+    while (o == set || o == R.pinned) ++o;
+    if (o > 2) o = (set + 1) % 3;
+    e0[vi] = e0v; nep[vi] = r + 1; in[vi] = set; out[vi] = o; zl[vi] = zb;
+    for (int i = 0; i < n_vol; ++i) if (i != vi) { in[i] = run[i].cur; out[i] = run[i].cur; }
+    return launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, cfg.iteration);
+  }
+
+  template <class Model, int LOSS, bool HAS_W>
+  int finalize() {
+    const int P = md.n_par;
+    for (int i = 0; i < n_vol; ++i) {
+      VolRun& R = run[i];
+      const size_t pbytes = size_t(P) * R.nv * sizeof(float);
+      // final parameters
+      int zb = R.final_from;
+      if (zb < 0) {
+        zb = (R.min_zlast == 0) ? 1 : 0;
+        int rc = replay<Model, LOSS, HAS_W>(i, R.final_set, R.final_e0, R.final_r, zb);
+        if (rc) return rc;
+      }
+      GC_HIP(ctx, hipMemcpyAsync(vols[i].z_dev, R.zlast[zb].p, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+      // parameters at minimum loss
+      if (vols[i].zmin_dev) {
+        if (R.min_zlast >= 0) {
+          GC_HIP(ctx, hipMemcpyAsync(vols[i].zmin_dev, R.zlast[R.min_zlast].p, pbytes, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        } else if (R.pin_r == 0) {
+          GC_HIP(ctx, hipMemcpyAsync(vols[i].zmin_dev, R.set[R.pinned][0].p, pbytes, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        } else {
+          const int zb2 = 0;
+          int rc = replay<Model, LOSS, HAS_W>(i, R.pinned, R.pin_e0, R.pin_r, zb2);
+          if (rc) return rc;
+          GC_HIP(ctx, hipMemcpyAsync(vols[i].zmin_dev, R.zlast[zb2].p, pbytes, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        }
+      }
+      if (cfg.transform == GC_TRANSFORM_LINEAR) {
+        // set_boundary01 on both results (:314-317)
+        hipLaunchKernelGGL(k_clamp01, dim3(nblocks(int64_t(P) * R.nv)), dim3(kBlock), 0, ctx->stream,
+                           vols[i].z_dev, int64_t(P) * R.nv);
+        if (vols[i].zmin_dev)
+          hipLaunchKernelGGL(k_clamp01, dim3(nblocks(int64_t(P) * R.nv)), dim3(kBlock), 0, ctx->stream,
+                             vols[i].zmin_dev, int64_t(P) * R.nv);
+      }
+      // residual outputs (out.final.resloss / residual, out.min.*)
+      for (int which = 0; which < 2; ++which) {
+        float* rl = which == 0 ? vols[i].resloss_dev : vols[i].resloss_min_dev;
+        float* rs = which == 0 ? vols[i].residual_dev : vols[i].residual_min_dev;
+        const float* zsrc = which == 0 ? vols[i].z_dev : vols[i].zmin_dev;
+        if ((!rl && !rs) || !zsrc) continue;
+        EvalParams ep{};
+        ep.x = zsrc;
+        ep.y = vols[i].y_dev;
+        ep.w = HAS_W ? (vols[i].w_dev ? vols[i].w_dev : R.ones.as<float>()) : nullptr;
+        ep.ex = vols[i].extra_dev;
+        ep.residual = rs;
+        ep.resloss = rl;
+        ep.nv = R.nv;
+        ep.transform = cfg.transform;
+        ep.mode = 1;
+        ep.scale = 1.f;
+        hipLaunchKernelGGL((k_eval<Model, LOSS, HAS_W>), dim3(nblocks(R.nv)), dim3(kBlock), 0, ctx->stream, dmd, ep);
+        GC_HIP(ctx, hipGetLastError());
+      }
+      gc_volume& V = vols[i];
+      V.loss = R.loss; V.loss_fidelity = R.fid; V.loss_reg = R.reg;
+      V.min_loss = R.min_loss; V.min_loss_fidelity = R.min_fid; V.min_loss_reg = R.min_reg;
+      V.convergence = R.conv;
+      V.n_iteration = R.last_epoch;
+      V.min_iteration = R.min_iter;
+      V.stop_reason = R.stop;
+      V.n_launch = R.launches;
+    }
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    desc_in_flight = false;
+    collect_events();
+    for (int i = 0; i < n_vol; ++i) {
+      vols[i].kernel_ms = kernel_ms;
+      vols[i].kernel_launches = kernel_launches;
+      vols[i].iters_per_launch = K;
+    }
+    return GC_OK;
+  }
+};
+
+
+}  // namespace gc

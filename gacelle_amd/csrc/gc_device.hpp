@@ -54,32 +54,50 @@ GC_DEV void box_muller(float u1, float u2, float& n0, float& n1) {
 }
 
 // ---------------------------------------------------------------------------
-// Wavefront (64-lane) reductions.
-GC_DEV float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// Wavefront (64-lane) reductions.  Exchanges inside a row of 16 lanes use DPP
+// (quad_perm xor 1 / xor 2, row_ror 4 / 8), which fold into the VALU add;
+// the two cross-row steps (lane ^ 16, lane ^ 32) go through ds_bpermute.
+// (The gfx950 permlane16/32 swap builtins are not used: this compiler
+// returns the first register of the swapped pair for both results.)
+template <int CTRL>
+GC_DEV float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppRor4 = 0x124;  // row_ror:4
+constexpr int kDppRor8 = 0x128;  // row_ror:8
+
+// Sum over the 16 lanes of each row that share lane % 4, then over rows.
+GC_DEV float sum_class4(float k) {
+  k += dpp<kDppRor4>(k);
+  k += dpp<kDppRor8>(k);
+  k += __shfl_xor(k, 16, 64);
+  return k + __shfl_xor(k, 32, 64);
 }
 
-// Reduce four values across the wavefront with a butterfly transpose
-// (7 lane exchanges instead of 24).  On return lane L holds the total of
-// value index ((L & 1) << 1) | ((L >> 1) & 1); lanes 0..3 carry indices
-// 0, 2, 1, 3.
+GC_DEV float wave_sum(float v) {
+  v += dpp<kDppXor1>(v);
+  v += dpp<kDppXor2>(v);
+  return sum_class4(v);
+}
+
+// Reduce four values across the wavefront with a butterfly transpose: after
+// the two in-quad steps lane L carries value index
+// ((L & 1) << 1) | ((L >> 1) & 1) and the rest is a sum over that class.
+// On return every lane holds the total of its index; lanes 0..3 carry
+// indices 0, 2, 1, 3.
 GC_DEV float wave_sum4(float v0, float v1, float v2, float v3, int lane) {
   const bool b0 = lane & 1;
   const float k0 = b0 ? v2 : v0, k1 = b0 ? v3 : v1;
   const float s0 = b0 ? v0 : v2, s1 = b0 ? v1 : v3;
-  const float a0 = k0 + __shfl_xor(s0, 1, 64);
-  const float a1 = k1 + __shfl_xor(s1, 1, 64);
+  const float a0 = k0 + dpp<kDppXor1>(s0);
+  const float a1 = k1 + dpp<kDppXor1>(s1);
   const bool b1 = lane & 2;
   float k = b1 ? a1 : a0;
   const float s = b1 ? a0 : a1;
-  k += __shfl_xor(s, 2, 64);
-  k += __shfl_xor(k, 4, 64);
-  k += __shfl_xor(k, 8, 64);
-  k += __shfl_xor(k, 16, 64);
-  k += __shfl_xor(k, 32, 64);
-  return k;
+  k += dpp<kDppXor2>(s);
+  return sum_class4(k);
 }
 GC_DEV int wave_sum4_index(int lane) { return ((lane & 1) << 1) | ((lane >> 1) & 1); }
 

@@ -1,0 +1,176 @@
+// gc_host.hpp -- host-side infrastructure shared by the translation units of
+// libgacelle_hip.so: the context, error reporting, device and pinned
+// buffers, model checks, and the table of per-model operations.  Each
+// register-tiled model variant is compiled in a translation unit of its own
+// (ops_*.hip) and reached through its ModelOps table.
+#pragma once
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/gacelle.h"
+#include "gc_kernels.hpp"
+
+struct gc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  gc_model* d_model = nullptr;  // device copy of the last model description
+};
+
+
+namespace gc {
+
+
+inline int set_err(gc_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+#define GC_HIP(ctx, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return set_err((ctx), GC_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                                      \
+  } while (0)
+
+// RAII device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t bytes) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = bytes;
+    if (bytes == 0) return hipSuccess;
+    return hipMalloc(&p, bytes);
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { if (p) (void)hipHostFree(p); }
+  hipError_t alloc(size_t bytes) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    return hipHostMalloc(&p, std::max<size_t>(bytes, 64), hipHostMallocDefault);
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+inline unsigned nblocks(int64_t n, int b = kBlock) { return unsigned((n + b - 1) / b); }
+
+// Model descriptions are read by kernels from device memory (uniform
+// addresses: scalar loads), not from the kernel-argument segment, so that
+// table lookups with a loop index never force a private copy.
+inline int upload_model(gc_ctx* ctx, const gc_model* md, const gc_model** out) {
+  if (!ctx->d_model) GC_HIP(ctx, hipMalloc(&ctx->d_model, sizeof(gc_model)));
+  GC_HIP(ctx, hipMemcpyAsync(ctx->d_model, md, sizeof(gc_model), hipMemcpyHostToDevice, ctx->stream));
+  *out = ctx->d_model;
+  return GC_OK;
+}
+template <class F>
+int with_loss(gc_ctx* ctx, int loss, F&& f) {
+  switch (loss) {
+    case GC_LOSS_L1: return f(std::integral_constant<int, GC_LOSS_L1>{});
+    case GC_LOSS_L2: return f(std::integral_constant<int, GC_LOSS_L2>{});
+    case GC_LOSS_HUBER: return f(std::integral_constant<int, GC_LOSS_HUBER>{});
+    case GC_LOSS_MSE: return f(std::integral_constant<int, GC_LOSS_MSE>{});
+    default: return set_err(ctx, GC_ERR_ARG, "unknown loss function %d", loss);
+  }
+}
+
+template <class F>
+int with_bool(bool b, F&& f) {
+  if (b) return f(std::true_type{});
+  return f(std::false_type{});
+}
+
+inline int model_np(const gc_model& md) {
+  switch (md.model_id) {
+    case GC_MODEL_R2STAR: return 2;
+    case GC_MODEL_JOINT_R1R2STAR: return 3;
+    case GC_MODEL_NEXI: return md.i[1] == 2 ? 5 : 4;
+    default: return -1;
+  }
+}
+
+inline int check_model(gc_ctx* ctx, const gc_model* md) {
+  if (!md) return set_err(ctx, GC_ERR_ARG, "model is NULL");
+  if (md->n_par < 1 || md->n_par > GC_MAX_PAR) return set_err(ctx, GC_ERR_ARG, "n_par out of range");
+  if (md->n_extra < 0 || md->n_extra > 8) return set_err(ctx, GC_ERR_ARG, "n_extra out of range (0..8)");
+  const int np = model_np(*md);
+  if (np < 0) return set_err(ctx, GC_ERR_UNSUPPORTED, "model %d is not supported", md->model_id);
+  if (md->n_par < np) return set_err(ctx, GC_ERR_ARG, "n_par (%d) below the model's %d parameters", md->n_par, np);
+  int m_expect = 0;
+  switch (md->model_id) {
+    case GC_MODEL_R2STAR: m_expect = md->i[0]; break;
+    case GC_MODEL_JOINT_R1R2STAR: m_expect = md->i[0] * md->i[1]; break;
+    case GC_MODEL_NEXI: m_expect = md->i[0] * (md->i[1] == 2 ? 2 : 1); break;
+  }
+  if (md->n_meas != m_expect)
+    return set_err(ctx, GC_ERR_INPUT_MISMATCH, "n_meas (%d) does not match the acquisition (%d)", md->n_meas, m_expect);
+  if (md->model_id == GC_MODEL_JOINT_R1R2STAR) {
+    const int need = (md->flags & GC_FLAG_R1R2S_TRUE_FA) ? md->i[1] : 1;
+    if (md->n_extra < need) return set_err(ctx, GC_ERR_ARG, "joint R1-R2* needs %d extra rows", need);
+  }
+  if (md->model_id == GC_MODEL_NEXI) {
+    const int ns = md->i[0], nn = md->i[2];
+    if (nn < 2 || 2 * ns + 2 * nn + 34 > GC_MAX_TABLE) return set_err(ctx, GC_ERR_ARG, "NEXI table too large");
+    if ((md->flags & GC_FLAG_NEXI_RICIAN) && md->n_extra < 1)
+      return set_err(ctx, GC_ERR_ARG, "Rician NEXI needs sigma as extra row 0");
+  }
+  return GC_OK;
+}
+
+// Operations that depend on the register-tiled model type.
+struct ModelOps {
+  // forward model: s = FWD(theta) (model_forward of each model class)
+  int (*fwd)(gc_ctx* ctx, const gc_model* dmd, const float* theta, const float* extra, float* s, int64_t nv);
+  // loss, adjoint and residuals at x (askadam.model_gradient); part gets
+  // one FP64 loss partial per block of kBlock voxels when not null
+  int (*fwd_adj)(gc_ctx* ctx, const gc_model* dmd, int loss, int transform, const float* x, const float* y,
+                 const float* w, const float* extra, int64_t nv, float scale, float* grad, double* part,
+                 float* residual);
+  // the askadam loop over a batch of volumes
+  int (*askadam)(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol);
+  // one MH or GW launch (kernel-only timing through ev0/ev1)
+  int (*mcmc)(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
+              hipEvent_t ev0, hipEvent_t ev1);
+};
+
+// one getter per model variant, each defined in its own ops_*.hip
+const ModelOps* ops_r2s12_uniform();
+const ModelOps* ops_r2s12();
+const ModelOps* ops_r2s16();
+const ModelOps* ops_r2s32();
+const ModelOps* ops_joint12x4();
+const ModelOps* ops_joint16x4();
+const ModelOps* ops_joint16x8();
+const ModelOps* ops_nexi16();
+const ModelOps* ops_nexi16_l2();
+
+}  // namespace gc

@@ -21,9 +21,16 @@ namespace gc {
 
 constexpr int kBlock = 256;
 
+// Kernels that are not templates have internal linkage (static): several
+// translation units include this header and launch them.
+
 // reciprocal with the hardware v_rcp_f32 (1 ulp)
 GC_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// square root with the hardware v_sqrt_f32 (1 ulp), not the correctly
+// rounded expansion
+GC_DEV float sqrtv(float x) { return __builtin_amdgcn_sqrtf(x); }
 constexpr int kRing = 16;  // iterations buffered in LDS before a partial flush
+constexpr int kReduceChunk = 4096;  // block partials per k_reduce_rows block
 
 // ---- loss functions of askadam.model_gradient (utils/askadam.m:86-95) -----
 template <int LOSS>
@@ -45,18 +52,54 @@ GC_DEV void loss_fn(float r, float w, float& l, float& dl) {
 }
 
 // ---- parameter transform (utils/askadam.m:1063-1073, forward_pass :424) ---
-// Branch-free on purpose: the selection is a select, never control flow
-// around values that are later combined (see the build notes on SLP).
-GC_DEV void transform_fwd(int transform, float z, float lb, float ub, float& th, float& dth) {
+// theta = lb + (ub - lb) * sigmoid(z)  or  lb + (ub - lb) * clamp(z, 0, 1),
+// with d theta / d z.
+template <int TR>
+GC_DEV void transform_fwd_t(float z, float lb, float ub, float& th, float& dth) {
   const float r = ub - lb;
-  const float sg = rcpf(1.f + __expf(-z));
-  const float zc = fminf(fmaxf(z, 0.f), 1.f);
+  if (TR == GC_TRANSFORM_SIGMOID) {
+    const float sg = rcpf(1.f + __expf(-z));
+    th = lb + r * sg;
+    dth = r * sg * (1.f - sg);
+  } else if (TR == GC_TRANSFORM_LINEAR) {
+    th = fminf(fmaxf(z, 0.f), 1.f) * r + lb;
+    dth = (z >= 0.f && z <= 1.f) ? r : 0.f;
+  } else {
+    th = z;
+    dth = 1.f;
+  }
+}
+
+// Run-time selection for the single-evaluation kernels.  Branch-free: the
+// selection is a select, never control flow around values that are later
+// combined.
+GC_DEV void transform_fwd(int transform, float z, float lb, float ub, float& th, float& dth) {
+  float ts, ds, tl, dl;
+  transform_fwd_t<GC_TRANSFORM_SIGMOID>(z, lb, ub, ts, ds);
+  transform_fwd_t<GC_TRANSFORM_LINEAR>(z, lb, ub, tl, dl);
   const bool sig = transform == GC_TRANSFORM_SIGMOID;
   const bool lin = transform == GC_TRANSFORM_LINEAR;
-  const float th_sig = lb + r * sg, d_sig = r * sg * (1.f - sg);
-  const float th_lin = zc * r + lb, d_lin = (z >= 0.f && z <= 1.f) ? r : 0.f;
-  th = sig ? th_sig : (lin ? th_lin : z);
-  dth = sig ? d_sig : (lin ? d_lin : 1.f);
+  th = sig ? ts : (lin ? tl : z);
+  dth = sig ? ds : (lin ? dl : 1.f);
+}
+
+// ---- d loss / d theta from one measurement --------------------------------
+// A model with kRelJ hands J[0] = d s / d theta_0 >= 0 and, for p > 0, the
+// ratio (d s / d theta_p) / J[0] (its post_grad restores any common factor).
+// With the L1 fast-path derivative dl = copysign(w, r) and J[0] >= 0,
+// dl * J[0] = copysign(w * J[0], r): one bit operation.
+template <class Model, bool L1FAST, bool HAS_W>
+GC_DEV void accum_grad(float dl, float w, float res, const float* J, float* gth) {
+  constexpr int NP = Model::NP;
+  if constexpr (Model::kRelJ) {
+    const float a = L1FAST ? copysignf(HAS_W ? w * J[0] : J[0], res) : dl * J[0];
+    gth[0] += a;
+#pragma unroll
+    for (int p = 1; p < NP; ++p) gth[p] = fmaf(a, J[p], gth[p]);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) gth[p] = fmaf(dl, J[p], gth[p]);
+  }
 }
 
 // ---- askadam block kernel -------------------------------------------------
@@ -93,6 +136,7 @@ struct AskParams {
   int32_t opt;
   int32_t transform;
   int32_t adam_form;
+  int32_t need_norms;  // step / |z|^2 / |g|^2 sums wanted (step or gradient tolerance active)
   float beta1, beta2, eps, momentum, rms_decay, rms_eps;
 };
 
@@ -151,97 +195,131 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
     dead_z2 += zd * zd;
   }
 
+  // The epoch table is read-only for the whole launch: reading it through
+  // the constant address space lets the wave-uniform loads go scalar.
+  using ConstEpochScal = __attribute__((address_space(4))) const EpochScal;
+  ConstEpochScal* es_table = reinterpret_cast<ConstEpochScal*>(reinterpret_cast<uintptr_t>(V.es));
   __shared__ float red[kRing][kBlock / 64][4];
   const float inv_count = V.inv_count;
-  for (int k = 0; k < V.n_epochs; ++k) {
-    const int e = V.epoch0 + k;
-    float th[NP], dth[NP];
+  const bool norms = ap.need_norms != 0;
+
+  // The epoch loop, instantiated per parameter transform so that the
+  // transform costs no per-iteration selection.
+  auto epochs = [&](auto tr) {
+    constexpr int TR = decltype(tr)::value;
+    for (int k = 0; k < V.n_epochs; ++k) {
+      const int e = V.epoch0 + k;
+      float th[NP], dth[NP];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) transform_fwd(ap.transform, z[p], lbp[p], ubp[p], th[p], dth[p]);
-    float lsum = 0.f;
-    float gth[NP];
+      for (int p = 0; p < NP; ++p) transform_fwd_t<TR>(z[p], lbp[p], ubp[p], th[p], dth[p]);
+      float lsum = 0.f;
+      float gth[NP];
+      // Fast pass without per-element checks; a non-finite model signal makes
+      // the loss sum non-finite, and only then is the voxel re-evaluated with
+      // utils.set_nan_inf_zero semantics (zero signal, no gradient).  The fast
+      // pass takes the L1 derivative as copysign(w, r) and keeps min |r|: an
+      // exact zero residual (sign 0) sends the voxel to the checked pass too.
+      float amin = 1.f;
+      auto accumulate = [&](auto checked) {
+        constexpr bool CHECK = decltype(checked)::value;
+        constexpr bool L1FAST = !CHECK && LOSS == GC_LOSS_L1;
+        lsum = 0.f;
 #pragma unroll
-    for (int p = 0; p < NP; ++p) gth[p] = 0.f;
-    // Fast pass without per-element checks; a non-finite model signal makes
-    // the loss sum non-finite, and only then is the voxel re-evaluated with
-    // utils.set_nan_inf_zero semantics (zero signal, zero gradient).
-    auto accumulate = [&](auto checked) {
-      constexpr bool CHECK = decltype(checked)::value;
-      lsum = 0.f;
+        for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+        Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
+          bool fin = true;
+          float sv = s;
+          if (CHECK) {
+            fin = isfinite(s);
+            sv = fin ? s : 0.f;
+          }
+          const float w = HAS_W ? wr[HAS_W ? r : 0] : 1.f;
+          const float res = sv - yr[r];
+          float l, dl = 0.f;
+          if (L1FAST) {
+            l = w * fabsf(res);
+            amin = fminf(amin, fabsf(res));
+          } else {
+            loss_fn<LOSS>(res, w, l, dl);
+          }
+          lsum += l;
+          if (!CHECK || fin) accum_grad<Model, L1FAST, HAS_W>(L1FAST ? copysignf(w, res) : dl, w, res, J, gth);
+        });
+      };
+      accumulate(std::false_type{});
+      if (!isfinite(lsum) || !(amin > 0.f)) accumulate(std::true_type{});
+      Model::post_grad(th, gth);
+      float den = dead_z2, gn = 0.f, step = 0.f;
+      float g[NP];
 #pragma unroll
-      for (int p = 0; p < NP; ++p) gth[p] = 0.f;
-      Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
-        bool fin = true;
-        float sv = s;
-        if (CHECK) {
-          fin = isfinite(s);
-          sv = fin ? s : 0.f;
+      for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * inv_count;
+      if (norms) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          den += z[p] * z[p];
+          gn += g[p] * g[p];
         }
-        float l, dl;
-        loss_fn<LOSS>(sv - yr[r], HAS_W ? wr[HAS_W ? r : 0] : 1.f, l, dl);
-        lsum += l;
-        if (CHECK) dl = fin ? dl : 0.f;
+      }
+      if (k == V.n_epochs - 1 && V.z_last != nullptr && valid) {
 #pragma unroll
-        for (int p = 0; p < NP; ++p) gth[p] = fmaf(dl, J[p], gth[p]);
-      });
-    };
-    accumulate(std::false_type{});
-    if (!isfinite(lsum)) accumulate(std::true_type{});
-    float den = dead_z2, gn = 0.f, step = 0.f;
-    float g[NP];
+        for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
+      }
+      if (e < V.iteration) {  // update_parameters skips the last iteration (:677)
+        const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2, 0.f};
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      g[p] = gth[p] * dth[p] * inv_count;
-      den += z[p] * z[p];
-      gn += g[p] * g[p];
-    }
-    if (k == V.n_epochs - 1 && V.z_last != nullptr && valid) {
-#pragma unroll
-      for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
-    }
-    if (e < V.iteration) {  // update_parameters skips the last iteration (:677)
-      const EpochScal es = V.es[k];
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        float dz;
-        if (ap.opt == GC_OPT_ADAM) {
-          m1[p] = ap.beta1 * m1[p] + (1.f - ap.beta1) * g[p];
-          m2[p] = ap.beta2 * m2[p] + (1.f - ap.beta2) * g[p] * g[p];
-          if (ap.adam_form == GC_ADAM_KINGMA_BA)
-            dz = -es.lr * (m1[p] * es.c1) * rcpf(sqrtf(m2[p] * es.c2) + ap.eps);
-          else
-            dz = -es.lr * m1[p] * rcpf(sqrtf(m2[p]) + ap.eps);
-        } else if (ap.opt == GC_OPT_SGDM) {
-          m1[p] = ap.momentum * m1[p] - es.lr * g[p];
-          dz = m1[p];
-        } else {
-          m2[p] = ap.rms_decay * m2[p] + (1.f - ap.rms_decay) * g[p] * g[p];
-          dz = -es.lr * g[p] * rcpf(sqrtf(m2[p]) + ap.rms_eps);
+        for (int p = 0; p < NP; ++p) {
+          float dz;
+          if (ap.opt == GC_OPT_ADAM) {
+            m1[p] = ap.beta1 * m1[p] + (1.f - ap.beta1) * g[p];
+            m2[p] = ap.beta2 * m2[p] + (1.f - ap.beta2) * g[p] * g[p];
+            if (ap.adam_form == GC_ADAM_KINGMA_BA)
+              dz = -es.lr * (m1[p] * es.c1) * rcpf(sqrtv(m2[p] * es.c2) + ap.eps);
+            else
+              dz = -es.lr * m1[p] * rcpf(sqrtv(m2[p]) + ap.eps);
+          } else if (ap.opt == GC_OPT_SGDM) {
+            m1[p] = ap.momentum * m1[p] - es.lr * g[p];
+            dz = m1[p];
+          } else {
+            m2[p] = ap.rms_decay * m2[p] + (1.f - ap.rms_decay) * g[p] * g[p];
+            dz = -es.lr * g[p] * rcpf(sqrtv(m2[p]) + ap.rms_eps);
+          }
+          const float zn = z[p] + dz;
+          if (norms) {
+            const float d = zn - z[p];
+            step += d * d;
+          }
+          z[p] = zn;
         }
-        const float zn = z[p] + dz;
-        const float d = zn - z[p];
-        step += d * d;
-        z[p] = zn;
+      }
+      if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
+      const int slot = k & (kRing - 1);
+      if (norms) {
+        const float tot = wave_sum4(lsum, step, den, gn, lane);
+        if (lane < 4) red[slot][wave][wave_sum4_index(lane)] = tot;
+      } else {
+        const float tot = wave_sum(lsum);
+        if (lane < 4) red[slot][wave][lane] = (lane == 0) ? tot : 0.f;
+      }
+      if (slot == kRing - 1 || k == V.n_epochs - 1) {
+        __syncthreads();
+        const int cnt = slot + 1;
+        const int kbase = k - slot;
+        if (threadIdx.x < cnt * 4) {
+          const int kk = threadIdx.x >> 2, j = threadIdx.x & 3;
+          float s = 0.f;
+#pragma unroll
+          for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
+          V.partials[(int64_t(kbase + kk) * 4 + j) * V.nblk + lblk] = s;
+        }
+        __syncthreads();
       }
     }
-    if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
-    const float tot = wave_sum4(lsum, step, den, gn, lane);
-    const int slot = k & (kRing - 1);
-    if (lane < 4) red[slot][wave][wave_sum4_index(lane)] = tot;
-    if (slot == kRing - 1 || k == V.n_epochs - 1) {
-      __syncthreads();
-      const int cnt = slot + 1;
-      const int kbase = k - slot;
-      if (threadIdx.x < cnt * 4) {
-        const int kk = threadIdx.x >> 2, j = threadIdx.x & 3;
-        float s = 0.f;
-#pragma unroll
-        for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
-        V.partials[(int64_t(kbase + kk) * 4 + j) * V.nblk + lblk] = s;
-      }
-      __syncthreads();
-    }
-  }
+  };
+  if (ap.transform == GC_TRANSFORM_LINEAR)
+    epochs(std::integral_constant<int, GC_TRANSFORM_LINEAR>{});
+  else
+    epochs(std::integral_constant<int, GC_TRANSFORM_SIGMOID>{});
+
   if (valid) {
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
@@ -260,11 +338,17 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
 }
 
 // ---- deterministic FP64 reduction: out[j] = sum_b in[j*nb + b] ------------
-__global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb,
+// Row j is split into `chunks` contiguous chunks of `clen` entries; block
+// (j, c) writes the FP64 sum of chunk c to out[j*chunks + c] and the host
+// adds the chunk sums of a row in order, so the result does not depend on
+// timing.
+static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb, int64_t clen,
                                                          double* __restrict__ out) {
-  const int64_t row = blockIdx.x;
+  const int64_t row = blockIdx.y;
+  const int64_t c = blockIdx.x;
+  const int64_t b0 = c * clen, b1 = min(nb, b0 + clen);
   double s = 0.0;
-  for (int64_t b = threadIdx.x; b < nb; b += kBlock) s += double(in[row * nb + b]);
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += kBlock) s += double(in[row * nb + b]);
   s = wave_sum_d(s);
   __shared__ double sh[kBlock / 64];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
@@ -272,11 +356,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict_
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
-    out[row] = t;
+    out[row * gridDim.x + c] = t;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce_rows_d(const double* __restrict__ in, int64_t nb,
+static __global__ __launch_bounds__(kBlock) void k_reduce_rows_d(const double* __restrict__ in, int64_t nb,
                                                            double* __restrict__ out) {
   const int64_t row = blockIdx.x;
   double s = 0.0;
@@ -344,11 +428,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ md
       loss_fn<LOSS>(sv - ep.y[m * nv + vv], wv, l, dl);
       lvox += l;
       if (ep.residual != nullptr && valid) ep.residual[m * nv + v] = l;
-      if (fin) {
-#pragma unroll
-        for (int p = 0; p < NP; ++p) gth[p] += dl * J[p];
-      }
+      if (fin) accum_grad<Model, false, HAS_W>(dl, wv, sv - ep.y[m * nv + vv], J, gth);
     });
+    Model::post_grad(th, gth);
     lsum = valid ? double(lvox) : 0.0;
     if (valid && ep.resloss != nullptr) ep.resloss[v] = lvox / float(md.n_meas);
     if (valid && ep.grad != nullptr) {
@@ -751,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
 }
 
 // ---- layout / transform kernels ---------------------------------------------
-__global__ void k_nd_to_gd(const float* __restrict__ nd, int64_t nvox, int32_t M, const int64_t* __restrict__ vox,
+static __global__ void k_nd_to_gd(const float* __restrict__ nd, int64_t nvox, int32_t M, const int64_t* __restrict__ vox,
                            int64_t nv, float* __restrict__ gd) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= nv * M) return;
@@ -759,17 +841,17 @@ __global__ void k_nd_to_gd(const float* __restrict__ nd, int64_t nvox, int32_t M
   gd[i] = nd[vox[v] + nvox * m];
 }
 
-__global__ void k_clamp01(float* __restrict__ p, int64_t n) {
+static __global__ void k_clamp01(float* __restrict__ p, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = fminf(fmaxf(p[i], 0.f), 1.f);
 }
 
-__global__ void k_fill(float* __restrict__ p, int64_t n, float val) {
+static __global__ void k_fill(float* __restrict__ p, int64_t n, float val) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = val;
 }
 
-__global__ void k_gd_to_nd(const float* __restrict__ gd, int64_t nv, int32_t M, const int64_t* __restrict__ vox,
+static __global__ void k_gd_to_nd(const float* __restrict__ gd, int64_t nv, int32_t M, const int64_t* __restrict__ vox,
                            int64_t nvox, float* __restrict__ nd) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= nv * M) return;
@@ -778,7 +860,7 @@ __global__ void k_gd_to_nd(const float* __restrict__ gd, int64_t nv, int32_t M, 
 }
 
 // theta -> z (askadam.initialise_parameter / rescale_parameters, :368-412, :1077-1089)
-__global__ void k_rescale(const gc_model* __restrict__ mdp, int32_t transform, const float* __restrict__ th,
+static __global__ void k_rescale(const gc_model* __restrict__ mdp, int32_t transform, const float* __restrict__ th,
                           float* __restrict__ z, int64_t nv) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const gc_model& md = *mdp;
@@ -799,7 +881,7 @@ __global__ void k_rescale(const gc_model* __restrict__ mdp, int32_t transform, c
 }
 
 // z -> theta (unscale_parameters, :1063-1073)
-__global__ void k_unscale(const gc_model* __restrict__ mdp, int32_t transform, const float* __restrict__ z,
+static __global__ void k_unscale(const gc_model* __restrict__ mdp, int32_t transform, const float* __restrict__ z,
                           float* __restrict__ th, int64_t nv) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const gc_model& md = *mdp;
@@ -815,7 +897,7 @@ __global__ void k_unscale(const gc_model* __restrict__ mdp, int32_t transform, c
 // One thread per grid voxel; masked voxels add their forward-difference
 // terms and scatter-free gather the adjoint: the gradient at voxel i is
 // sum_d (1/v_d)[m(i-e_d) sgn(D_d(i-e_d)) - m(i) sgn(D_d(i))] * lambda/N.
-__global__ __launch_bounds__(kBlock) void k_tv(const float* __restrict__ img, const uint8_t* __restrict__ mask,
+static __global__ __launch_bounds__(kBlock) void k_tv(const float* __restrict__ img, const uint8_t* __restrict__ mask,
                                                 int32_t X, int32_t Y, int32_t Z, int32_t nd, float ivx, float ivy,
                                                 float ivz, float scale, double* __restrict__ partial,
                                                 float* __restrict__ grad) {

@@ -28,6 +28,7 @@ namespace gc {
 template <int MT, bool EXACT = false, bool UNI = false>
 struct R2sModel {
   static constexpr int NP = 2, MR = MT, NE = 0;
+  static constexpr bool kRelJ = true;  // J[1] is relative to J[0] (see accum_grad)
   GC_DEV static bool valid(const gc_model& md, int r) { return EXACT ? (r < MT) : (r < md.i[0]); }
   GC_DEV static int meas(const gc_model&, int r) { return r; }
   template <bool JAC, class F>
@@ -54,13 +55,15 @@ struct R2sModel {
         const float s = m0 * et;
         float J[NP];
         if (JAC) {
-          J[0] = et;
-          J[1] = -te * s;
+          J[0] = et;   // d s / d M0 >= 0
+          J[1] = -te;  // (d s / d R2*) / J[0] without the M0 factor (post_grad)
         }
         f(t, s, J);
       }
     }
   }
+  // the M0 factor common to every echo's d s / d R2*, applied once
+  GC_DEV static void post_grad(const float* th, float* g) { g[1] *= th[0]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -71,6 +74,8 @@ struct R2sModel {
 template <int MT, int MF>
 struct JointModel {
   static constexpr int NP = 3, MR = MT * MF, NE = 1;
+  static constexpr bool kRelJ = false;
+  GC_DEV static void post_grad(const float*, float*) {}
   GC_DEV static bool valid(const gc_model& md, int r) { return (r % MT) < md.i[0] && (r / MT) < md.i[1]; }
   GC_DEV static int meas(const gc_model& md, int r) { return (r % MT) + md.i[0] * (r / MT); }
   template <bool JAC, class F>
@@ -121,6 +126,8 @@ struct JointModel {
 template <int MS, bool L2>
 struct NexiModel {
   static constexpr int NP = L2 ? 5 : 4, MR = L2 ? 2 * MS : MS, NE = 1;
+  static constexpr bool kRelJ = false;
+  GC_DEV static void post_grad(const float*, float*) {}
   GC_DEV static bool valid(const gc_model& md, int r) { return (r % MS) < md.i[0]; }
   GC_DEV static int meas(const gc_model& md, int r) { return (r < MS) ? r : md.i[0] + (r - MS); }
 

@@ -1,0 +1,86 @@
+// gc_ops_impl.hpp -- the model-dependent operations behind ModelOps, as
+// templates over the register-tiled model type.  Included only by the
+// ops_*.hip translation units, each of which instantiates one model.
+#pragma once
+#include "gc_askadam.hpp"
+
+namespace gc {
+
+template <class Model>
+struct OpsImpl {
+  static int fwd(gc_ctx* ctx, const gc_model* dmd, const float* theta, const float* extra, float* s, int64_t nv) {
+    EvalParams ep{};
+    ep.x = theta;
+    ep.ex = extra;
+    ep.s_out = s;
+    ep.nv = nv;
+    ep.transform = GC_TRANSFORM_NONE;
+    ep.mode = 0;
+    hipLaunchKernelGGL((k_eval<Model, GC_LOSS_L2, false>), dim3(nblocks(nv)), dim3(kBlock), 0, ctx->stream, dmd, ep);
+    GC_HIP(ctx, hipGetLastError());
+    return GC_OK;
+  }
+
+  static int fwd_adj(gc_ctx* ctx, const gc_model* dmd, int loss, int transform, const float* x, const float* y,
+                     const float* w, const float* extra, int64_t nv, float scale, float* grad, double* part,
+                     float* residual) {
+    return with_loss(ctx, loss, [&](auto lc) -> int {
+      constexpr int LOSS = decltype(lc)::value;
+      return with_bool(w != nullptr, [&](auto hw) -> int {
+        constexpr bool HW = decltype(hw)::value;
+        EvalParams ep{};
+        ep.x = x;
+        ep.y = y;
+        ep.w = w;
+        ep.ex = extra;
+        ep.grad = grad;
+        ep.residual = residual;
+        ep.partial = part;
+        ep.nv = nv;
+        ep.transform = transform;
+        ep.mode = 1;
+        ep.scale = scale;
+        hipLaunchKernelGGL((k_eval<Model, LOSS, HW>), dim3(nblocks(nv)), dim3(kBlock), 0, ctx->stream, dmd, ep);
+        GC_HIP(ctx, hipGetLastError());
+        return GC_OK;
+      });
+    });
+  }
+
+  static int askadam(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol) {
+    AskDriver drv(ctx, *md, *cfg, vols, n_vol);
+    int rc = drv.setup();
+    if (rc) return rc;
+    return with_loss(ctx, cfg->loss, [&](auto lc) -> int {
+      constexpr int LOSS = decltype(lc)::value;
+      return with_bool(drv.has_w, [&](auto hw) -> int {
+        constexpr bool HW = decltype(hw)::value;
+        int r = drv.template run_all<Model, LOSS, HW>();
+        if (r) return r;
+        return drv.template finalize<Model, LOSS, HW>();
+      });
+    });
+  }
+
+  static int mcmc(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
+                  hipEvent_t ev0, hipEvent_t ev1) {
+    return with_bool(hw, [&](auto hwc) -> int {
+      constexpr bool HW = decltype(hwc)::value;
+      if (gw)
+        hipExtLaunchKernelGGL((k_gw<Model, HW>), dim3(grid), dim3(kBlock), 0, ctx->stream, ev0, ev1, 0, dmd, P);
+      else
+        hipExtLaunchKernelGGL((k_mh<Model, HW>), dim3(grid), dim3(kBlock), 0, ctx->stream, ev0, ev1, 0, dmd, P);
+      GC_HIP(ctx, hipGetLastError());
+      return GC_OK;
+    });
+  }
+};
+
+template <class Model>
+const ModelOps* ops_table() {
+  static const ModelOps t{&OpsImpl<Model>::fwd, &OpsImpl<Model>::fwd_adj, &OpsImpl<Model>::askadam,
+                          &OpsImpl<Model>::mcmc};
+  return &t;
+}
+
+}  // namespace gc

@@ -19,6 +19,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -34,6 +35,21 @@ PEAK_FP32_TFLOPS = 157.3         # MI355X published FP32 vector (packed)
 PEAK_FP32_NONPACKED_TFLOPS = 78.6
 PEAK_TRANS_PER_S = 9.8e12        # quarter-rate transcendental issue
 PEAK_HBM_GBS = 8000.0
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "k_askadam_r2star12_5873183vox_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_askadam launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy)."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", text)
+    return int(m.group(1)) if m else None
 
 
 def log(*a):
@@ -324,6 +340,7 @@ def main():
             cpu["mh_sample"] = f"MH {args.mh_chains}x{args.mh_iterations} on {ns_mh} voxels: {t_cmh:.1f} s"
             log(f"CPU MH: {t_cmh:.2f} s for {ns_mh} voxels")
 
+    traffic = measured_traffic() if tuple(args.dims) == (256, 256, 128) and args.iterations == 2000 else None
     out = {
         "metric": "voxels fitted/s (askadam) + MCMC samples/s vs plain GPU path at equal error",
         "value": vox_per_s,
@@ -350,7 +367,13 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, os.path.dirname(os.path.abspath(__file__)))
+            if traffic is not None else None,
+            "hbm_frac": (traffic / (k_ms / max(launches, 1) * 1e-3) / (PEAK_HBM_GBS * 1e9))
+            if traffic is not None else None,
+            "bound_note": "FP32 vector ALU and transcendental issue: no dense contraction on this path (no MFMA) "
+                          "and O(1) HBM bytes per voxel per launch; peak = published FP32 157.3 TFLOP/s",
             "kernel": "k_askadam (fused forward+adjoint+Adam, K iterations per launch)",
             "kernel_ms_total": k_ms,
             "launches": launches,

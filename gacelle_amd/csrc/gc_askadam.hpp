@@ -79,16 +79,22 @@ struct AskDriver {
   // add the chunk sums of every row in a fixed order (after a stream sync)
   void fold_sums(const std::vector<int>& n_ep) {
     const double* h = h_sums.as<double>();
+    const int npv = npv_count();
     for (int i = 0; i < n_vol; ++i) {
       const int C = run[i].chunks;
-      for (int r = 0; r < n_ep[i] * 4; ++r) {
-        const double* src = h + (size_t(i) * K * 4 + r) * max_chunks;
+      double* dst = row_sums.data() + size_t(i) * K * 4;
+      std::fill(dst, dst + size_t(K) * 4, 0.0);
+      for (int r = 0; r < n_ep[i] * npv; ++r) {
+        const double* src = h + size_t(i) * K * 4 * max_chunks + size_t(r) * C;
         double t = 0.0;
         for (int c = 0; c < C; ++c) t += src[c];
-        row_sums[size_t(i) * K * 4 + r] = t;
+        dst[(r / npv) * 4 + (r % npv)] = t;
       }
     }
   }
+
+  // partial sums per iteration: loss, step^2, |z|^2, |g|^2, or the loss alone
+  int npv_count() const { return (cfg.step_tol > 0 || cfg.grad_tol > 0) ? 4 : 1; }
 
   AskDriver(gc_ctx* c, const gc_model& m, const gc_askadam_cfg& a, gc_volume* v, int n)
       : ctx(c), md(m), cfg(a), vols(v), n_vol(n), run(n) {}
@@ -121,7 +127,7 @@ struct AskDriver {
       const double k = 2.5e8 / double(std::max<int64_t>(nv_total, 1));
       K = int(std::min(256.0, std::max(16.0, k)));
     } else {
-      K = std::min(k_mem, 8192);  // 4 partial rows per iteration on grid.y
+      K = std::min(k_mem, 8192);
     }
     K = std::max(1, std::min(K, std::max(1, cfg.iteration)));
     if (K > 16383) return set_err(ctx, GC_ERR_ARG, "iterationsPerLaunch %d exceeds 16383", K);
@@ -231,7 +237,7 @@ struct AskDriver {
     ap.opt = cfg.optimiser;
     ap.transform = cfg.transform;
     ap.adam_form = cfg.adam_form;
-    ap.need_norms = (cfg.step_tol > 0 || cfg.grad_tol > 0) ? 1 : 0;
+    ap.need_norms = npv_count() == 4 ? 1 : 0;
     ap.beta1 = float(cfg.beta1);
     ap.beta2 = float(cfg.beta2);
     ap.eps = float(cfg.eps);
@@ -254,8 +260,9 @@ struct AskDriver {
     ++kernel_launches;
     for (int i = 0; i < n_vol; ++i) {
       if (n_ep[i] <= 0) continue;
-      hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, n_ep[i] * 4), dim3(kBlock), 0, ctx->stream,
-                         run[i].partials.as<float>(), int64_t(run[i].nblk), int64_t(kReduceChunk),
+      const int nrows = n_ep[i] * npv_count();
+      hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, (nrows + 63) / 64), dim3(kBlock), 0, ctx->stream,
+                         run[i].partials.as<float>(), int64_t(run[i].nblk), nrows, int64_t(kReduceChunk),
                          d_sums.as<double>() + size_t(i) * K * 4 * max_chunks);
       GC_HIP(ctx, hipGetLastError());
     }

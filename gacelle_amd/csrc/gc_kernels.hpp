@@ -30,7 +30,7 @@ GC_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 // rounded expansion
 GC_DEV float sqrtv(float x) { return __builtin_amdgcn_sqrtf(x); }
 constexpr int kRing = 16;  // iterations buffered in LDS before a partial flush
-constexpr int kReduceChunk = 4096;  // block partials per k_reduce_rows block
+constexpr int kReduceChunk = 1024;  // block partials per k_reduce_rows block
 
 // ---- loss functions of askadam.model_gradient (utils/askadam.m:86-95) -----
 template <int LOSS>
@@ -121,7 +121,7 @@ struct VolDev {
   float* m_out;
   float* v_out;
   float* z_last;     // z (pre-update) at the launch's last iteration, or null
-  float* partials;   // [K][4][nblk]
+  float* partials;   // [nblk][n_epochs][npv], npv = 4 or 1: a block's sums are contiguous
   const EpochScal* es;
   int64_t nv;
   int32_t blk0, nblk;
@@ -304,12 +304,14 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
         __syncthreads();
         const int cnt = slot + 1;
         const int kbase = k - slot;
-        if (threadIdx.x < cnt * 4) {
-          const int kk = threadIdx.x >> 2, j = threadIdx.x & 3;
+        // npv sums per iteration: 4 with the norms, else the loss alone
+        const int lg = norms ? 2 : 0;
+        if (threadIdx.x < (cnt << lg)) {
+          const int kk = threadIdx.x >> lg, j = threadIdx.x & ((1 << lg) - 1);
           float s = 0.f;
 #pragma unroll
           for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
-          V.partials[(int64_t(kbase + kk) * 4 + j) * V.nblk + lblk] = s;
+          V.partials[((int64_t(lblk) * V.n_epochs + kbase + kk) << lg) + j] = s;
         }
         __syncthreads();
       }
@@ -337,26 +339,28 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
   }
 }
 
-// ---- deterministic FP64 reduction: out[j] = sum_b in[j*nb + b] ------------
-// Row j is split into `chunks` contiguous chunks of `clen` entries; block
-// (j, c) writes the FP64 sum of chunk c to out[j*chunks + c] and the host
-// adds the chunk sums of a row in order, so the result does not depend on
-// timing.
-static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb, int64_t clen,
-                                                         double* __restrict__ out) {
-  const int64_t row = blockIdx.y;
-  const int64_t c = blockIdx.x;
-  const int64_t b0 = c * clen, b1 = min(nb, b0 + clen);
+// ---- deterministic FP64 reduction of block partials ------------------------
+// in is [nb][nrows] (each block's row sums contiguous).  The blocks are split
+// into `gridDim.x` chunks of `clen`; CUDA block (c, t) sums rows
+// 64t..64t+63 over chunk c with 4 lanes per row (coalesced 256-byte reads)
+// and writes out[row * gridDim.x + c].  The host adds the chunk sums of a
+// row in order, so the result does not depend on timing.
+static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb,
+                                                                int32_t nrows, int64_t clen,
+                                                                double* __restrict__ out) {
+  const int c = blockIdx.x;
+  const int row = blockIdx.y * 64 + (threadIdx.x & 63);
+  const int sub = threadIdx.x >> 6;
+  const int64_t b0 = int64_t(c) * clen, b1 = min(nb, b0 + clen);
   double s = 0.0;
-  for (int64_t b = b0 + threadIdx.x; b < b1; b += kBlock) s += double(in[row * nb + b]);
-  s = wave_sum_d(s);
-  __shared__ double sh[kBlock / 64];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  if (row < nrows)
+    for (int64_t b = b0 + sub; b < b1; b += 4) s += double(in[b * nrows + row]);
+  __shared__ double sh[4][64];
+  sh[sub][threadIdx.x & 63] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
-    out[row * gridDim.x + c] = t;
+  if (sub == 0 && row < nrows) {
+    const int l = threadIdx.x;
+    out[int64_t(row) * gridDim.x + c] = ((sh[0][l] + sh[1][l]) + sh[2][l]) + sh[3][l];
   }
 }
 

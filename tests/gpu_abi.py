@@ -125,3 +125,19 @@ def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partne
     return dict(posterior=None if post is None else runtime.to_host(post), mean=runtime.to_host(mean),
                 std=runtime.to_host(std), accept=runtime.to_host(acc), xfinal=runtime.to_host(xf),
                 trace=None if tr is None else runtime.to_host(tr), n_sample=io.n_sample, n_stored=io.n_stored)
+
+
+def tv(img, mask, mode, vsize, lam, nmasked):
+    """gc_tv_fwd_adj on a column-major [x, y, z] grid: (loss, grad)."""
+    c = ctx()
+    img = np.asfortranarray(img, np.float32)
+    dims3 = (list(img.shape) + [1, 1, 1])[:3]
+    imgd = c.to_device(img.ravel(order="F"))
+    maskd = c.to_device(np.asfortranarray(mask, np.uint8).ravel(order="F"), np.uint8)
+    g = c.zeros((img.size,))
+    dims = (ctypes.c_int32 * 3)(*[int(d) for d in dims3])
+    vs = (ctypes.c_double * 3)(*[float(v) for v in vsize])
+    loss = ctypes.c_double()
+    c.check(c.lib.gc_tv_fwd_adj(c.handle, runtime.ptr(imgd), runtime.ptr(maskd), dims, mode, vs, float(lam),
+                                int(nmasked), ctypes.byref(loss), runtime.ptr(g)))
+    return loss.value, runtime.to_host(g).reshape(dims3, order="F")

@@ -71,3 +71,39 @@ def mcmc_cfg(algo, iteration, thinning, burnin, rep=1, W=1, seed=48463, replay=F
     for q, s in enumerate(steps):
         c.x_step[q] = s
     return c
+
+
+def replay_loss_stopping(trace, f):
+    """utils/askadam.m:512-671 restated on a loss trace (loss signals only):
+    the iteration at which the reference loop would stop.  `f` is a fitting
+    dict, or a gc_askadam_cfg."""
+    if isinstance(f, _abi.gc_askadam_cfg):
+        f = dict(convergenceWindow=f.convergence_window, emaDecay=f.ema_decay, convergenceValue=f.convergence_value,
+                 patienceConvergence=f.patience_conv, tol=f.tol,
+                 convergenceModel="ema" if f.convergence_model == _abi.GC_CONV_EMA else "linear")
+    ema = trace[0]
+    buf = np.ones(int(f["convergenceWindow"]))
+    min_loss = trace[0]
+    pc = 0
+    for e in range(1, len(trace)):
+        loss = trace[e]
+        min_loss = min(min_loss, loss)
+        if f["convergenceModel"] == "ema":
+            en = f["emaDecay"] * ema + (1 - f["emaDecay"]) * loss
+            conv = abs(en - ema) / (abs(ema) + 1e-8)
+            ema = en
+        else:
+            buf = np.append(buf[1:], loss)
+            x = np.arange(1, buf.size + 1)
+            conv = -np.polyfit(x, buf, 1)[0]
+        if conv > f["convergenceValue"] or e <= f["convergenceWindow"]:
+            pc = 0
+        elif (min_loss - loss) > f["convergenceValue"]:
+            pc = 0
+        else:
+            pc += 1
+        if pc > f["patienceConvergence"]:
+            return e, _abi.GC_STOP_CONVERGENCE
+        if loss < f["tol"]:
+            return e, _abi.GC_STOP_TOL
+    return len(trace) - 1, _abi.GC_STOP_NONE

@@ -8,7 +8,8 @@ from gacelle_amd.askadam import askadam
 from oracle import oracle as orc
 
 from . import gpu_abi
-from .helpers import FA_JOINT, TE12, TE_JOINT, TR_JOINT, logit_start, no_stop, r2s_problem, r2s_setup
+from .helpers import (FA_JOINT, TE12, TE_JOINT, TR_JOINT, logit_start, no_stop, r2s_problem, r2s_setup,
+                      replay_loss_stopping)
 
 pytestmark = pytest.mark.gpu
 
@@ -159,37 +160,6 @@ def test_askadam_optimisers_short(opt, form):
     np.testing.assert_allclose(got["loss"], ref["loss"], rtol=1e-4)
 
 
-def _replay_loss_stopping(trace, f):
-    """utils/askadam.m:512-671 restated on a loss trace (loss signals only):
-    the iteration at which the reference loop would stop."""
-    ema = trace[0]
-    buf = np.ones(int(f["convergenceWindow"]))
-    min_loss = trace[0]
-    pc = 0
-    for e in range(1, len(trace)):
-        loss = trace[e]
-        min_loss = min(min_loss, loss)
-        if f["convergenceModel"] == "ema":
-            en = f["emaDecay"] * ema + (1 - f["emaDecay"]) * loss
-            conv = abs(en - ema) / (abs(ema) + 1e-8)
-            ema = en
-        else:
-            buf = np.append(buf[1:], loss)
-            x = np.arange(1, buf.size + 1)
-            conv = -np.polyfit(x, buf, 1)[0]
-        if conv > f["convergenceValue"] or e <= f["convergenceWindow"]:
-            pc = 0
-        elif (min_loss - loss) > f["convergenceValue"]:
-            pc = 0
-        else:
-            pc += 1
-        if pc > f["patienceConvergence"]:
-            return e, _abi.GC_STOP_CONVERGENCE
-        if loss < f["tol"]:
-            return e, _abi.GC_STOP_TOL
-    return len(trace) - 1, _abi.GC_STOP_NONE
-
-
 @pytest.mark.parametrize("model", ["ema", "linear"])
 def test_askadam_loss_stopping(model):
     """Loss-convergence stopping: the device run stops exactly where the
@@ -203,7 +173,7 @@ def test_askadam_loss_stopping(model):
     z0 = logit_start(np.stack([np.ones(nv), np.full(nv, 30.0)]), md)
     got = gpu_abi.askadam(md, cfg, [dict(y=y, z0=z0)], trace=True)[0]
     ref = orc.askadam(md, cfg, y, z0)
-    e, reason = _replay_loss_stopping(got["trace"], f)
+    e, reason = replay_loss_stopping(got["trace"], f)
     assert (got["n_iteration"], got["stop_reason"]) == (e, reason)
     assert got["stop_reason"] == ref["stop_reason"] != _abi.GC_STOP_NONE
     assert abs(got["n_iteration"] - ref["n_iteration"]) <= max(3, 0.03 * ref["n_iteration"])

@@ -269,8 +269,10 @@ class askadam:
                 raise GacelleError(_abi.GC_ERR_ARG, "fitting.regmap must name one map per lambda")
             c.n_reg = len(lam)
             names = list(fitting["modelParams"])
+            slots = dict(model.param_slots(fitting)) if model is not None else None
             for k, name in enumerate(regmap[: len(lam)]):
-                c.regmap[k] = names.index(name)
+                # the parameter slot of the map (multi-slot maps precede it)
+                c.regmap[k] = slots[name].start if slots is not None else names.index(name)
                 c.lambda_[k] = lam[k]
         c.robust = int(bool(fitting["robustConvergence"]))
         c.lr0 = float(fitting["initialLearnRate"])

@@ -39,6 +39,14 @@ int select_ops(gc_ctx* ctx, const gc_model& md, const ModelOps** out) {
       *out = (md.i[1] == 2) ? ops_nexi16_l2() : ops_nexi16();
       return GC_OK;
     }
+    case GC_MODEL_MCRMWI: {
+      if (md.i[1] != 4)
+        return set_err(ctx, GC_ERR_UNSUPPORTED, "MCRMWI: 4 flip angles supported (got %d)", md.i[1]);
+      const bool ex = (md.flags & GC_FLAG_MCRMWI_EXCHANGE) != 0, cx = (md.flags & GC_FLAG_MCRMWI_COMPLEX) != 0;
+      *out = ex ? (cx ? ops_mcrmwi4_exch_cplx() : ops_mcrmwi4_exch_mag())
+                : (cx ? ops_mcrmwi4_cplx() : ops_mcrmwi4_mag());
+      return GC_OK;
+    }
     default:
       return set_err(ctx, GC_ERR_UNSUPPORTED, "model %d is not supported by this entry point", md.model_id);
   }
@@ -208,8 +216,6 @@ int gc_askadam_run(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, g
   if (rc) return rc;
   if (!cfg || !vols || n_vol < 1 || n_vol > GC_MAX_VOL) return set_err(ctx, GC_ERR_ARG, "gc_askadam_run: bad argument");
   if (cfg->robust) return set_err(ctx, GC_ERR_UNSUPPORTED, "robustConvergence is not supported");
-  if (cfg->n_reg > 0 && cfg->lambda[0] > 0)
-    return set_err(ctx, GC_ERR_UNSUPPORTED, "TV regularisation needs the per-iteration path (not available for this model)");
   if (cfg->optimiser < 0 || cfg->optimiser > 2) return set_err(ctx, GC_ERR_ARG, "unknown optimiser");
   if (cfg->transform != GC_TRANSFORM_SIGMOID && cfg->transform != GC_TRANSFORM_LINEAR)
     return set_err(ctx, GC_ERR_ARG, "unknown parameter transform");

@@ -4,6 +4,12 @@
 // fused kernel produces K iterations at a time.  A stop or a new minimum that
 // falls inside a block of K iterations is resolved exactly by replaying that
 // volume from the block's starting state, which is kept (state sets).
+//
+// Two launch forms share that logic.  The blocked form runs K iterations in
+// one k_askadam launch with each voxel's data and state in registers.  The
+// step form runs one k_askadam_step launch per iteration (state ping-pong
+// through HBM): it carries the spatial TV regulariser, whose stencil needs
+// the neighbours' previous parameters, and models whose data are streamed.
 #pragma once
 #include "gc_host.hpp"
 
@@ -37,7 +43,13 @@ struct VolRun {
   int final_set = -1, final_e0 = 0, final_r = 0;
   int launches = 0;
   int chunks = 1;       // chunks per partial-sum row (k_reduce_rows)
+  DevBuf pp[2][3];      // step form: intermediate states of a launch sequence
+  DevBuf nbr;           // step form with TV: [2*nd][nv] grid neighbours
 };
+
+// per-iteration sums kept for the host loop: fidelity, step^2, |z|^2, |g|^2,
+// regularisation
+constexpr int kSum = 5;
 
 struct AskDriver {
   gc_ctx* ctx;
@@ -53,8 +65,12 @@ struct AskDriver {
   DevBuf d_vols, d_es, d_sums;
   HostBuf h_vols, h_es, h_sums;
   int max_chunks = 1;
-  std::vector<double> row_sums;  // [vol][K*4] after fold_sums
+  std::vector<double> row_sums;  // [vol][K*kSum] after fold_sums
   bool desc_in_flight = false;  // an async copy may still read h_vols/h_es
+  bool step_mode = false;       // one launch per iteration (TV or streamed model)
+  bool use_tv = false;
+  DevBuf d_steps;               // step form: [K][n_vol] descriptors
+  HostBuf h_steps;
   // start/stop events attached to every fused launch (kernel-only timing),
   // created once and reused
   std::vector<hipEvent_t> ev_start, ev_stop;
@@ -82,19 +98,20 @@ struct AskDriver {
     const int npv = npv_count();
     for (int i = 0; i < n_vol; ++i) {
       const int C = run[i].chunks;
-      double* dst = row_sums.data() + size_t(i) * K * 4;
-      std::fill(dst, dst + size_t(K) * 4, 0.0);
+      double* dst = row_sums.data() + size_t(i) * K * kSum;
+      std::fill(dst, dst + size_t(K) * kSum, 0.0);
       for (int r = 0; r < n_ep[i] * npv; ++r) {
-        const double* src = h + size_t(i) * K * 4 * max_chunks + size_t(r) * C;
+        const double* src = h + size_t(i) * K * kSum * max_chunks + size_t(r) * C;
         double t = 0.0;
         for (int c = 0; c < C; ++c) t += src[c];
-        dst[(r / npv) * 4 + (r % npv)] = t;
+        dst[(r / npv) * kSum + (r % npv)] = t;
       }
     }
   }
 
-  // partial sums per iteration: loss, step^2, |z|^2, |g|^2, or the loss alone
-  int npv_count() const { return (cfg.step_tol > 0 || cfg.grad_tol > 0) ? 4 : 1; }
+  // partial sums per iteration: all five in the step form, else loss,
+  // step^2, |z|^2, |g|^2 or the loss alone
+  int npv_count() const { return step_mode ? kStepSums : ((cfg.step_tol > 0 || cfg.grad_tol > 0) ? 4 : 1); }
 
   AskDriver(gc_ctx* c, const gc_model& m, const gc_askadam_cfg& a, gc_volume* v, int n)
       : ctx(c), md(m), cfg(a), vols(v), n_vol(n), run(n) {}
@@ -103,7 +120,19 @@ struct AskDriver {
     return cfg.tol > 0 || cfg.convergence_value >= 0 || cfg.step_tol > 0 || cfg.grad_tol > 0;
   }
 
-  int setup() {
+  // stream_model: the model's data do not fit in registers (step form only)
+  int setup(bool stream_model) {
+    use_tv = cfg.n_reg > 0 && cfg.lambda[0] > 0;
+    step_mode = use_tv || stream_model;
+    if (use_tv) {
+      if (cfg.n_reg > GC_MAX_REG) return set_err(ctx, GC_ERR_ARG, "at most %d regularised maps", GC_MAX_REG);
+      if (cfg.tv_mode != GC_TV_2D && cfg.tv_mode != GC_TV_3D) return set_err(ctx, GC_ERR_ARG, "unknown TVmode");
+      for (int k = 0; k < cfg.n_reg; ++k)
+        if (cfg.regmap[k] < 0 || cfg.regmap[k] >= model_np(md))
+          return set_err(ctx, GC_ERR_ARG, "regmap %d names slot %d, not a model parameter", k, cfg.regmap[k]);
+      for (int d = 0; d < 3; ++d)
+        if (!(cfg.voxel_size[d] > 0)) return set_err(ctx, GC_ERR_ARG, "voxelSize must be positive");
+    }
     int rcm = upload_model(ctx, &md, &dmd);
     if (rcm) return rcm;
     int64_t nv_total = 0;
@@ -121,7 +150,9 @@ struct AskDriver {
     // (16 bytes per block and iteration) allow within 1 GiB.
     int64_t nblk_total = 0;
     for (int i = 0; i < n_vol; ++i) nblk_total += (vols[i].nv + kBlock - 1) / kBlock;
-    const int k_mem = int(std::max<int64_t>(16, (int64_t(1) << 30) / (16 * std::max<int64_t>(nblk_total, 1))));
+    const int64_t row_bytes = 4 * int64_t(step_mode ? kStepSums : 4);
+    const int k_mem =
+        int(std::max<int64_t>(16, (int64_t(1) << 30) / (row_bytes * std::max<int64_t>(nblk_total, 1))));
     if (cfg.block_iters > 0) K = cfg.block_iters;
     else if (stopping_possible()) {
       const double k = 2.5e8 / double(std::max<int64_t>(nv_total, 1));
@@ -141,7 +172,14 @@ struct AskDriver {
       for (int s = 0; s < 3; ++s)
         for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.set[s][j].alloc(pbytes));
       for (int s = 0; s < 2; ++s) GC_HIP(ctx, R.zlast[s].alloc(pbytes));
-      GC_HIP(ctx, R.partials.alloc(size_t(K) * 4 * R.nblk * sizeof(float)));
+      GC_HIP(ctx, R.partials.alloc(size_t(K) * kStepSums * R.nblk * sizeof(float)));
+      if (step_mode)
+        for (int s = 0; s < 2; ++s)
+          for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.pp[s][j].alloc(pbytes));
+      if (use_tv) {
+        const int rc = build_neighbours(i);
+        if (rc) return rc;
+      }
       GC_HIP(ctx, hipMemcpyAsync(R.set[0][0].p, vols[i].z_dev, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
       GC_HIP(ctx, hipMemsetAsync(R.set[0][1].p, 0, pbytes, ctx->stream));
       GC_HIP(ctx, hipMemsetAsync(R.set[0][2].p, 0, pbytes, ctx->stream));
@@ -164,9 +202,47 @@ struct AskDriver {
     GC_HIP(ctx, h_vols.alloc(sizeof(VolDev) * n_vol));
     GC_HIP(ctx, d_es.alloc(sizeof(EpochScal) * K * n_vol));
     GC_HIP(ctx, h_es.alloc(sizeof(EpochScal) * K * n_vol));
-    GC_HIP(ctx, d_sums.alloc(sizeof(double) * K * 4 * max_chunks * n_vol));
-    GC_HIP(ctx, h_sums.alloc(sizeof(double) * K * 4 * max_chunks * n_vol));
-    row_sums.assign(size_t(K) * 4 * n_vol, 0.0);
+    GC_HIP(ctx, d_sums.alloc(sizeof(double) * K * kSum * max_chunks * n_vol));
+    GC_HIP(ctx, h_sums.alloc(sizeof(double) * K * kSum * max_chunks * n_vol));
+    row_sums.assign(size_t(K) * kSum * n_vol, 0.0);
+    if (step_mode) {
+      GC_HIP(ctx, d_steps.alloc(sizeof(StepVol) * K * n_vol));
+      GC_HIP(ctx, h_steps.alloc(sizeof(StepVol) * K * n_vol));
+    }
+    return GC_OK;
+  }
+
+  // Grid neighbours of a volume's masked voxels for the TV stencil.  The
+  // voxel indices are checked on the host first: they index the grid.
+  int build_neighbours(int i) {
+    const gc_volume& v = vols[i];
+    const int64_t nv = v.nv;
+    const int64_t X = v.dims[0], Y = v.dims[1], Z = v.dims[2];
+    if (!v.vox_dev || X < 1 || Y < 1 || Z < 1)
+      return set_err(ctx, GC_ERR_ARG, "volume %d: TV needs vox_dev and dims", i);
+    const int64_t ngrid = X * Y * Z;
+    if (ngrid >= (int64_t(1) << 31) || nv > ngrid)
+      return set_err(ctx, GC_ERR_ARG, "volume %d: grid of %lld voxels unsupported", i, (long long)ngrid);
+    std::vector<int64_t> hv(nv);
+    GC_HIP(ctx, hipMemcpyAsync(hv.data(), v.vox_dev, sizeof(int64_t) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t k = 0; k < nv; ++k)
+      if (hv[k] < 0 || hv[k] >= ngrid || (k > 0 && hv[k] <= hv[k - 1]))
+        return set_err(ctx, GC_ERR_ARG, "volume %d: vox must be increasing grid indices (entry %lld)", i,
+                       (long long)k);
+    const int nd = (cfg.tv_mode == GC_TV_3D) ? 3 : 2;
+    DevBuf inv;
+    GC_HIP(ctx, inv.alloc(sizeof(int32_t) * ngrid));
+    GC_HIP(ctx, run[i].nbr.alloc(sizeof(int32_t) * 2 * nd * nv));
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(ngrid)), dim3(kBlock), 0, ctx->stream, inv.as<int32_t>(), ngrid,
+                       int32_t(-1));
+    hipLaunchKernelGGL(k_scatter_index, dim3(nblocks(nv)), dim3(kBlock), 0, ctx->stream, v.vox_dev, nv,
+                       inv.as<int32_t>());
+    hipLaunchKernelGGL(k_neighbours, dim3(nblocks(nv)), dim3(kBlock), 0, ctx->stream, v.vox_dev, nv,
+                       inv.as<int32_t>(), int32_t(X), int32_t(Y), int32_t(Z), int32_t(nd),
+                       run[i].nbr.as<int32_t>());
+    GC_HIP(ctx, hipGetLastError());
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inv is freed on return
     return GC_OK;
   }
 
@@ -189,11 +265,63 @@ struct AskDriver {
     return s;
   }
 
+  OptParams opt_params() const {
+    OptParams o{};
+    o.opt = cfg.optimiser;
+    o.adam_form = cfg.adam_form;
+    o.beta1 = float(cfg.beta1);
+    o.beta2 = float(cfg.beta2);
+    o.eps = float(cfg.eps);
+    o.momentum = float(cfg.momentum);
+    o.rms_decay = float(cfg.rms_decay);
+    o.rms_eps = float(cfg.rms_eps);
+    return o;
+  }
+
+  // an event pair for the next timed dispatch (created once, reused)
+  int next_events(hipEvent_t& a, hipEvent_t& b) {
+    if (ev_used == ev_start.size()) {
+      hipEvent_t ea, eb;
+      GC_HIP(ctx, hipEventCreate(&ea));
+      GC_HIP(ctx, hipEventCreate(&eb));
+      ev_start.push_back(ea);
+      ev_stop.push_back(eb);
+    }
+    a = ev_start[ev_used];
+    b = ev_stop[ev_used];
+    ++ev_used;
+    return GC_OK;
+  }
+
+  // FP64 sums of the block partials of every volume that ran
+  int reduce_partials(const std::vector<int>& n_ep) {
+    for (int i = 0; i < n_vol; ++i) {
+      if (n_ep[i] <= 0) continue;
+      const int nrows = n_ep[i] * npv_count();
+      hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, (nrows + 63) / 64), dim3(kBlock), 0, ctx->stream,
+                         run[i].partials.as<float>(), int64_t(run[i].nblk), nrows, int64_t(kReduceChunk),
+                         d_sums.as<double>() + size_t(i) * K * kSum * max_chunks);
+      GC_HIP(ctx, hipGetLastError());
+    }
+    return GC_OK;
+  }
+
   // Launch one block of iterations for the volumes with n_ep[i] > 0.
   // in_set/out_set select state sets; zl selects the z_last buffer (-1 none).
   template <class Model, int LOSS, bool HAS_W>
   int launch(const std::vector<int>& e0, const std::vector<int>& n_ep, const std::vector<int>& in_set,
              const std::vector<int>& out_set, const std::vector<int>& zl, int iteration_cap) {
+    if (step_mode) return launch_steps<Model, LOSS, HAS_W>(e0, n_ep, in_set, out_set, zl, iteration_cap);
+    if constexpr (Model::kStreamOnly) {
+      return set_err(ctx, GC_ERR_UNSUPPORTED, "this model runs in the per-iteration form only");
+    } else {
+      return launch_blocked<Model, LOSS, HAS_W>(e0, n_ep, in_set, out_set, zl, iteration_cap);
+    }
+  }
+
+  template <class Model, int LOSS, bool HAS_W>
+  int launch_blocked(const std::vector<int>& e0, const std::vector<int>& n_ep, const std::vector<int>& in_set,
+                     const std::vector<int>& out_set, const std::vector<int>& zl, int iteration_cap) {
     // the pinned descriptor buffers are reused: wait for the copies of the
     // previous launch before overwriting them
     if (desc_in_flight) GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -234,48 +362,106 @@ struct AskDriver {
     AskParams ap{};
     ap.vols = d_vols.as<VolDev>();
     ap.n_vol = n_vol;
-    ap.opt = cfg.optimiser;
     ap.transform = cfg.transform;
-    ap.adam_form = cfg.adam_form;
     ap.need_norms = npv_count() == 4 ? 1 : 0;
-    ap.beta1 = float(cfg.beta1);
-    ap.beta2 = float(cfg.beta2);
-    ap.eps = float(cfg.eps);
-    ap.momentum = float(cfg.momentum);
-    ap.rms_decay = float(cfg.rms_decay);
-    ap.rms_eps = float(cfg.rms_eps);
-    if (ev_used == ev_start.size()) {
-      hipEvent_t ea, eb;
-      GC_HIP(ctx, hipEventCreate(&ea));
-      GC_HIP(ctx, hipEventCreate(&eb));
-      ev_start.push_back(ea);
-      ev_stop.push_back(eb);
-    }
-    hipEvent_t ev_a = ev_start[ev_used], ev_b = ev_stop[ev_used];
-    ++ev_used;
+    ap.o = opt_params();
+    hipEvent_t ev_a, ev_b;
+    int rce = next_events(ev_a, ev_b);
+    if (rce) return rce;
     // events attached to the dispatch itself time the kernel alone
     hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
                           ev_b, 0, dmd, ap);
     GC_HIP(ctx, hipGetLastError());
     ++kernel_launches;
-    for (int i = 0; i < n_vol; ++i) {
-      if (n_ep[i] <= 0) continue;
-      const int nrows = n_ep[i] * npv_count();
-      hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, (nrows + 63) / 64), dim3(kBlock), 0, ctx->stream,
-                         run[i].partials.as<float>(), int64_t(run[i].nblk), nrows, int64_t(kReduceChunk),
-                         d_sums.as<double>() + size_t(i) * K * 4 * max_chunks);
-      GC_HIP(ctx, hipGetLastError());
+    return reduce_partials(n_ep);
+  }
+
+  // The step form: one k_askadam_step launch per iteration.  Iteration k of
+  // volume i reads state S_k and writes S_k+1 with S_0 = in_set, S_n = out_set
+  // and the states between in the ping-pong sets, so the input set survives
+  // for a replay.
+  template <class Model, int LOSS, bool HAS_W>
+  int launch_steps(const std::vector<int>& e0, const std::vector<int>& n_ep, const std::vector<int>& in_set,
+                   const std::vector<int>& out_set, const std::vector<int>& zl, int iteration_cap) {
+    if (desc_in_flight) GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int nmax = 0;
+    for (int i = 0; i < n_vol; ++i) nmax = std::max(nmax, n_ep[i]);
+    StepVol* hs = h_steps.as<StepVol>();
+    for (int k = 0; k < nmax; ++k) {
+      int blk0 = 0;
+      for (int i = 0; i < n_vol; ++i) {
+        VolRun& R = run[i];
+        StepVol d{};
+        d.y = vols[i].y_dev;
+        d.w = HAS_W ? (vols[i].w_dev ? vols[i].w_dev : R.ones.as<float>()) : nullptr;
+        d.ex = vols[i].extra_dev;
+        d.nv = R.nv;
+        d.blk0 = blk0;
+        d.nblk = R.nblk;
+        d.active = k < n_ep[i] ? 1 : 0;
+        if (d.active) {
+          const int n = n_ep[i];
+          DevBuf* src = (k == 0) ? R.set[std::max(in_set[i], 0)] : R.pp[(k - 1) & 1];
+          DevBuf* dst = (k == n - 1) ? R.set[std::max(out_set[i], 0)] : R.pp[k & 1];
+          d.z_in = src[0].as<float>();
+          d.m_in = src[1].as<float>();
+          d.v_in = src[2].as<float>();
+          d.z_out = dst[0].as<float>();
+          d.m_out = dst[1].as<float>();
+          d.v_out = dst[2].as<float>();
+          d.z_last = (k == n - 1 && zl[i] >= 0) ? R.zlast[zl[i]].as<float>() : nullptr;
+          d.partials = R.partials.as<float>();
+          d.nbr = use_tv ? R.nbr.as<int32_t>() : nullptr;
+          d.k = k;
+          d.n_epochs = n;
+          d.update = (e0[i] + k < iteration_cap) ? 1 : 0;
+          d.inv_count = float(1.0 / (double(md.n_meas) * double(R.nv)));
+          d.inv_nv = float(1.0 / double(R.nv));
+          d.es = epoch_scal(e0[i] + k);
+        }
+        hs[size_t(k) * n_vol + i] = d;
+        blk0 += R.nblk;
+      }
     }
-    return GC_OK;
+    for (int i = 0; i < n_vol; ++i)
+      if (n_ep[i] > 0) run[i].launches++;
+    GC_HIP(ctx, hipMemcpyAsync(d_steps.p, h_steps.p, sizeof(StepVol) * size_t(nmax) * n_vol,
+                               hipMemcpyHostToDevice, ctx->stream));
+    desc_in_flight = true;
+    StepParams sp{};
+    sp.n_vol = n_vol;
+    sp.n_reg = use_tv ? cfg.n_reg : 0;
+    sp.nd = (cfg.tv_mode == GC_TV_3D) ? 3 : 2;
+    for (int k = 0; k < GC_MAX_REG; ++k) {
+      sp.regmap[k] = k < cfg.n_reg ? cfg.regmap[k] : 0;
+      sp.lam[k] = k < cfg.n_reg ? float(cfg.lambda[k]) : 0.f;
+    }
+    for (int d = 0; d < 3; ++d) sp.inv_vs[d] = float(1.0 / cfg.voxel_size[d]);
+    sp.o = opt_params();
+    for (int k = 0; k < nmax; ++k) {
+      sp.vols = d_steps.as<StepVol>() + size_t(k) * n_vol;
+      hipEvent_t ev_a, ev_b;
+      int rce = next_events(ev_a, ev_b);
+      if (rce) return rce;
+      if (cfg.transform == GC_TRANSFORM_LINEAR)
+        hipExtLaunchKernelGGL((k_askadam_step<Model, LOSS, HAS_W, GC_TRANSFORM_LINEAR>), dim3(total_blk),
+                              dim3(kBlock), 0, ctx->stream, ev_a, ev_b, 0, dmd, sp);
+      else
+        hipExtLaunchKernelGGL((k_askadam_step<Model, LOSS, HAS_W, GC_TRANSFORM_SIGMOID>), dim3(total_blk),
+                              dim3(kBlock), 0, ctx->stream, ev_a, ev_b, 0, dmd, sp);
+      GC_HIP(ctx, hipGetLastError());
+      ++kernel_launches;
+    }
+    return reduce_partials(n_ep);
   }
 
   // One iteration of the host-side loop logic (utils/askadam.m:264-292 minus
   // the update itself).  Returns true when the volume stops at `epoch`.
   bool epoch_logic(VolRun& R, int epoch, double sum_l, double sum_step_prev, double sum_den, double sum_g,
-                   double* trace) {
+                   double sum_reg, double* trace) {
     const double count = double(md.n_meas) * double(R.nv);
     R.fid = sum_l / count;
-    R.reg = 0.0;
+    R.reg = sum_reg / double(R.nv);  // spatial_total_variation: / Nmasked
     R.loss = R.fid + R.reg;
     if (trace) trace[epoch] = R.loss;
     R.last_epoch = epoch;
@@ -330,7 +516,7 @@ struct AskDriver {
     const int P = md.n_par;
     const int iter = cfg.iteration;
     const double* sums = row_sums.data();
-    const size_t sum_bytes = sizeof(double) * K * 4 * max_chunks * n_vol;
+    const size_t sum_bytes = sizeof(double) * K * kSum * max_chunks * n_vol;
     std::vector<int> e0(n_vol), nep(n_vol), in(n_vol), out(n_vol), zl(n_vol);
 
     if (iter <= 0) {
@@ -343,10 +529,11 @@ struct AskDriver {
       fold_sums(nep);
       for (int i = 0; i < n_vol; ++i) {
         VolRun& R = run[i];
-        const double* s = sums + size_t(i) * K * 4;
+        const double* s = sums + size_t(i) * K * kSum;
         R.fid = s[0] / (double(md.n_meas) * double(R.nv));
-        R.loss = R.fid;
-        R.min_loss = R.loss; R.min_fid = R.fid; R.min_iter = 0; R.last_epoch = 0;
+        R.reg = s[4] / double(R.nv);
+        R.loss = R.fid + R.reg;
+        R.min_loss = R.loss; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = 0; R.last_epoch = 0;
         if (vols[i].loss_trace_host) vols[i].loss_trace_host[0] = R.loss;
         R.final_from = 0;
         R.min_zlast = 0;
@@ -378,13 +565,15 @@ struct AskDriver {
       for (int i = 0; i < n_vol; ++i) {
         VolRun& R = run[i];
         if (!R.active) continue;
-        const double* s = sums + size_t(i) * K * 4;
+        const double* s = sums + size_t(i) * K * kSum;
         double* trace = vols[i].loss_trace_host;
         if (!R.have_init) {
           // starting-point evaluation (:197-219) equals the first iteration's
-          const double l0 = s[0] / (double(md.n_meas) * double(R.nv));
-          R.loss = R.fid = l0; R.reg = 0;
-          R.min_loss = l0; R.min_fid = l0; R.min_reg = 0; R.min_iter = 0;
+          R.fid = s[0] / (double(md.n_meas) * double(R.nv));
+          R.reg = s[4] / double(R.nv);
+          const double l0 = R.fid + R.reg;
+          R.loss = l0;
+          R.min_loss = l0; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = 0;
           R.pinned = in[i]; R.pin_e0 = e0[i]; R.pin_r = 0; R.min_zlast = -1;
           R.ema = l0;
           if (trace) trace[0] = l0;
@@ -395,8 +584,9 @@ struct AskDriver {
         for (; k < nep[i]; ++k) {
           const int epoch = e0[i] + k;
           const double prev_min = R.min_loss;
-          const double step_prev = (k == 0) ? R.carry_step : s[(k - 1) * 4 + 1];
-          stopped = epoch_logic(R, epoch, s[k * 4 + 0], step_prev, s[k * 4 + 2], s[k * 4 + 3], trace);
+          const double* sk = s + size_t(k) * kSum;
+          const double step_prev = (k == 0) ? R.carry_step : sk[1 - kSum];
+          stopped = epoch_logic(R, epoch, sk[0], step_prev, sk[2], sk[3], sk[4], trace);
           if (R.min_loss < prev_min) {
             if (k == nep[i] - 1) {
               R.min_zlast = zl[i];
@@ -413,7 +603,7 @@ struct AskDriver {
           if (k == nep[i] - 1) { R.final_from = zl[i]; }
           else { R.final_from = -1; R.final_set = in[i]; R.final_e0 = e0[i]; R.final_r = k; }
         } else {
-          R.carry_step = s[(nep[i] - 1) * 4 + 1];
+          R.carry_step = s[size_t(nep[i] - 1) * kSum + 1];
           R.cur = out[i];
           R.next_epoch = e0[i] + nep[i];
           if (R.next_epoch > iter) {

@@ -114,6 +114,9 @@ inline int model_np(const gc_model& md) {
     case GC_MODEL_R2STAR: return 2;
     case GC_MODEL_JOINT_R1R2STAR: return 3;
     case GC_MODEL_NEXI: return md.i[1] == 2 ? 5 : 4;
+    case GC_MODEL_MCRMWI:
+      return 9 + ((md.flags & GC_FLAG_MCRMWI_EXCHANGE) ? 1 : 0) +
+             ((md.flags & GC_FLAG_MCRMWI_COMPLEX) ? md.i[1] + 1 : 0);
     default: return -1;
   }
 }
@@ -130,12 +133,19 @@ inline int check_model(gc_ctx* ctx, const gc_model* md) {
     case GC_MODEL_R2STAR: m_expect = md->i[0]; break;
     case GC_MODEL_JOINT_R1R2STAR: m_expect = md->i[0] * md->i[1]; break;
     case GC_MODEL_NEXI: m_expect = md->i[0] * (md->i[1] == 2 ? 2 : 1); break;
+    case GC_MODEL_MCRMWI: m_expect = md->i[0] * md->i[1] * ((md->flags & GC_FLAG_MCRMWI_COMPLEX) ? 2 : 1); break;
   }
   if (md->n_meas != m_expect)
     return set_err(ctx, GC_ERR_INPUT_MISMATCH, "n_meas (%d) does not match the acquisition (%d)", md->n_meas, m_expect);
   if (md->model_id == GC_MODEL_JOINT_R1R2STAR) {
     const int need = (md->flags & GC_FLAG_R1R2S_TRUE_FA) ? md->i[1] : 1;
     if (md->n_extra < need) return set_err(ctx, GC_ERR_ARG, "joint R1-R2* needs %d extra rows", need);
+  }
+  if (md->model_id == GC_MODEL_MCRMWI) {
+    if (md->i[0] < 1 || md->i[0] > 32 || md->i[1] < 1) return set_err(ctx, GC_ERR_ARG, "MCRMWI: 1..32 echoes");
+    if (md->n_extra < 3 + md->i[1])
+      return set_err(ctx, GC_ERR_ARG, "MCRMWI needs %d extra rows (B1, pini, ff, freqBKG per flip angle)",
+                     3 + md->i[1]);
   }
   if (md->model_id == GC_MODEL_NEXI) {
     const int ns = md->i[0], nn = md->i[2];
@@ -172,5 +182,9 @@ const ModelOps* ops_joint16x4();
 const ModelOps* ops_joint16x8();
 const ModelOps* ops_nexi16();
 const ModelOps* ops_nexi16_l2();
+const ModelOps* ops_mcrmwi4_exch_cplx();
+const ModelOps* ops_mcrmwi4_exch_mag();
+const ModelOps* ops_mcrmwi4_cplx();
+const ModelOps* ops_mcrmwi4_mag();
 
 }  // namespace gc

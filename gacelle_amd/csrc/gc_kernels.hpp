@@ -102,6 +102,24 @@ GC_DEV void accum_grad(float dl, float w, float res, const float* J, float* gth)
   }
 }
 
+// ---- forward + loss + adjoint of one voxel -----------------------------------
+// lossfn(r, s) evaluates the loss of measurement r at model value s and
+// returns d loss / d s (0 for a measurement that carries no gradient).
+// Models with a fused adjoint (kFusedAdj) accumulate the gradient
+// themselves; the others hand a Jacobian per measurement.
+template <class Model, class LF>
+GC_DEV void model_adjoint(const gc_model& md, const float* th, const float* ex, LF&& lossfn, float* gth) {
+  if constexpr (Model::kFusedAdj) {
+    Model::eval_adj(md, th, ex, lossfn, gth);
+  } else {
+    Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
+      const float dl = lossfn(r, s);
+      if (dl != 0.f) accum_grad<Model, false, false>(dl, 1.f, 0.f, J, gth);
+    });
+  }
+  Model::post_grad(th, gth);
+}
+
 // ---- askadam block kernel -------------------------------------------------
 struct EpochScal {
   float lr;  // adam: lr*sqrt(1-b2^t)/(1-b1^t) (effective form) or lr; sgdm/rmsprop: lr
@@ -130,14 +148,40 @@ struct VolDev {
   float pad2[3];
 };
 
+// optimiser of askadam.update_parameters (utils/askadam.m:673-691)
+struct OptParams {
+  int32_t opt;
+  int32_t adam_form;
+  float beta1, beta2, eps, momentum, rms_decay, rms_eps;
+};
+
+// the parameter step dz of one slot; m1 / m2 are its moment estimates
+GC_DEV float opt_step(const OptParams& o, const EpochScal& es, float g, float& m1, float& m2) {
+  float dz;
+  if (o.opt == GC_OPT_ADAM) {
+    m1 = o.beta1 * m1 + (1.f - o.beta1) * g;
+    m2 = o.beta2 * m2 + (1.f - o.beta2) * g * g;
+    if (o.adam_form == GC_ADAM_KINGMA_BA)
+      dz = -es.lr * (m1 * es.c1) * rcpf(sqrtv(m2 * es.c2) + o.eps);
+    else
+      dz = -es.lr * m1 * rcpf(sqrtv(m2) + o.eps);
+  } else if (o.opt == GC_OPT_SGDM) {
+    m1 = o.momentum * m1 - es.lr * g;
+    dz = m1;
+  } else {
+    m2 = o.rms_decay * m2 + (1.f - o.rms_decay) * g * g;
+    dz = -es.lr * g * rcpf(sqrtv(m2) + o.rms_eps);
+  }
+  return dz;
+}
+
 struct AskParams {
   const VolDev* vols;
   int32_t n_vol;
-  int32_t opt;
   int32_t transform;
-  int32_t adam_form;
   int32_t need_norms;  // step / |z|^2 / |g|^2 sums wanted (step or gradient tolerance active)
-  float beta1, beta2, eps, momentum, rms_decay, rms_eps;
+  int32_t pad;
+  OptParams o;
 };
 
 GC_DEV int find_volume(const VolDev* vols, int n_vol, int blk) {
@@ -268,21 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
         const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2, 0.f};
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-          float dz;
-          if (ap.opt == GC_OPT_ADAM) {
-            m1[p] = ap.beta1 * m1[p] + (1.f - ap.beta1) * g[p];
-            m2[p] = ap.beta2 * m2[p] + (1.f - ap.beta2) * g[p] * g[p];
-            if (ap.adam_form == GC_ADAM_KINGMA_BA)
-              dz = -es.lr * (m1[p] * es.c1) * rcpf(sqrtv(m2[p] * es.c2) + ap.eps);
-            else
-              dz = -es.lr * m1[p] * rcpf(sqrtv(m2[p]) + ap.eps);
-          } else if (ap.opt == GC_OPT_SGDM) {
-            m1[p] = ap.momentum * m1[p] - es.lr * g[p];
-            dz = m1[p];
-          } else {
-            m2[p] = ap.rms_decay * m2[p] + (1.f - ap.rms_decay) * g[p] * g[p];
-            dz = -es.lr * g[p] * rcpf(sqrtv(m2[p]) + ap.rms_eps);
-          }
+          const float dz = opt_step(ap.o, es, g[p], m1[p], m2[p]);
           const float zn = z[p] + dz;
           if (norms) {
             const float d = zn - z[p];
@@ -336,6 +366,229 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
       V.v_out[int64_t(p) * nv + v] = 0.f;
       if (V.z_last != nullptr) V.z_last[int64_t(p) * nv + v] = zd;
     }
+  }
+}
+
+// ---- askadam, one iteration per launch --------------------------------------
+// For what the K-iteration kernel cannot hold: the spatial TV term of
+// askadam.model_gradient (utils/askadam.m:99, utils/spatial_total_variation.m:
+// 24-66), which couples each voxel to its grid neighbours' previous
+// parameters, and models whose measurements do not fit in registers
+// (streamed from HBM every iteration).  Each launch advances every active
+// volume by one iteration from (z, m, v)_in to (z, m, v)_out.
+constexpr int kStepSums = 5;  // fidelity, step^2, |z|^2, |g|^2, regularisation
+
+struct StepVol {
+  const float* y;
+  const float* w;
+  const float* ex;
+  const float* z_in;
+  const float* m_in;
+  const float* v_in;
+  float* z_out;
+  float* m_out;
+  float* v_out;
+  float* z_last;       // z (pre-update) of this iteration, or null
+  float* partials;     // [nblk][n_epochs][kStepSums]
+  const int32_t* nbr;  // [2*nd][nv]: +axis, -axis neighbour G-D index per axis, -1 outside the mask
+  int64_t nv;
+  int32_t blk0, nblk;
+  int32_t active;      // the volume runs this iteration
+  int32_t k, n_epochs; // this iteration's row among the launch sequence's partials
+  int32_t update;      // update_parameters runs (epoch < iteration, :677)
+  float inv_count;     // 1/(M nv)
+  float inv_nv;        // 1/Nmasked of spatial_total_variation
+  EpochScal es;
+};
+
+struct StepParams {
+  const StepVol* vols;
+  int32_t n_vol;
+  int32_t n_reg;       // regularised maps (0: no TV)
+  int32_t nd;          // TV axes: 2 ('2D') or 3 ('3D')
+  int32_t pad;
+  int32_t regmap[4];   // parameter slot of each regularised map
+  float lam[4];
+  float inv_vs[3];     // 1/voxelSize
+  OptParams o;
+};
+
+template <class Model, int LOSS, bool HAS_W, int TR>
+__global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
+  const gc_model& md = *mdp;
+  constexpr int NP = Model::NP;
+  int vi = 0;
+  for (int i = 1; i < sp.n_vol; ++i)
+    if (int(blockIdx.x) >= sp.vols[i].blk0) vi = i;
+  const StepVol& V = sp.vols[vi];
+  if (!V.active) return;
+  const int lblk = blockIdx.x - V.blk0;
+  const int64_t nv = V.nv;
+  const int64_t v = int64_t(lblk) * kBlock + threadIdx.x;
+  const bool valid = v < nv;
+  const int64_t vv = valid ? v : (nv - 1);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+  float ex[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
+  float z[NP], m1[NP], m2[NP], th[NP], dth[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    z[p] = V.z_in[int64_t(p) * nv + vv];
+    m1[p] = V.m_in[int64_t(p) * nv + vv];
+    m2[p] = V.v_in[int64_t(p) * nv + vv];
+    transform_fwd_t<TR>(z[p], md.lb[p], md.ub[p], th[p], dth[p]);
+  }
+  float dead_z2 = 0.f;
+  for (int p = NP; p < md.n_par; ++p) {
+    const float zd = V.z_in[int64_t(p) * nv + vv];
+    dead_z2 += zd * zd;
+  }
+
+  // data term (set_nan_inf_zero semantics on the model signal)
+  float lsum = 0.f;
+  float gth[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+  model_adjoint<Model>(
+      md, th, ex,
+      [&](int r, float s) -> float {
+        const int64_t m = Model::meas(md, r);
+        const bool fin = isfinite(s);
+        const float sv = fin ? s : 0.f;
+        const float wv = HAS_W ? V.w[m * nv + vv] : 1.f;
+        float l, dl;
+        loss_fn<LOSS>(sv - V.y[m * nv + vv], wv, l, dl);
+        lsum += l;
+        return fin ? dl : 0.f;
+      },
+      gth);
+  float g[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * V.inv_count;
+
+  // spatial TV: cost sum_d |theta(i + e_d) - theta(i)| / vs_d over masked i
+  // (forward differences with periodic wrap, zeros outside the mask), and
+  // its subgradient (sign(0) = 0) at i from i's own and its -e_d
+  // neighbour's terms
+  float reg = 0.f;
+  for (int kr = 0; kr < sp.n_reg; ++kr) {
+    const int pr = sp.regmap[kr];
+    const float lam = sp.lam[kr];
+    const float lb = md.lb[pr], ub = md.ub[pr];
+    float ti = 0.f, dti = 0.f;
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (p == pr) { ti = th[p]; dti = dth[p]; }
+    float gt = 0.f;
+    for (int d = 0; d < sp.nd; ++d) {
+      const float ivs = sp.inv_vs[d];
+      const int32_t jf = V.nbr[int64_t(2 * d) * nv + vv];
+      float tf = 0.f, dd;
+      if (jf >= 0) transform_fwd_t<TR>(V.z_in[int64_t(pr) * nv + jf], lb, ub, tf, dd);
+      const float Di = (tf - ti) * ivs;
+      reg += lam * fabsf(Di);
+      gt -= sgnf(Di) * ivs * lam;
+      const int32_t jb = V.nbr[int64_t(2 * d + 1) * nv + vv];
+      if (jb >= 0) {
+        float tb;
+        transform_fwd_t<TR>(V.z_in[int64_t(pr) * nv + jb], lb, ub, tb, dd);
+        gt += sgnf((ti - tb) * ivs) * ivs * lam;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (p == pr) g[p] += gt * V.inv_nv * dti;
+  }
+
+  float den = dead_z2, gn = 0.f, step = 0.f;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    den += z[p] * z[p];
+    gn += g[p] * g[p];
+  }
+  if (V.z_last != nullptr && valid) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
+  }
+  if (V.update) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const float dz = opt_step(sp.o, V.es, g[p], m1[p], m2[p]);
+      const float zn = z[p] + dz;
+      const float dlt = zn - z[p];
+      step += dlt * dlt;
+      z[p] = zn;
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      V.z_out[int64_t(p) * nv + v] = z[p];
+      V.m_out[int64_t(p) * nv + v] = m1[p];
+      V.v_out[int64_t(p) * nv + v] = m2[p];
+    }
+    for (int p = NP; p < md.n_par; ++p) {
+      const float zd = V.z_in[int64_t(p) * nv + v];
+      V.z_out[int64_t(p) * nv + v] = zd;
+      V.m_out[int64_t(p) * nv + v] = 0.f;
+      V.v_out[int64_t(p) * nv + v] = 0.f;
+      if (V.z_last != nullptr) V.z_last[int64_t(p) * nv + v] = zd;
+    }
+  }
+
+  // block sums of this iteration
+  if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; reg = 0.f; }
+  const float t4 = wave_sum4(lsum, step, den, gn, lane);
+  const float tr = wave_sum(reg);
+  __shared__ float red[kBlock / 64][kStepSums];
+  if (lane < 4) red[wave][wave_sum4_index(lane)] = t4;
+  if (lane == 0) red[wave][4] = tr;
+  __syncthreads();
+  if (threadIdx.x < kStepSums) {
+    float t = 0.f;
+#pragma unroll
+    for (int wv = 0; wv < kBlock / 64; ++wv) t += red[wv][threadIdx.x];
+    V.partials[(int64_t(lblk) * V.n_epochs + V.k) * kStepSums + threadIdx.x] = t;
+  }
+}
+
+// ---- grid neighbours of masked voxels (TV stencil) --------------------------
+static __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t val) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = val;
+}
+
+// inv[vox[v]] = v (vox: column-major grid index of masked voxel v, in range)
+static __global__ void k_scatter_index(const int64_t* __restrict__ vox, int64_t nv, int32_t* __restrict__ inv) {
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < nv) inv[vox[v]] = int32_t(v);
+}
+
+// nbr[2d][v] / nbr[2d+1][v]: G-D index of the +1 / -1 neighbour of voxel v
+// along axis d with periodic wrap (circshift), or -1 outside the mask
+static __global__ void k_neighbours(const int64_t* __restrict__ vox, int64_t nv, const int32_t* __restrict__ inv,
+                                    int32_t X, int32_t Y, int32_t Z, int32_t nd, int32_t* __restrict__ nbr) {
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int64_t lin = vox[v];
+  const int64_t x = lin % X, y = (lin / X) % Y, z = lin / (int64_t(X) * Y);
+  const int64_t XY = int64_t(X) * Y;
+  for (int d = 0; d < nd; ++d) {
+    int64_t fw, bw;
+    if (d == 0) {
+      fw = ((x + 1) % X) + X * y + XY * z;
+      bw = ((x + X - 1) % X) + X * y + XY * z;
+    } else if (d == 1) {
+      fw = x + X * ((y + 1) % Y) + XY * z;
+      bw = x + X * ((y + Y - 1) % Y) + XY * z;
+    } else {
+      fw = x + X * y + XY * ((z + 1) % Z);
+      bw = x + X * y + XY * ((z + Z - 1) % Z);
+    }
+    nbr[int64_t(2 * d) * nv + v] = inv[fw];
+    nbr[int64_t(2 * d + 1) * nv + v] = inv[bw];
   }
 }
 
@@ -423,18 +676,20 @@ __global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ md
 #pragma unroll
     for (int p = 0; p < NP; ++p) gth[p] = 0.f;
     float lvox = 0.f;
-    Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
-      const int64_t m = Model::meas(md, r);
-      const bool fin = isfinite(s);
-      const float sv = fin ? s : 0.f;
-      const float wv = HAS_W ? ep.w[m * nv + vv] : 1.f;
-      float l, dl;
-      loss_fn<LOSS>(sv - ep.y[m * nv + vv], wv, l, dl);
-      lvox += l;
-      if (ep.residual != nullptr && valid) ep.residual[m * nv + v] = l;
-      if (fin) accum_grad<Model, false, HAS_W>(dl, wv, sv - ep.y[m * nv + vv], J, gth);
-    });
-    Model::post_grad(th, gth);
+    model_adjoint<Model>(
+        md, th, ex,
+        [&](int r, float s) -> float {
+          const int64_t m = Model::meas(md, r);
+          const bool fin = isfinite(s);
+          const float sv = fin ? s : 0.f;
+          const float wv = HAS_W ? ep.w[m * nv + vv] : 1.f;
+          float l, dl;
+          loss_fn<LOSS>(sv - ep.y[m * nv + vv], wv, l, dl);
+          lvox += l;
+          if (ep.residual != nullptr && valid) ep.residual[m * nv + v] = l;
+          return fin ? dl : 0.f;
+        },
+        gth);
     lsum = valid ? double(lvox) : 0.0;
     if (valid && ep.resloss != nullptr) ep.resloss[v] = lvox / float(md.n_meas);
     if (valid && ep.grad != nullptr) {

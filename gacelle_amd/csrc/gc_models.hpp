@@ -29,6 +29,7 @@ template <int MT, bool EXACT = false, bool UNI = false>
 struct R2sModel {
   static constexpr int NP = 2, MR = MT, NE = 0;
   static constexpr bool kRelJ = true;  // J[1] is relative to J[0] (see accum_grad)
+  static constexpr bool kFusedAdj = false, kStreamOnly = false;
   GC_DEV static bool valid(const gc_model& md, int r) { return EXACT ? (r < MT) : (r < md.i[0]); }
   GC_DEV static int meas(const gc_model&, int r) { return r; }
   template <bool JAC, class F>
@@ -75,6 +76,7 @@ template <int MT, int MF>
 struct JointModel {
   static constexpr int NP = 3, MR = MT * MF, NE = 1;
   static constexpr bool kRelJ = false;
+  static constexpr bool kFusedAdj = false, kStreamOnly = false;
   GC_DEV static void post_grad(const float*, float*) {}
   GC_DEV static bool valid(const gc_model& md, int r) { return (r % MT) < md.i[0] && (r / MT) < md.i[1]; }
   GC_DEV static int meas(const gc_model& md, int r) { return (r % MT) + md.i[0] * (r / MT); }
@@ -127,6 +129,7 @@ template <int MS, bool L2>
 struct NexiModel {
   static constexpr int NP = L2 ? 5 : 4, MR = L2 ? 2 * MS : MS, NE = 1;
   static constexpr bool kRelJ = false;
+  static constexpr bool kFusedAdj = false, kStreamOnly = false;
   GC_DEV static void post_grad(const float*, float*) {}
   GC_DEV static bool valid(const gc_model& md, int r) { return (r % MS) < md.i[0]; }
   GC_DEV static int meas(const gc_model& md, int r) { return (r < MS) ? r : md.i[0] + (r - MS); }
@@ -280,6 +283,317 @@ struct NexiModel {
           f(MS + k, sl2, J2);
         }
       }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual number with N derivative slots (device side).
+template <int N>
+struct DualF {
+  float v;
+  float d[N];
+  GC_DEV static DualF cst(float x) {
+    DualF r;
+    r.v = x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = 0.f;
+    return r;
+  }
+  GC_DEV static DualF var(float x, int k) {
+    DualF r = cst(x);
+    r.d[k] = 1.f;
+    return r;
+  }
+};
+template <int N>
+GC_DEV DualF<N> operator+(const DualF<N>& a, const DualF<N>& b) {
+  DualF<N> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator-(const DualF<N>& a, const DualF<N>& b) {
+  DualF<N> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator*(const DualF<N>& a, const DualF<N>& b) {
+  DualF<N> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator/(const DualF<N>& a, const DualF<N>& b) {
+  DualF<N> r;
+  const float ib = 1.f / b.v;
+  r.v = a.v * ib;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator*(float a, const DualF<N>& b) {
+  DualF<N> r;
+  r.v = a * b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a * b.d[i];
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator+(float a, const DualF<N>& b) {
+  DualF<N> r = b;
+  r.v = a + b.v;
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator-(float a, const DualF<N>& b) {
+  DualF<N> r;
+  r.v = a - b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -b.d[i];
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> operator-(const DualF<N>& a) {
+  return 0.f - a;
+}
+template <int N>
+GC_DEV DualF<N> dsqrt(const DualF<N>& a) {
+  DualF<N> r;
+  r.v = sqrtf(a.v);
+  const float h = (r.v > 0.f) ? 0.5f / r.v : 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
+  return r;
+}
+template <int N>
+GC_DEV DualF<N> dexp(const DualF<N>& a) {
+  DualF<N> r;
+  r.v = __expf(a.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * r.v;
+  return r;
+}
+// max(a, c) with the derivative of the selected branch
+template <int N>
+GC_DEV DualF<N> dmax(const DualF<N>& a, float c) {
+  return (a.v >= c) ? a : DualF<N>::cst(c);
+}
+
+// ---------------------------------------------------------------------------
+// MCR-MWI three-pool GRE model, MCRMWI/gpuMCRMWI.m:566-714 with the DIMWI
+// terms fitted (no HCFM), isEPG = false, and compute_gremwi_signal
+// (MCRMWI/private/compute_gremwi_signal.m:1-15).  T1 steady state per flip
+// angle: Bloch-McConnell two-pool analytical solution (model_BM_2T1_analytical,
+// :992-1023) when EXCH, else the non-exchanging Bloch model (:1026-1032).
+//
+// theta = {S0, MWF, IWF, R1IEW, [kIEWM], R2sMW, R2sIW, R2sEW, freqMW, freqIW,
+//          [dfreqBKG x MF, dpini]}; NA = 4 + EXCH amplitude parameters first.
+// Measurement r = t + nte*fa (real part or magnitude), + nte*nfa for the
+// imaginary part when CPLX.  The adjoint is fused: the amplitude parameters'
+// derivatives come from duals over the per-flip-angle steady state (NA
+// slots), every per-echo term has closed-form derivatives, and the gradient
+// is accumulated from the loss derivatives the caller returns per
+// measurement.  Data are streamed (kStreamOnly): M = 2 x 32 x MF is too
+// large to keep in registers.
+template <int MF, bool EXCH, bool CPLX>
+struct McrmwiModel {
+  static constexpr int NA = EXCH ? 5 : 4;
+  static constexpr int NP = NA + 5 + (CPLX ? MF + 1 : 0);
+  static constexpr int MT = 32;
+  static constexpr int MR = MT * MF * (CPLX ? 2 : 1);
+  static constexpr int NE = 3 + MF;
+  static constexpr bool kRelJ = false;
+  static constexpr bool kFusedAdj = true, kStreamOnly = true;
+  using D = DualF<NA>;
+  GC_DEV static void post_grad(const float*, float*) {}
+  GC_DEV static bool valid(const gc_model& md, int r) { return r < md.n_meas; }
+  GC_DEV static int meas(const gc_model&, int r) { return r; }
+
+  // compartment amplitudes (MW, IW, EW) at flip angle index fi
+  GC_DEV static void amplitudes(const gc_model& md, const float* th, const float* ex, int fi, D& aMW, D& aIW,
+                                D& aEW) {
+    const int nte = md.i[0];
+    const float TR = md.f[0], rho = md.f[3], t1mw = md.f[4];
+    const D S0 = D::var(th[0], 0), mwf = D::var(th[1], 1), iwf = D::var(th[2], 2), r1iew = D::var(th[3], 3);
+    const D S0MW = S0 * mwf;
+    const D S0IW = S0 * (1.f - mwf) * iwf;
+    const D S0EW = S0 * (1.f - mwf) * (1.f - iwf);
+    const D tv = dmax(S0IW + S0EW + (1.f / rho) * S0MW, 1e-8f);
+    const float famp = md.table[nte + fi] * ex[0];
+    float sa, ca;
+    sincosf(famp, &sa, &ca);
+    D aIEW;
+    if constexpr (EXCH) {
+      const D kfr = D::var(th[4], 4);
+      const D f = dmax(((1.f / rho) * S0MW) / tv, 1e-8f);  // MVF
+      const float R1r = 1.f / t1mw;
+      const D a = -r1iew - kfr;
+      const D b = (1.f - f) * kfr / f;
+      const D c = kfr;
+      const D d = -R1r - b;
+      const D apd = a + d;
+      const D det = a * d - b * c;
+      const D disc = dsqrt(apd * apd - 4.f * det);
+      const D l1 = 0.5f * (apd + disc), l2 = 0.5f * (apd - disc);
+      const D E1 = dexp(TR * l1), E2 = dexp(TR * l2);
+      const D dl = l1 - l2;
+      const D e11 = (E1 * (a - l2) - E2 * (a - l1)) / dl;
+      const D e12 = b * (E1 - E2) / dl;
+      const D e21 = c * (E1 - E2) / dl;
+      const D e22 = (E1 * (d - l2) - E2 * (d - l1)) / dl;
+      const D C = D::cst(sa) / ((1.f - ca * (e22 + e11) + (ca * ca) * (e11 * e22 - e21 * e12)) * det);
+      const D ap = (1.f - ca * e22) * (e11 - D::cst(1.f)) + ca * (e12 * e21);
+      const D bp = (1.f - ca * e22) * e12 + ca * e12 * (e22 - D::cst(1.f));
+      const D cp = ca * e21 * (e11 - D::cst(1.f)) + (1.f - ca * e11) * e21;
+      const D dp = ca * (e21 * e12) + (1.f - ca * e11) * (e22 - D::cst(1.f));
+      const D siew = ((ap * d - bp * c) * r1iew * (1.f - f) + R1r * (bp * a - ap * b) * f) * C;
+      const D sm = ((cp * d - c * dp) * r1iew * (1.f - f) + R1r * (a * dp - cp * b) * f) * C;
+      aMW = rho * (sm * tv);
+      aIEW = siew * tv;
+    } else {
+      const float e1mw = __expf(-TR / t1mw);
+      aMW = (sa * (1.f - e1mw) / (1.f - ca * e1mw)) * S0MW;
+      const D e1iew = dexp(-TR * r1iew);
+      aIEW = (S0IW + S0EW) * (sa * (1.f - e1iew)) / (1.f - ca * e1iew);
+    }
+    aIW = aIEW * iwf;
+    aEW = aIEW * (1.f - iwf);
+  }
+
+  // Visit the echoes of flip angle fi with each pool's decay e_p =
+  // exp(-te r2s_p) and phase (cos, sin); g(t, te, e[3], c[3], s[3]).  With
+  // equally spaced echoes both are advanced by recurrence (one multiply
+  // and one complex rotation per pool and echo).
+  template <class G>
+  GC_DEV static void per_echo(const gc_model& md, const float* th, const float* ex, int fi, G&& g) {
+    const int nte = md.i[0];
+    const float B0g = md.f[1] * md.f[2];
+    const float freqEW = md.f[5];
+    const float r2s[3] = {th[NA], th[NA + 1], th[NA + 2]};
+    const float freq[3] = {th[NA + 3], th[NA + 4], freqEW};
+    float fbkg = 0.f, pini = 0.f;
+    if constexpr (CPLX) {
+      fbkg = th[NA + 5 + fi] + ex[3 + fi];
+      pini = th[NA + 5 + MF] + ex[1];
+    }
+    const bool uni = (md.flags & GC_FLAG_UNIFORM_TE) != 0;
+    constexpr float k2pi = 6.283185307179586f;
+    float w[3], e[3], cs[3], sn[3], qe[3], qc[3], qs[3];
+    const float te0 = md.table[0], dte = md.f[8];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      w[p] = k2pi * (freq[p] + fbkg) * B0g;
+      if (uni) {
+        e[p] = __expf(-te0 * r2s[p]);
+        sincosf(te0 * w[p] + pini, &sn[p], &cs[p]);
+        qe[p] = __expf(-dte * r2s[p]);
+        sincosf(dte * w[p], &qs[p], &qc[p]);
+      }
+    }
+    for (int t = 0; t < nte; ++t) {
+      const float te = md.table[t];
+      if (!uni) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          e[p] = __expf(-te * r2s[p]);
+          sincosf(te * w[p] + pini, &sn[p], &cs[p]);
+        }
+      }
+      g(t, te, e, cs, sn);
+      if (uni) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          e[p] *= qe[p];
+          const float c2 = cs[p] * qc[p] - sn[p] * qs[p];
+          const float s2 = sn[p] * qc[p] + cs[p] * qs[p];
+          cs[p] = c2;
+          sn[p] = s2;
+        }
+      }
+    }
+  }
+
+  template <bool JAC, class F>
+  GC_DEV static void eval(const gc_model& md, const float* th, const float* ex, F&& f) {
+    static_assert(!JAC, "McrmwiModel provides its derivatives through eval_adj");
+    const int nte = md.i[0], nfa = md.i[1];
+    const float ff = ex[2];
+    for (int fi = 0; fi < nfa; ++fi) {
+      D aMW, aIW, aEW;
+      amplitudes(md, th, ex, fi, aMW, aIW, aEW);
+      const float amp[3] = {aMW.v, aIW.v, aEW.v};
+      per_echo(md, th, ex, fi, [&](int t, float, const float* e, const float* c, const float* s) {
+        const float d[3] = {amp[0] * e[0], amp[1] * e[1], amp[2] * e[2]};
+        const float re = (d[0] * c[0] + d[1] * c[1] + d[2] * c[2]) * ff;
+        const float im = (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) * ff;
+        if constexpr (CPLX) {
+          f(t + nte * fi, re, nullptr);
+          f(t + nte * fi + nte * nfa, im, nullptr);
+        } else {
+          f(t + nte * fi, sqrtf(re * re + im * im), nullptr);
+        }
+      });
+    }
+  }
+
+  // dloss(r, s) returns d loss / d s_r (0 when the measurement carries no
+  // gradient); g[NP] accumulates d loss / d theta.
+  template <class F>
+  GC_DEV static void eval_adj(const gc_model& md, const float* th, const float* ex, F&& dloss, float* g) {
+    const int nte = md.i[0], nfa = md.i[1];
+    const float ff = ex[2];
+    const float B0g = md.f[1] * md.f[2];
+    constexpr float k2pi = 6.283185307179586f;
+    for (int fi = 0; fi < nfa; ++fi) {
+      D aMW, aIW, aEW;
+      amplitudes(md, th, ex, fi, aMW, aIW, aEW);
+      const float amp[3] = {aMW.v, aIW.v, aEW.v};
+      float gamp[3] = {0.f, 0.f, 0.f};  // d loss / d a_p at this flip angle
+      float gbkg = 0.f;                  // d loss / d (phase slope common to all pools) / w0
+      per_echo(md, th, ex, fi, [&](int t, float te, const float* e, const float* c, const float* s) {
+        const float d[3] = {amp[0] * e[0], amp[1] * e[1], amp[2] * e[2]};
+        const float re = (d[0] * c[0] + d[1] * c[1] + d[2] * c[2]) * ff;
+        const float im = (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) * ff;
+        float dre, dim;
+        if constexpr (CPLX) {
+          dre = dloss(t + nte * fi, re);
+          dim = dloss(t + nte * fi + nte * nfa, im);
+        } else {
+          const float mag = sqrtf(re * re + im * im);
+          const float dm = dloss(t + nte * fi, mag);
+          const float im_ = (mag > 0.f) ? 1.f / mag : 0.f;
+          dre = dm * re * im_;
+          dim = dm * im * im_;
+        }
+        if (dre == 0.f && dim == 0.f) return;
+        const float w0 = k2pi * te * B0g;
+        float vsum = 0.f;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const float u = ff * (dre * c[p] + dim * s[p]);   // d loss / d d_p
+          const float v = ff * d[p] * (dim * c[p] - dre * s[p]);  // d loss / d phase_p
+          gamp[p] += u * e[p];
+          g[NA + p] += u * d[p] * (-te);                    // R2s of pool p
+          if (p < 2) g[NA + 3 + p] += v * w0;              // freqMW, freqIW
+          vsum += v;
+        }
+        if constexpr (CPLX) {
+          gbkg += vsum * w0;
+          g[NA + 5 + MF] += vsum;                           // dpini
+        }
+      });
+#pragma unroll
+      for (int q = 0; q < NA; ++q) g[q] += gamp[0] * aMW.d[q] + gamp[1] * aIW.d[q] + gamp[2] * aEW.d[q];
+      if constexpr (CPLX) g[NA + 5 + fi] += gbkg;
     }
   }
 };

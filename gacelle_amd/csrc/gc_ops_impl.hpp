@@ -49,7 +49,7 @@ struct OpsImpl {
 
   static int askadam(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol) {
     AskDriver drv(ctx, *md, *cfg, vols, n_vol);
-    int rc = drv.setup();
+    int rc = drv.setup(Model::kStreamOnly);
     if (rc) return rc;
     return with_loss(ctx, cfg->loss, [&](auto lc) -> int {
       constexpr int LOSS = decltype(lc)::value;
@@ -64,6 +64,16 @@ struct OpsImpl {
 
   static int mcmc(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
                   hipEvent_t ev0, hipEvent_t ev1) {
+    if constexpr (Model::kStreamOnly) {
+      // the samplers keep a voxel's data in registers
+      return set_err(ctx, GC_ERR_UNSUPPORTED, "MCMC is not available for this model");
+    } else {
+      return mcmc_launch(ctx, dmd, gw, hw, P, grid, ev0, ev1);
+    }
+  }
+
+  static int mcmc_launch(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
+                         hipEvent_t ev0, hipEvent_t ev1) {
     return with_bool(hw, [&](auto hwc) -> int {
       constexpr bool HW = decltype(hwc)::value;
       if (gw)

@@ -128,12 +128,14 @@ class _ModelBase:
 
 
 def uniform_spacing(te):
-    """Spacing of equally spaced echo times, or None."""
+    """Spacing of equally spaced echo times, or None.  Spacings may differ by
+    the float32 rounding of the echo times (up to 1e-5 of the spacing: an
+    exponent error below 1e-5 * R * spacing)."""
     te = np.asarray(te, dtype=np.float64).ravel()
     if te.size < 3:
         return None
     d = np.diff(te)
-    if np.all(np.abs(d - d[0]) <= 1e-6 * max(abs(d[0]), 1e-12)) and d[0] > 0:
+    if np.all(np.abs(d - d[0]) <= 1e-5 * max(abs(d[0]), 1e-12)) and d[0] > 0:
         return float(np.float32(te[1]) - np.float32(te[0]))
     return None
 
@@ -500,3 +502,139 @@ class gpuNEXI(_ModelBase):
 
 
 gpuNEXIrice = gpuNEXI  # the Rician variant is gpuNEXI(..., rician=True)
+
+
+# ===========================================================================
+class gpuMCRMWI(_ModelBase):
+    """Multi-compartment relaxometry myelin water imaging (3-pool complex
+    GRE with VFA T1 weighting), MCRMWI/gpuMCRMWI.m.
+
+    The device model covers the DIMWI-fitted path (isFitFreqMW,
+    isFitFreqIW, isFitR2sEW and isFitIWF true) with isEPG = false: the
+    T1 steady state is the Bloch-McConnell analytical solution when
+    isFitExchange, else the non-exchanging Bloch model.  Complex data fit
+    dfreqBKG (one map per flip angle) and dpini.  Four flip angles.
+    """
+
+    modelParams = ["S0", "MWF", "IWF", "R1IEW", "kIEWM", "R2sMW", "R2sIW", "R2sEW", "freqMW", "freqIW", "dfreqBKG",
+                   "dpini", "noise"]
+    ub = [2, 0.3, 1, 2, 10, 200, 50, 50, 0.25, 0.05, 0.4, np.pi / 2, 0.1]
+    lb = [1e-8, 1e-8, 1e-8, 1 / 3, 1e-8, 50, 1 / 150e-3, 1 / 150e-3, -0.05, -0.1, -0.4, -np.pi / 2, 0.001]
+    startPoint = [1, 0.1, 0.6, 1, 1, 100, 15, 21, 0.04, 1e-8, 1e-8, 1e-8, 0.005]
+    step = [0.1, 0.015, 0.05, 0.1, 1, 10, 3.5, 3.5, 0.02, 0.01, 0.05, 0.075, 0.005]
+    model_id = _abi.GC_MODEL_MCRMWI
+    gyro = 42.57747892
+
+    def __init__(self, te, tr, fa, fixed_params=None):
+        self.te = np.asarray(te, dtype=np.float32).ravel()
+        self.tr = float(np.float32(tr))
+        self.fa = np.asarray(fa, dtype=np.float32).ravel()
+        fp = fixed_params or {}
+        self.B0 = float(fp.get("B0", 3.0))
+        self.rho_mw = float(fp.get("rho_mw", 0.36 / 0.86))
+        self.t1_mw = float(fp.get("t1_mw", 234e-3))
+        self.freq_EW = float(fp.get("freq_EW", 0.0))
+        self._init_contract()
+
+    @staticmethod
+    def check_set_default(fitting, data=None):
+        """MCRMWI/gpuMCRMWI.m:1063-1107 (the ANN paths are not used)."""
+        from .askadam import askadam as _ask
+        from .mcmc import mcmc as _mcmc
+
+        fitting = dict(fitting or {})
+        fitting.setdefault("solver", "askadam")
+        if str(fitting["solver"]).lower() == "mcmc":
+            f2 = _mcmc.check_set_default_basic(fitting)
+        else:
+            f2 = _ask.check_set_default_basic(fitting)
+            if "regmap" not in fitting:
+                f2["regmap"] = "MWF"
+        f2.setdefault("isWeighted", True)
+        f2.setdefault("weightMethod", "1stecho")
+        f2.setdefault("weightPower", 2)
+        f2.setdefault("start", "prior")
+        dimwi = dict(fitting.get("DIMWI", {}))
+        for k in ("isFitFreqMW", "isFitFreqIW", "isFitR2sEW", "isFitIWF"):
+            dimwi.setdefault(k, True)
+        f2["DIMWI"] = dimwi
+        f2.setdefault("isFitExchange", True)
+        # the EPG-X MLP path (reference default) is not part of this library
+        f2.setdefault("isEPG", False)
+        f2.setdefault("isComplex", True)
+        if data is not None and not np.iscomplexobj(data):
+            f2["isComplex"] = False
+        return f2
+
+    def updateProperty(self, fitting):
+        super().updateProperty(fitting)
+        drop = []
+        if not fitting.get("isComplex", True):
+            drop += ["dfreqBKG", "dpini"]
+        if not fitting.get("isFitExchange", True):
+            drop += ["kIEWM"]
+        for name in drop:
+            if name in self.modelParams:
+                idx = self.modelParams.index(name)
+                for a in ("modelParams", "lb", "ub", "startPoint", "step"):
+                    getattr(self, a).pop(idx)
+        return self
+
+    def _param_size(self, name) -> int:
+        return self.fa.size if name == "dfreqBKG" else 1
+
+    def _flags(self, fitting):
+        f = 0
+        if fitting.get("isComplex", True):
+            f |= _abi.GC_FLAG_MCRMWI_COMPLEX
+        if fitting.get("isFitExchange", True):
+            f |= _abi.GC_FLAG_MCRMWI_EXCHANGE
+        return f
+
+    def gc_model(self, fitting, solver, nmeas):
+        dimwi = fitting.get("DIMWI", {})
+        if fitting.get("isEPG", False) or not all(dimwi.get(k, True) for k in
+                                                  ("isFitFreqMW", "isFitFreqIW", "isFitR2sEW", "isFitIWF")):
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED,
+                               "gpuMCRMWI: the EPG-X and HCFM paths are not available (isEPG=false, DIMWI fitted)")
+        md = self._base_model(fitting)
+        md.flags = self._flags(fitting)
+        nte, nfa = self.te.size, self.fa.size
+        md.i[0], md.i[1] = nte, nfa
+        md.n_meas = nte * nfa * (2 if md.flags & _abi.GC_FLAG_MCRMWI_COMPLEX else 1)
+        md.n_extra = 3 + nfa
+        md.f[0], md.f[1], md.f[2] = self.tr, self.B0, self.gyro
+        md.f[3], md.f[4], md.f[5] = self.rho_mw, self.t1_mw, self.freq_EW
+        off = _fill_table(md, self.te)
+        _fill_table(md, np.deg2rad(self.fa.astype(np.float64)).astype(np.float32), off)
+        dte = uniform_spacing(self.te)
+        if dte is not None:
+            md.flags |= _abi.GC_FLAG_UNIFORM_TE
+            md.f[8] = dte
+        if nmeas is not None and nmeas != md.n_meas:
+            raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The size of TE/FA does not match the data.")
+        return md
+
+    def extra_rows(self, extraData, mask, vox, nv):
+        """[B1, pini, ff, freqBKG x nfa] per voxel (extraData maps on the image
+        grid, masked to G-D; defaults 1, 0, 1, 0)."""
+        ex = extraData or {}
+        nfa = self.fa.size
+        rows = np.zeros((3 + nfa, nv), np.float32)
+
+        def gd(key, default, k=None):
+            if key not in ex or ex[key] is None:
+                return np.full(nv, default, np.float32)
+            a = np.asarray(ex[key], np.float32)
+            if a.size == 1:
+                return np.full(nv, float(a), np.float32)
+            d = utils.dims3(mask)
+            flat = a.reshape(d[0] * d[1] * d[2], -1, order="F")
+            return flat[vox, 0 if k is None else k]
+
+        rows[0] = gd("b1", 1.0)
+        rows[1] = gd("pini", 0.0)
+        rows[2] = gd("ff", 1.0)
+        for k in range(nfa):
+            rows[3 + k] = gd("freqBKG", 0.0, k)
+        return rows

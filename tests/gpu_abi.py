@@ -77,6 +77,12 @@ def askadam(md, cfg, vols, trace=False):
         a.residual_dev = runtime.ptr(rs)
         a.residual_min_dev = runtime.ptr(rsm)
         a.loss_trace_host = ctypes.cast(tr, ctypes.POINTER(ctypes.c_double)) if tr is not None else None
+        if v.get("vox") is not None:
+            vd = c.to_device(np.asarray(v["vox"], np.int64), np.int64)
+            a.vox_dev = runtime.ptr(vd)
+            for k in range(3):
+                a.dims[k] = int(v["dims"][k])
+            keep.append(vd)
         keep += [yd, wd, ex, zd, zm, rl, rlm, rs, rsm]
         metas.append((zd, zm, rl, rlm, rs, rsm, tr))
     c.check(c.lib.gc_askadam_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), arr, n))

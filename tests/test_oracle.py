@@ -193,11 +193,18 @@ def test_rician_mean_against_bessel_and_limits():
 
 
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("kind", ["r2s", "joint", "nexi", "nexi_l2", "nexi_rice"])
+@pytest.mark.parametrize("kind", ["r2s", "joint", "nexi", "nexi_l2", "nexi_rice", "mcrmwi", "mcrmwi_bloch_mag"])
 def test_dual_gradient_matches_central_differences(kind):
     rng = np.random.default_rng(1)
     extra = None
-    if kind == "r2s":
+    if kind.startswith("mcrmwi"):
+        exch = kind == "mcrmwi"
+        _, _, md = _mcrmwi_md(exch, cplx=exch, te=np.arange(2, 25, 2) * 1e-3)
+        amp = [1.3, 0.15, 0.6, 1.1] + ([2.5] if exch else [])
+        th = np.array(amp + [80.0, 20.0, 25.0, 0.08, -0.02] + ([0.01, -0.02, 0.03, 0.0, 0.1] if exch else []))
+        th = th[:, None]
+        extra = np.array([[0.95], [0.3], [0.9], [0.05], [0.0], [-0.05], [0.02]])
+    elif kind == "r2s":
         md = _r2s_md()
         th = np.array([[1.1], [33.0]])
     elif kind == "joint":
@@ -389,3 +396,106 @@ def test_gw_stretch_density():
     ref = 1 / np.sqrt(c)
     ref /= np.sum(ref * np.diff(edges))
     np.testing.assert_allclose(hist, ref, rtol=0.05)
+
+
+# ---------------------------------------------------------------------------
+# MCR-MWI: the Bloch-McConnell steady state against a matrix-exponential
+# solve of the two-pool exchange equations, and its no-exchange limit
+def _mcrmwi_md(exch, cplx, te=(1e-6, 2e-6, 3e-6)):
+    m = models.gpuMCRMWI(np.asarray(te), 38e-3, np.array([5.0, 10.0, 20.0, 50.0]))
+    f = m.check_set_default(dict(isComplex=cplx, isFitExchange=exch))
+    m.updateProperty(f)
+    f["modelParams"], f["lb"], f["ub"] = list(m.modelParams), list(m.lb), list(m.ub)
+    return m, f, m.gc_model(f, "askadam", None)
+
+
+def _bm_expm(TR, alpha, f, R1f, R1r, kfr):
+    """Spoiled-GRE steady state of dM/dt = A M + C (pools free f-pool, restricted
+    r-pool, exchange kfr and kfr (1-f)/f): Mz- = (I - cos(a) E)^-1 A^-1 (E - I) C,
+    signal = sin(a) Mz- per pool."""
+    krf = kfr * (1 - f) / f
+    A = np.array([[-R1f - kfr, krf], [kfr, -R1r - krf]])
+    C = np.array([R1f * (1 - f), R1r * f])
+    E = linalg.expm(A * TR)
+    Mz = np.linalg.solve(np.eye(2) - np.cos(alpha) * E, np.linalg.solve(A, (E - np.eye(2)) @ C))
+    return np.sin(alpha) * Mz  # [IEW, MW]
+
+
+@pytest.mark.parametrize("exch", [True, False])
+def test_mcrmwi_t1_steady_state(exch):
+    """At TE -> 0 with zero frequencies the magnitude equals ff (a_MW + a_IW +
+    a_EW): the BM analytical solution (MCRMWI/gpuMCRMWI.m:992-1023) matches
+    the matrix-exponential steady state; without exchange the Bloch model."""
+    m, f, md = _mcrmwi_md(exch, cplx=False)
+    S0, mwf, iwf, r1iew, kiewm = 1.3, 0.15, 0.6, 1.1, 2.5
+    th = [S0, mwf, iwf, r1iew] + ([kiewm] if exch else []) + [80.0, 20.0, 25.0, 0.0, 0.0]
+    b1, ff = 0.95, 1.0
+    ex = np.array([[b1], [0.0], [ff], [0.0], [0.0], [0.0], [0.0]])
+    s = orc.fwd_value(md, np.array(th)[:, None], ex)[:, 0]
+    rho, t1mw, TR = md.f[3], md.f[4], md.f[0]
+    S0MW, S0IW, S0EW = S0 * mwf, S0 * (1 - mwf) * iwf, S0 * (1 - mwf) * (1 - iwf)
+    tv = S0IW + S0EW + S0MW / rho
+    mvf = (S0MW / rho) / tv
+    for fi, fa in enumerate([5.0, 10.0, 20.0, 50.0]):
+        alpha = np.deg2rad(np.float32(fa)) * b1
+        if exch:
+            iew, mw = _bm_expm(TR, alpha, mvf, r1iew, 1 / t1mw, kiewm)
+            amp = rho * mw * tv + iew * tv
+        else:
+            e_mw, e_iew = np.exp(-TR / t1mw), np.exp(-TR * r1iew)
+            amp = S0MW * np.sin(alpha) * (1 - e_mw) / (1 - np.cos(alpha) * e_mw) + \
+                (S0IW + S0EW) * np.sin(alpha) * (1 - e_iew) / (1 - np.cos(alpha) * e_iew)
+        decay = np.exp(-1e-6 * np.array([80.0, 20.0, 25.0]))
+        assert s[3 * fi] == pytest.approx(amp * ff, rel=1e-3)  # TE = 1 us: decay ~ 1e-4
+        assert s[3 * fi] <= amp * ff * decay.max() * (1 + 1e-9)
+
+
+def test_mcrmwi_no_exchange_limit():
+    """kIEWM -> 0 turns the BM steady state into the Bloch model."""
+    _, _, md_x = _mcrmwi_md(True, cplx=False)
+    _, _, md_b = _mcrmwi_md(False, cplx=False)
+    base = [1.3, 0.15, 0.6, 1.1]
+    rest = [80.0, 20.0, 25.0, 0.0, 0.0]
+    ex = np.array([[0.95], [0.0], [1.0], [0.0], [0.0], [0.0], [0.0]])
+    sx = orc.fwd_value(md_x, np.array(base + [1e-7] + rest)[:, None], ex)
+    sb = orc.fwd_value(md_b, np.array(base + rest)[:, None], ex)
+    np.testing.assert_allclose(sx, sb, rtol=1e-5)
+
+
+def test_mcrmwi_complex_phase_and_sum():
+    """compute_gremwi_signal: re/im of sum_p a_p exp(-TE R2s_p) exp(i(2 pi TE
+    (f_p + f_bkg) B0 gamma + pini)) times ff; the magnitude model is |.| of
+    the complex one."""
+    te = np.array([3e-3, 7e-3, 12e-3])
+    _, _, mc = _mcrmwi_md(True, cplx=True, te=te)
+    _, _, mm = _mcrmwi_md(True, cplx=False, te=te)
+    amp_par = [1.3, 0.15, 0.6, 1.1, 2.5]
+    r2 = [80.0, 20.0, 25.0]
+    fr = [0.08, -0.02]
+    bkg = [0.01, -0.02, 0.03, 0.0]
+    ex = np.array([[1.0], [0.3], [0.9], [0.05], [0.0], [-0.05], [0.02]])
+    thc = np.array(amp_par + r2 + fr + bkg + [0.1])[:, None]
+    thm = np.array(amp_par + r2 + fr)[:, None]
+    sc = orc.fwd_value(mc, thc, ex)[:, 0]
+    ex_m = ex.copy()
+    sm = orc.fwd_value(mm, thm, ex_m)[:, 0]
+    nte, nfa = 3, 4
+    re, im = sc[: nte * nfa], sc[nte * nfa:]
+    # the magnitude model has no background field, pini or dpini
+    thz = np.array(amp_par + r2 + fr + [0, 0, 0, 0] + [0.0])[:, None]
+    ex0 = ex.copy()
+    ex0[1] = 0.0
+    ex0[3:] = 0.0
+    s0 = orc.fwd_value(mc, thz, ex0)[:, 0]
+    np.testing.assert_allclose(np.hypot(s0[: nte * nfa], s0[nte * nfa:]), sm, rtol=1e-12)
+    # a common phase shift (dpini) rotates every (re, im) pair rigidly
+    thr = thc.copy()
+    thr[-1] += 0.7
+    sr = orc.fwd_value(mc, thr, ex)[:, 0]
+    z = (re + 1j * im) * np.exp(1j * 0.7)
+    np.testing.assert_allclose(sr[: nte * nfa], z.real, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(sr[nte * nfa:], z.imag, rtol=1e-10, atol=1e-14)
+    # ff scales the signal linearly
+    ex2 = ex.copy()
+    ex2[2] *= 2
+    np.testing.assert_allclose(orc.fwd_value(mc, thc, ex2)[:, 0], 2 * sc, rtol=1e-12)

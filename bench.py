@@ -167,6 +167,232 @@ def theta_of(z, md):
     return np.stack(out)
 
 
+def device_fwd(ctx, md, theta, extra=None):
+    """Noiseless signal of the library's own forward model (gc_fwd)."""
+    import torch
+
+    from gacelle_amd import runtime
+
+    nv = theta.shape[1]
+    s = torch.empty((md.n_meas, nv), dtype=torch.float32, device=theta.device)
+    ctx.check(ctx.lib.gc_fwd(ctx.handle, ctypes.byref(md), runtime.ptr(theta), runtime.ptr(extra), runtime.ptr(s), nv))
+    return s
+
+
+def timed(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def bench_c3(ctx, device, iterations):
+    """Joint R1-R2* VFA-SPGR, 4 flip angles x 12 echoes, TR 30 ms, three fully
+    masked volumes of 0.4M / 0.9M / 1.6M voxels fitted in one batched launch."""
+    import torch
+
+    from gacelle_amd import _abi, models, runtime
+    from gacelle_amd.askadam import askadam
+
+    te = np.arange(2, 25, 2) * 1e-3
+    m = models.gpuJointR1R2starMapping(te, 30e-3, np.array([3.0, 8.0, 15.0, 25.0]))
+    f = askadam.check_set_default_basic(dict(modelParams=["M0", "R1", "R2star"], lb=[0, 0.1, 0.1], ub=[2, 10, 200],
+                                             iteration=iterations, tol=0, convergenceValue=-1,
+                                             convergenceStepTol=0, convergenceGradTol=0, verbose=False))
+    md = m.gc_model(f, "askadam", None)
+    cfg = askadam().make_cfg(f, m)
+    g = torch.Generator(device=device)
+    g.manual_seed(3)
+    vols = []
+    for nx, ny, nz in ((100, 100, 40), (150, 150, 40), (200, 200, 40)):
+        nv = nx * ny * nz
+        th = torch.stack([torch.rand(nv, generator=g, device=device) * 1.5 + 0.5,
+                          torch.rand(nv, generator=g, device=device) * 1.5 + 0.5,
+                          torch.rand(nv, generator=g, device=device) * 50 + 10]).contiguous()
+        b1 = (torch.rand((1, nv), generator=g, device=device) * 0.4 + 0.8).contiguous()
+        s = device_fwd(ctx, md, th, b1)
+        sigma = float(th[0].mean()) / 80.0
+        n1 = torch.randn(s.shape, generator=g, device=device)
+        n2 = torch.randn(s.shape, generator=g, device=device)
+        y = torch.sqrt((s + sigma * n1) ** 2 + (sigma * n2) ** 2).contiguous()
+        # start at M0 = 1, R1 = 1 /s, R2* = 30 /s (logit space), as the R2* workload
+        zc = [float(np.log(t / (1 - t))) for t in ((1.0 - 0) / 2, (1.0 - 0.1) / 9.9, (30.0 - 0.1) / 199.9)]
+        z0 = torch.tensor(zc, dtype=torch.float32, device=device)[:, None].repeat(1, nv).contiguous()
+        vols.append((y, b1, z0, th))
+
+    def run():
+        arr = (_abi.gc_volume * 3)()
+        keep = []
+        for i, (y, b1, z0, _) in enumerate(vols):
+            z = z0.clone()
+            arr[i].nv = y.shape[1]
+            arr[i].y_dev = runtime.ptr(y)
+            arr[i].extra_dev = runtime.ptr(b1)
+            arr[i].z_dev = runtime.ptr(z)
+            keep.append(z)
+        ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(cfg), arr, 3))
+        return arr, keep
+
+    run()  # warm-up
+    t, (arr, zs) = timed(run)
+    nv_total = sum(v[0].shape[1] for v in vols)
+    err = {}
+    for p, name in enumerate(("M0", "R1", "R2star")):
+        rel = []
+        for (y, b1, z0, th), z in zip(vols, zs):
+            est = md.lb[p] + (md.ub[p] - md.lb[p]) * torch.sigmoid(z[p])
+            rel.append(torch.abs(est - th[p]) / th[p])
+        err[name] = float(torch.median(torch.cat(rel)))
+    return dict(workload="joint R1-R2* 4 FA x 12 TE (TE 2:2:24 ms, TR 30 ms), Rician SNR 80, volumes 0.4M/0.9M/1.6M "
+                         f"voxels in one batched launch, {iterations} iterations L1/Adam",
+                voxels=nv_total, seconds=t, voxels_per_s=nv_total / t,
+                voxel_iterations_per_s=nv_total * iterations / t, kernel_ms=arr[0].kernel_ms,
+                launches=arr[0].kernel_launches, median_rel_err=err)
+
+
+def bench_c4(ctx, device, iterations, nv=500_000, walkers=16):
+    """NEXI Karger exchange, 6 shells x 2 diffusion times, 64-point
+    Gauss-Legendre quadrature, Rician SNR 50; Goodman-Weare with 16 walkers."""
+    import torch
+
+    from gacelle_amd import _abi, models, runtime
+    from gacelle_amd.mcmc import mcmc
+
+    b = np.repeat(np.arange(1, 7, dtype=float), 2)
+    D = np.tile([20.0, 30.0], 6)
+    m = models.gpuNEXI(b, D, quad="gl64", rician=True)
+    names = ["fa", "Da", "De", "ra", "noise"]
+    f = mcmc.check_set_default_basic(dict(modelParams=names, lb=list(m.lb[:4]) + [0.01], ub=list(m.ub[:4]) + [0.1]))
+    md = m.gc_model(f, "mcmc", None)
+    g = torch.Generator(device=device)
+    g.manual_seed(4)
+    fa = torch.rand(nv, generator=g, device=device) * 0.6 + 0.2
+    tex = torch.rand(nv, generator=g, device=device) * 48 + 2
+    th = torch.stack([fa, torch.rand(nv, generator=g, device=device) * 1.5 + 1.5,
+                      torch.rand(nv, generator=g, device=device) + 0.5, (1 - fa) / tex,
+                      torch.full((nv,), 0.02, device=device)]).contiguous()
+    sigma = torch.full((1, nv), 0.02, device=device)
+    # noiseless spherical mean without the Rician correction, then Rician noise
+    md0 = models.gpuNEXI(b, D, quad="gl64").gc_model(
+        mcmc.check_set_default_basic(dict(modelParams=names[:4], lb=list(m.lb[:4]), ub=list(m.ub[:4]))), "mcmc", None)
+    md0.n_par = 5
+    s = device_fwd(ctx, md0, th, None)
+    y = torch.sqrt((s + 0.02 * torch.randn(s.shape, generator=g, device=device)) ** 2 +
+                   (0.02 * torch.randn(s.shape, generator=g, device=device)) ** 2).contiguous()
+    c = _abi.gc_mcmc_cfg()
+    c.algorithm, c.iteration, c.thinning = _abi.GC_MCMC_GW, iterations, 20
+    c.n_burnin = int(iterations * 0.1)
+    c.repetition, c.n_walker = 1, walkers
+    c.ensemble_update = _abi.GC_ENSEMBLE_SIMULTANEOUS
+    c.rng_mode, c.noise_slot, c.n_var, c.seed = _abi.GC_RNG_NATIVE, 4, 5, 48463
+    c.step_size, c.start_range = 2.0, 1e-3
+    mean = torch.empty((5, nv), device=device)
+    std = torch.empty((5, nv), device=device)
+
+    def run(cfg):
+        io = _abi.gc_mcmc_io()
+        io.nv = nv
+        io.y_dev = runtime.ptr(y)
+        io.extra_dev = runtime.ptr(sigma)
+        io.x0_dev = runtime.ptr(th)
+        io.mean_dev = runtime.ptr(mean)
+        io.std_dev = runtime.ptr(std)
+        ctx.check(ctx.lib.gc_mcmc_run(ctx.handle, ctypes.byref(md), ctypes.byref(cfg), ctypes.byref(io)))
+        return io
+
+    warm = _abi.gc_mcmc_cfg.from_buffer_copy(c)
+    warm.iteration, warm.n_burnin = 20, 2
+    run(warm)
+    t, io = timed(lambda: run(c))
+    steps = float(nv) * walkers * iterations
+    err = {n: float(torch.median(torch.abs(mean[q] - th[q]) / th[q])) for q, n in enumerate(names[:4])}
+    return dict(workload=f"NEXI (6 shells x 2 Delta, GL64, Rician mean) Goodman-Weare, {nv} voxels x {walkers} walkers x "
+                         f"{iterations} steps (simultaneous update, native stream)",
+                seconds=t, walker_steps_per_s=steps / t, kernel_ms=io.kernel_ms,
+                transcendentals_per_s=steps * 3072 / t, posterior_mean_median_rel_err=err)
+
+
+def bench_c5(ctx, device, iterations, dims=(128, 128, 64)):
+    """MCR-MWI complex 3-pool GRE (BM exchange), 24 echoes x 4 flip angles,
+    complex Gaussian SNR 150, askadam with 3D TV on MWF (one launch per
+    iteration through the step kernel)."""
+    import torch
+
+    from gacelle_amd import _abi, models, runtime, utils
+    from gacelle_amd.askadam import askadam
+
+    te = np.arange(2, 49, 2) * 1e-3
+    m = models.gpuMCRMWI(te, 38e-3, np.array([5.0, 10.0, 20.0, 50.0]))
+    f = m.check_set_default(dict(isComplex=True, isFitExchange=True))
+    m.updateProperty(f)
+    f.update(modelParams=list(m.modelParams), lb=list(m.lb), ub=list(m.ub), iteration=iterations, tol=0,
+             convergenceValue=-1, convergenceStepTol=0, convergenceGradTol=0, TVmode="3D", voxelSize=[1, 1, 1],
+             verbose=False, **{"lambda": [1e-3]})
+    f = askadam.check_set_default_basic(f)
+    md = m.gc_model(f, "askadam", None)
+    cfg = askadam().make_cfg(f, m)
+    nx, ny, nz = dims
+    nv = nx * ny * nz
+    g = torch.Generator(device=device)
+    g.manual_seed(5)
+    # piecewise-smooth ground truth: smooth fields with a step along x
+    xs = torch.linspace(0, 1, nx, device=device)[:, None, None].expand(nx, ny, nz)
+    ys = torch.linspace(0, 1, ny, device=device)[None, :, None].expand(nx, ny, nz)
+    zs = torch.linspace(0, 1, nz, device=device)[None, None, :].expand(nx, ny, nz)
+    step = (xs > 0.5).float()
+
+    def field(lo, hi, ph):
+        v = 0.5 + 0.25 * torch.sin(2 * np.pi * (xs + ph)) * torch.cos(2 * np.pi * (ys - ph)) + 0.2 * step - 0.1
+        return (lo + (hi - lo) * v.clamp(0, 1)).permute(2, 1, 0).reshape(-1)  # column-major voxel order
+
+    maps = {"S0": field(0.8, 1.2, 0.1), "MWF": field(0.05, 0.2, 0.2), "IWF": field(0.4, 0.8, 0.3),
+            "R1IEW": field(0.8, 1.4, 0.4), "kIEWM": field(1.0, 3.0, 0.5), "R2sMW": field(80, 150, 0.6),
+            "R2sIW": field(15, 30, 0.7), "R2sEW": field(15, 30, 0.8), "freqMW": field(0.02, 0.1, 0.9),
+            "freqIW": field(-0.05, 0.0, 0.15)}
+    rows = [maps[n] for n in m.modelParams if n in maps]
+    rows += [torch.zeros(nv, device=device)] * 4 + [torch.zeros(nv, device=device)]  # dfreqBKG x 4, dpini
+    th = torch.stack(rows).contiguous()
+    ex = torch.zeros((7, nv), device=device)
+    ex[0] = 1.0
+    ex[2] = 1.0
+    ex[3:] = torch.randn((4, nv), generator=g, device=device) * 0.02
+    s = device_fwd(ctx, md, th, ex)
+    sigma = float(maps["S0"].mean()) / 150.0
+    y = (s + sigma * torch.randn(s.shape, generator=g, device=device)).contiguous()
+    mask = np.ones(dims, bool)
+    vox = torch.from_numpy(utils.find_mask(mask)).to(device)
+    z0 = torch.zeros((md.n_par, nv), device=device)
+
+    def run(c):
+        z = z0.clone()
+        arr = (_abi.gc_volume * 1)()
+        arr[0].nv = nv
+        arr[0].y_dev = runtime.ptr(y)
+        arr[0].extra_dev = runtime.ptr(ex)
+        arr[0].vox_dev = runtime.ptr(vox)
+        for k in range(3):
+            arr[0].dims[k] = dims[k]
+        arr[0].z_dev = runtime.ptr(z)
+        ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(c), arr, 1))
+        return arr[0]
+
+    warm = _abi.gc_askadam_cfg.from_buffer_copy(cfg)
+    warm.iteration = 5
+    run(warm)
+    t, v = timed(lambda: run(cfg))
+    vi = float(nv) * iterations
+    bytes_per_vi = 4 * md.n_meas + 2 * 3 * md.n_par * 4 + 6 * 4  # y, (z, m, v) r/w, TV neighbour reads
+    return dict(workload=f"MCR-MWI complex BM 24 TE x 4 FA, {dims[0]}x{dims[1]}x{dims[2]} ({nv} voxels), 3D TV on "
+                         f"MWF lambda 1e-3, {iterations} iterations L1/Adam, one launch per iteration",
+                seconds=t, voxel_iterations_per_s=vi / t, kernel_ms=v.kernel_ms, launches=v.kernel_launches,
+                avg_launch_ms=v.kernel_ms / max(v.kernel_launches, 1),
+                hbm_gbs_algorithmic=vi * bytes_per_vi / (v.kernel_ms * 1e-3) / 1e9,
+                loss=v.loss, loss_reg=v.loss_reg)
+
+
 def cpu_baseline_askadam(s0, r2, sigma, iterations, nthreads, nsample, seed=7):
     """The CPU path: the oracle's FP32 restatement, OpenMP over host cores."""
     from oracle import oracle as orc
@@ -219,6 +445,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mh", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--configs", default="c3,c4,c5", help="further workloads to measure (c3, c4, c5 or none)")
+    ap.add_argument("--c3-iterations", type=int, default=2000)
+    ap.add_argument("--c4-iterations", type=int, default=500)
+    ap.add_argument("--c5-iterations", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
@@ -321,6 +551,16 @@ def main():
                                                    R2star=float(np.median(np.abs(m[1] - r2) / r2)))
         log(f"MH: {t_mh:.2f} s, {mh['samples_per_s']:.3e} chain-steps/s")
 
+    # ---- further workloads (SURVEY configs 3-5), reported beside the line ------
+    extra = {}
+    wanted = [c.strip() for c in args.configs.split(",") if c.strip() and c.strip() != "none"]
+    for name, fn, it in (("c3", bench_c3, args.c3_iterations), ("c4", bench_c4, args.c4_iterations),
+                         ("c5", bench_c5, args.c5_iterations)):
+        if name in wanted:
+            extra[name] = fn(ctx, device, it)
+            log(f"{name}: {json.dumps(extra[name])}")
+            torch.cuda.empty_cache()
+
     # ---- CPU path -------------------------------------------------------------
     cpu = None
     if not args.skip_cpu:
@@ -388,6 +628,7 @@ def main():
         "fused": {"final_loss": fused_loss, "median_rel_err": err_fused},
         "tape": tape,
         "mcmc": mh,
+        "workloads": extra,
     }
     print(json.dumps(out), flush=True)
 

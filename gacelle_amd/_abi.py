@@ -10,7 +10,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint64, c_void_p
 
-GC_ABI_VERSION = 1
+GC_ABI_VERSION = 2
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
@@ -111,6 +111,13 @@ class gc_askadam_cfg(ctypes.Structure):
         ("grad_tol", c_double),
         ("lambda_", c_double * GC_MAX_REG),
         ("voxel_size", c_double * 3),
+        ("weight_update_interval", c_int32),
+        ("outlier_check_window", c_int32),
+        ("outlier_weight", c_double),
+        ("outlier_voxel_thres", c_double),
+        ("outlier_pop_thres", c_double),
+        ("outlier_init_thres", c_double),
+        ("outlier_init_pop_thres", c_double),
     ]
 
 

@@ -144,8 +144,6 @@ class askadam:
         parameters[, extraData]), ...]."""
         model = _resolve_model(userfuncCell)
         fitting = self.check_set_default_basic(fitting)
-        if fitting["robustConvergence"]:
-            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "robustConvergence is not supported by the fused path")
         ctx = runtime.context()
         lib = ctx.lib
         start = time.time()
@@ -291,6 +289,13 @@ class askadam:
         vs = list(np.atleast_1d(fitting["voxelSize"]).astype(float)) + [1.0, 1.0, 1.0]
         for k in range(3):
             c.voxel_size[k] = vs[k]
+        c.weight_update_interval = int(fitting["weightUpdateInterval"])
+        c.outlier_check_window = int(fitting["outlierCheckWindow"])
+        c.outlier_weight = float(fitting["outlierWeight"])
+        c.outlier_voxel_thres = float(fitting["outlierVoxelThres"])
+        c.outlier_pop_thres = float(fitting["outlierPopThres"])
+        c.outlier_init_thres = float(fitting["outlierInitThres"])
+        c.outlier_init_pop_thres = float(fitting["outlierInitPopThres"])
         return c
 
     def _assemble(self, ctx, md, fitting, model, v, m) -> dict:

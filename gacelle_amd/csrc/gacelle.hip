@@ -215,7 +215,6 @@ int gc_askadam_run(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, g
   int rc = check_model(ctx, md);
   if (rc) return rc;
   if (!cfg || !vols || n_vol < 1 || n_vol > GC_MAX_VOL) return set_err(ctx, GC_ERR_ARG, "gc_askadam_run: bad argument");
-  if (cfg->robust) return set_err(ctx, GC_ERR_UNSUPPORTED, "robustConvergence is not supported");
   if (cfg->optimiser < 0 || cfg->optimiser > 2) return set_err(ctx, GC_ERR_ARG, "unknown optimiser");
   if (cfg->transform != GC_TRANSFORM_SIGMOID && cfg->transform != GC_TRANSFORM_LINEAR)
     return set_err(ctx, GC_ERR_ARG, "unknown parameter transform");

@@ -45,11 +45,30 @@ struct VolRun {
   int chunks = 1;       // chunks per partial-sum row (k_reduce_rows)
   DevBuf pp[2][3];      // step form: intermediate states of a launch sequence
   DevBuf nbr;           // step form with TV: [2*nd][nv] grid neighbours
+  // robustConvergence (utils/askadam.m:448-510, 541-566)
+  std::vector<double> pvl_init, pvl, hist;  // per-voxel mean loss; history [W][nv]
+  std::vector<uint8_t> main_mask, mask_conv;
+  std::vector<int> flag_count;
+  int64_t n_conv = 0;   // voxels in mask_conv
+  int reprime = 0;
+  DevBuf wmod, wmod_pin, mconv, pvl0_d, pvl_d;
+  bool pin_mod = false; // the pinned block started with modulated weights
 };
 
 // per-iteration sums kept for the host loop: fidelity, step^2, |z|^2, |g|^2,
-// regularisation
-constexpr int kSum = 5;
+// regularisation, per-voxel mean loss over the convergence set
+constexpr int kSum = kStepSums;
+
+// MATLAB median of a vector (mean of the two middle values for even n)
+inline double median_of(std::vector<double>& v) {
+  const size_t n = v.size();
+  if (n == 0) return std::nan("");
+  auto mid = v.begin() + n / 2;
+  std::nth_element(v.begin(), mid, v.end());
+  const double hi = *mid;
+  if (n % 2) return hi;
+  return 0.5 * (*std::max_element(v.begin(), mid) + hi);
+}
 
 struct AskDriver {
   gc_ctx* ctx;
@@ -69,6 +88,9 @@ struct AskDriver {
   bool desc_in_flight = false;  // an async copy may still read h_vols/h_es
   bool step_mode = false;       // one launch per iteration (TV or streamed model)
   bool use_tv = false;
+  bool robust = false;
+  bool replaying = false;       // a replay launch: no per-voxel loss outputs
+  bool replay_pinned = false;   // ... from the pinned block (its weight modulator)
   DevBuf d_steps;               // step form: [K][n_vol] descriptors
   HostBuf h_steps;
   // start/stop events attached to every fused launch (kernel-only timing),
@@ -123,7 +145,11 @@ struct AskDriver {
   // stream_model: the model's data do not fit in registers (step form only)
   int setup(bool stream_model) {
     use_tv = cfg.n_reg > 0 && cfg.lambda[0] > 0;
-    step_mode = use_tv || stream_model;
+    robust = cfg.robust != 0;
+    step_mode = use_tv || stream_model || robust;
+    if (robust && (cfg.weight_update_interval < 1 || cfg.outlier_check_window < 1 || cfg.convergence_window < 1))
+      return set_err(ctx, GC_ERR_ARG, "robustConvergence needs weightUpdateInterval, outlierCheckWindow and "
+                                      "convergenceWindow >= 1");
     if (use_tv) {
       if (cfg.n_reg > GC_MAX_REG) return set_err(ctx, GC_ERR_ARG, "at most %d regularised maps", GC_MAX_REG);
       if (cfg.tv_mode != GC_TV_2D && cfg.tv_mode != GC_TV_3D) return set_err(ctx, GC_ERR_ARG, "unknown TVmode");
@@ -160,6 +186,12 @@ struct AskDriver {
     } else {
       K = std::min(k_mem, 8192);
     }
+    if (robust) {
+      // outlier checks and convergence-set refreshes fall on block ends
+      int g = std::gcd(cfg.weight_update_interval, cfg.convergence_window);
+      if (cfg.block_iters > 0) g = std::gcd(g, cfg.block_iters);
+      K = g;
+    }
     K = std::max(1, std::min(K, std::max(1, cfg.iteration)));
     if (K > 16383) return set_err(ctx, GC_ERR_ARG, "iterationsPerLaunch %d exceeds 16383", K);
     const int P = md.n_par;
@@ -179,6 +211,19 @@ struct AskDriver {
       if (use_tv) {
         const int rc = build_neighbours(i);
         if (rc) return rc;
+      }
+      if (robust) {
+        GC_HIP(ctx, R.wmod.alloc(sizeof(float) * R.nv));
+        GC_HIP(ctx, R.wmod_pin.alloc(sizeof(float) * R.nv));
+        GC_HIP(ctx, R.mconv.alloc(R.nv));
+        GC_HIP(ctx, R.pvl0_d.alloc(sizeof(float) * R.nv));
+        GC_HIP(ctx, R.pvl_d.alloc(sizeof(float) * R.nv));
+        hipLaunchKernelGGL(k_fill, dim3(nblocks(R.nv)), dim3(kBlock), 0, ctx->stream, R.wmod.as<float>(), R.nv, 1.0f);
+        GC_HIP(ctx, hipMemsetAsync(R.mconv.p, 1, R.nv, ctx->stream));
+        R.main_mask.assign(R.nv, 1);
+        R.mask_conv.assign(R.nv, 1);
+        R.flag_count.assign(R.nv, 0);
+        R.n_conv = R.nv;
       }
       GC_HIP(ctx, hipMemcpyAsync(R.set[0][0].p, vols[i].z_dev, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
       GC_HIP(ctx, hipMemsetAsync(R.set[0][1].p, 0, pbytes, ctx->stream));
@@ -412,6 +457,14 @@ struct AskDriver {
           d.z_last = (k == n - 1 && zl[i] >= 0) ? R.zlast[zl[i]].as<float>() : nullptr;
           d.partials = R.partials.as<float>();
           d.nbr = use_tv ? R.nbr.as<int32_t>() : nullptr;
+          if (robust) {
+            const int e = e0[i] + k;
+            // update_outlier_mask returns modulated weights only on a check epoch
+            if (k == 0 && e - 1 >= 1 && (e - 1) % cfg.weight_update_interval == 0)
+              d.wmod = (replaying && replay_pinned && R.pin_mod) ? R.wmod_pin.as<float>() : R.wmod.as<float>();
+            d.mconv = R.mconv.as<uint8_t>();
+            if (!replaying) d.pvl_out = (k == n - 1) ? R.pvl_d.as<float>() : (e == 1 ? R.pvl0_d.as<float>() : nullptr);
+          }
           d.k = k;
           d.n_epochs = n;
           d.update = (e0[i] + k < iteration_cap) ? 1 : 0;
@@ -457,8 +510,10 @@ struct AskDriver {
 
   // One iteration of the host-side loop logic (utils/askadam.m:264-292 minus
   // the update itself).  Returns true when the volume stops at `epoch`.
+  // loss_c: the loss the convergence signal follows (the full loss, or with
+  // robustConvergence the mean per-voxel loss over the convergence set)
   bool epoch_logic(VolRun& R, int epoch, double sum_l, double sum_step_prev, double sum_den, double sum_g,
-                   double sum_reg, double* trace) {
+                   double sum_reg, double* trace, const double* loss_c_robust = nullptr) {
     const double count = double(md.n_meas) * double(R.nv);
     R.fid = sum_l / count;
     R.reg = sum_reg / double(R.nv);  // spatial_total_variation: / Nmasked
@@ -473,13 +528,14 @@ struct AskDriver {
     }
     (void)new_min;
     // update_convergence_signals (:512-632)
+    const double loss_c = loss_c_robust ? *loss_c_robust : R.loss;
     if (cfg.convergence_model == GC_CONV_EMA) {
-      const double e_new = cfg.ema_decay * R.ema + (1.0 - cfg.ema_decay) * R.loss;
+      const double e_new = cfg.ema_decay * R.ema + (1.0 - cfg.ema_decay) * loss_c;
       R.conv = std::fabs(e_new - R.ema) / (std::fabs(R.ema) + 1e-8);
       R.ema = e_new;
     } else {
       std::rotate(R.cbuf.begin(), R.cbuf.begin() + 1, R.cbuf.end());
-      R.cbuf.back() = R.loss;
+      R.cbuf.back() = loss_c;
       const int n = int(R.cbuf.size());
       double sx = 0, sy = 0, sxx = 0, sxy = 0;
       for (int i = 0; i < n; ++i) {
@@ -489,8 +545,9 @@ struct AskDriver {
       const double den = n * sxx - sx * sx;
       R.conv = -((den != 0) ? (n * sxy - sx * sy) / den : 0.0);
     }
-    if (R.conv > cfg.convergence_value || epoch <= cfg.convergence_window) R.pc = 0;
-    else if ((R.min_loss - R.loss) > cfg.convergence_value) R.pc = 0;
+    if (R.reprime > 0) { --R.reprime; R.pc = 0; }
+    else if (R.conv > cfg.convergence_value || epoch <= cfg.convergence_window) R.pc = 0;
+    else if (!robust && (R.min_loss - R.loss) > cfg.convergence_value) R.pc = 0;
     else R.pc += 1;
     if (cfg.step_tol > 0) {
       const double num = (epoch == 1) ? 0.0 : sum_step_prev;
@@ -509,6 +566,78 @@ struct AskDriver {
     else if (cfg.grad_tol > 0 && R.pg > cfg.patience_grad) stop = GC_STOP_GRAD;
     R.stop = stop;
     return stop != GC_STOP_NONE;
+  }
+
+  // robustConvergence at a block end `epoch`: update_outlier_mask
+  // (utils/askadam.m:448-510) on a check epoch, the convergence-set refresh
+  // on a convergenceWindow boundary (:541-566), and the mean per-voxel loss
+  // over the convergence set.  The new weight modulator and set are
+  // uploaded for the next block.
+  int robust_block_end(int i, int epoch, double& loss_c) {
+    VolRun& R = run[i];
+    const int64_t nv = R.nv;
+    const int W = cfg.outlier_check_window;
+    bool mod_changed = false;
+    if (epoch % cfg.weight_update_interval == 0) {
+      std::rotate(R.hist.begin(), R.hist.begin() + nv, R.hist.end());
+      std::copy(R.pvl.begin(), R.pvl.end(), R.hist.end() - nv);
+      std::vector<double> impA(nv), impB(nv);
+      for (int64_t v = 0; v < nv; ++v) {
+        const double h0 = R.hist[v], h1 = R.hist[size_t(W - 1) * nv + v];
+        impA[v] = (h0 - h1) / (h0 + 1e-8);
+        impB[v] = (R.pvl_init[v] - R.pvl[v]) / (R.pvl_init[v] + 1e-8);
+      }
+      std::vector<double> a = impA, b = impB;
+      const double medA = median_of(a), medB = median_of(b);
+      for (int64_t v = 0; v < nv; ++v) {
+        const bool failA = impA[v] < cfg.outlier_voxel_thres && medA > cfg.outlier_pop_thres;
+        const bool failB = impB[v] < cfg.outlier_init_thres && medB > cfg.outlier_init_pop_thres;
+        if (failA && failB) R.flag_count[v] += 1;
+        else R.flag_count[v] = std::max(R.flag_count[v] - 1, 0);
+        R.main_mask[v] = R.flag_count[v] == 0;
+      }
+      mod_changed = true;
+    }
+    if (epoch % cfg.convergence_window == 0 && R.main_mask != R.mask_conv) {
+      R.mask_conv = R.main_mask;
+      R.reprime = cfg.convergence_window;
+      R.n_conv = 0;
+      for (int64_t v = 0; v < nv; ++v) R.n_conv += R.mask_conv[v];
+      GC_HIP(ctx, hipMemcpyAsync(R.mconv.p, R.mask_conv.data(), nv, hipMemcpyHostToDevice, ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    double acc = 0.0;
+    for (int64_t v = 0; v < nv; ++v)
+      if (R.mask_conv[v]) acc += R.pvl[v];
+    loss_c = R.n_conv ? acc / double(R.n_conv) : std::nan("");
+    if (mod_changed) {
+      std::vector<float> m(nv);
+      for (int64_t v = 0; v < nv; ++v) m[v] = R.main_mask[v] ? 1.0f : float(cfg.outlier_weight);
+      GC_HIP(ctx, hipMemcpyAsync(R.wmod.p, m.data(), sizeof(float) * nv, hipMemcpyHostToDevice, ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return GC_OK;
+  }
+
+  // per-voxel mean losses of a finished block (block end; epoch 1 as well)
+  int fetch_pvl(int i, bool first) {
+    VolRun& R = run[i];
+    std::vector<float> h(R.nv);
+    GC_HIP(ctx, hipMemcpyAsync(h.data(), R.pvl_d.p, sizeof(float) * R.nv, hipMemcpyDeviceToHost, ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    R.pvl.assign(h.begin(), h.end());
+    if (first) {
+      // starting-point evaluation = epoch 1 (written at the block end when K = 1)
+      if (K > 1) {
+        GC_HIP(ctx, hipMemcpyAsync(h.data(), R.pvl0_d.p, sizeof(float) * R.nv, hipMemcpyDeviceToHost, ctx->stream));
+        GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      }
+      R.pvl_init.assign(h.begin(), h.end());
+      R.hist.resize(size_t(cfg.outlier_check_window) * R.nv);
+      for (int k = 0; k < cfg.outlier_check_window; ++k)
+        std::copy(R.pvl_init.begin(), R.pvl_init.end(), R.hist.begin() + size_t(k) * R.nv);
+    }
+    return GC_OK;
   }
 
   template <class Model, int LOSS, bool HAS_W>
@@ -567,6 +696,10 @@ struct AskDriver {
         if (!R.active) continue;
         const double* s = sums + size_t(i) * K * kSum;
         double* trace = vols[i].loss_trace_host;
+        if (robust) {
+          const int rcp = fetch_pvl(i, !R.have_init);
+          if (rcp) return rcp;
+        }
         if (!R.have_init) {
           // starting-point evaluation (:197-219) equals the first iteration's
           R.fid = s[0] / (double(md.n_meas) * double(R.nv));
@@ -586,7 +719,16 @@ struct AskDriver {
           const double prev_min = R.min_loss;
           const double* sk = s + size_t(k) * kSum;
           const double step_prev = (k == 0) ? R.carry_step : sk[1 - kSum];
-          stopped = epoch_logic(R, epoch, sk[0], step_prev, sk[2], sk[3], sk[4], trace);
+          double loss_c = 0.0;
+          if (robust) {
+            if (k == nep[i] - 1) {
+              const int rcr = robust_block_end(i, epoch, loss_c);
+              if (rcr) return rcr;
+            } else {
+              loss_c = R.n_conv ? sk[5] / double(R.n_conv) : std::nan("");
+            }
+          }
+          stopped = epoch_logic(R, epoch, sk[0], step_prev, sk[2], sk[3], sk[4], trace, robust ? &loss_c : nullptr);
           if (R.min_loss < prev_min) {
             if (k == nep[i] - 1) {
               R.min_zlast = zl[i];
@@ -594,6 +736,14 @@ struct AskDriver {
             } else {
               R.min_zlast = -1;
               R.pinned = in[i]; R.pin_e0 = e0[i]; R.pin_r = k;
+              if (robust) {
+                // keep the weight modulator this block started with for its replay
+                const int e = e0[i];
+                R.pin_mod = e - 1 >= 1 && (e - 1) % cfg.weight_update_interval == 0;
+                if (R.pin_mod)
+                  GC_HIP(ctx, hipMemcpyAsync(R.wmod_pin.p, R.wmod.p, sizeof(float) * R.nv, hipMemcpyDeviceToDevice,
+                                             ctx->stream));
+              }
             }
           }
           if (stopped) break;
@@ -632,7 +782,11 @@ struct AskDriver {
     if (o > 2) o = (set + 1) % 3;
     e0[vi] = e0v; nep[vi] = r + 1; in[vi] = set; out[vi] = o; zl[vi] = zb;
     for (int i = 0; i < n_vol; ++i) if (i != vi) { in[i] = run[i].cur; out[i] = run[i].cur; }
-    return launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, cfg.iteration);
+    replaying = true;
+    replay_pinned = (set == R.pinned);
+    const int rc = launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, cfg.iteration);
+    replaying = replay_pinned = false;
+    return rc;
   }
 
   template <class Model, int LOSS, bool HAS_W>

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <numeric>
 #include <vector>
 
 #include "../../include/gacelle.h"

@@ -376,7 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
 // parameters, and models whose measurements do not fit in registers
 // (streamed from HBM every iteration).  Each launch advances every active
 // volume by one iteration from (z, m, v)_in to (z, m, v)_out.
-constexpr int kStepSums = 5;  // fidelity, step^2, |z|^2, |g|^2, regularisation
+// fidelity, step^2, |z|^2, |g|^2, regularisation, per-voxel mean loss over
+// the convergence set (robustConvergence)
+constexpr int kStepSums = 6;
 
 struct StepVol {
   const float* y;
@@ -391,6 +393,9 @@ struct StepVol {
   float* z_last;       // z (pre-update) of this iteration, or null
   float* partials;     // [nblk][n_epochs][kStepSums]
   const int32_t* nbr;  // [2*nd][nv]: +axis, -axis neighbour G-D index per axis, -1 outside the mask
+  const float* wmod;   // robustConvergence: per-voxel weight factor of this iteration, or null
+  const uint8_t* mconv;  // robustConvergence: convergence voxel set, or null
+  float* pvl_out;      // robustConvergence: per-voxel mean loss of this iteration, or null
   int64_t nv;
   int32_t blk0, nblk;
   int32_t active;      // the volume runs this iteration
@@ -451,13 +456,14 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
   float gth[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+  const float wm = (V.wmod != nullptr) ? V.wmod[vv] : 1.f;
   model_adjoint<Model>(
       md, th, ex,
       [&](int r, float s) -> float {
         const int64_t m = Model::meas(md, r);
         const bool fin = isfinite(s);
         const float sv = fin ? s : 0.f;
-        const float wv = HAS_W ? V.w[m * nv + vv] : 1.f;
+        const float wv = (HAS_W ? V.w[m * nv + vv] : 1.f) * wm;
         float l, dl;
         loss_fn<LOSS>(sv - V.y[m * nv + vv], wv, l, dl);
         lsum += l;
@@ -467,6 +473,9 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
   float g[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * V.inv_count;
+  const float pvl = lsum / float(md.n_meas);
+  if (V.pvl_out != nullptr && valid) V.pvl_out[v] = pvl;
+  float msum = (V.mconv != nullptr && V.mconv[vv]) ? pvl : 0.f;
 
   // spatial TV: cost sum_d |theta(i + e_d) - theta(i)| / vs_d over masked i
   // (forward differences with periodic wrap, zeros outside the mask), and
@@ -539,12 +548,16 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
   }
 
   // block sums of this iteration
-  if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; reg = 0.f; }
+  if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; reg = 0.f; msum = 0.f; }
   const float t4 = wave_sum4(lsum, step, den, gn, lane);
   const float tr = wave_sum(reg);
+  const float tm = wave_sum(msum);
   __shared__ float red[kBlock / 64][kStepSums];
   if (lane < 4) red[wave][wave_sum4_index(lane)] = t4;
-  if (lane == 0) red[wave][4] = tr;
+  if (lane == 0) {
+    red[wave][4] = tr;
+    red[wave][5] = tm;
+  }
   __syncthreads();
   if (threadIdx.x < kStepSums) {
     float t = 0.f;

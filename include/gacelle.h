@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 1
+#define GC_ABI_VERSION 2
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
@@ -150,7 +150,7 @@ typedef struct gc_askadam_cfg {
   int32_t tv_mode;            /* GC_TV_2D | GC_TV_3D                    */
   int32_t n_reg;              /* number of regularised maps (0 = none)  */
   int32_t regmap[GC_MAX_REG]; /* parameter slot of each regularised map */
-  int32_t robust;             /* fitting.robustConvergence (must be 0)  */
+  int32_t robust;             /* fitting.robustConvergence              */
   int32_t reserved;
   double lr0;                 /* fitting.initialLearnRate               */
   double decay;               /* fitting.decayRate                      */
@@ -164,6 +164,15 @@ typedef struct gc_askadam_cfg {
   double grad_tol;            /* fitting.convergenceGradTol (0 = off)   */
   double lambda[GC_MAX_REG];  /* fitting.lambda                         */
   double voxel_size[3];       /* fitting.voxelSize                      */
+  /* robustConvergence (askadam.update_outlier_mask, utils/askadam.m:448-510,
+   * and the frozen convergence set of :541-566) */
+  int32_t weight_update_interval; /* fitting.weightUpdateInterval        */
+  int32_t outlier_check_window;   /* fitting.outlierCheckWindow          */
+  double outlier_weight;          /* fitting.outlierWeight               */
+  double outlier_voxel_thres;     /* fitting.outlierVoxelThres           */
+  double outlier_pop_thres;       /* fitting.outlierPopThres             */
+  double outlier_init_thres;      /* fitting.outlierInitThres            */
+  double outlier_init_pop_thres;  /* fitting.outlierInitPopThres         */
 } gc_askadam_cfg;
 
 /* One masked volume of a batched askadam run.  Inputs are device arrays;

@@ -234,7 +234,16 @@ static double tv_loss_grad(const S* img, const uint8_t* mask, const int32_t dims
 }
 
 // ===========================================================================
-// askadam().optimisation, utils/askadam.m:121-365 (robustConvergence off).
+// MATLAB median of a vector (mean of the two middle values for even n)
+static double median_of(std::vector<double> v) {
+  const size_t n = v.size();
+  if (n == 0) return std::nan("");
+  std::sort(v.begin(), v.end());
+  return (n % 2) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+
+// askadam().optimisation, utils/askadam.m:121-365, with robustConvergence
+// (update_outlier_mask :448-510, the frozen convergence set :541-566).
 struct AskOut {
   double loss, fid, reg, min_loss, min_fid, min_reg, conv;
   int32_t niter, min_iter, stop;
@@ -269,6 +278,17 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
     gimg.assign(nvox, S(0));
   }
 
+  // robustConvergence: weights are the caller's times a per-voxel modulator
+  // on the epoch after each outlier check (update_outlier_mask returns the
+  // unmodulated weights on every other epoch)
+  const bool robust = cfg.robust != 0;
+  std::vector<float> w_orig, w_cur;
+  if (robust) {
+    w_orig.assign(size_t(M) * nv, 1.0f);
+    if (w) for (size_t i = 0; i < w_orig.size(); ++i) w_orig[i] = w[i];
+    w_cur = w_orig;
+    w = w_cur.data();
+  }
   std::vector<S> elem_cur(size_t(M) * nv);
   // model_gradient: returns (loss, fid, reg) and fills g, elem_cur
   auto evaluate = [&](double& loss, double& fid, double& reg) {
@@ -308,9 +328,31 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
     loss = fid + reg;  // :102
   };
 
+  // per-voxel mean elementwise loss, mean(reshape(residuals, Nmeas, Nvol), 1)
+  auto per_voxel_loss = [&](std::vector<double>& pvl) {
+    pvl.assign(nv, 0.0);
+    for (int64_t v = 0; v < nv; ++v) {
+      double a = 0;
+      for (int m = 0; m < M; ++m) a += double(elem_cur[size_t(m) * nv + v]);
+      pvl[v] = a / M;
+    }
+  };
+
   // 2.5 starting point evaluation (:197-200)
   double loss, fid, reg;
   evaluate(loss, fid, reg);
+  std::vector<double> pvl_init, pvl;
+  if (robust) per_voxel_loss(pvl_init);
+  // 2.8 robust state (:223-241)
+  const int W = std::max(cfg.outlier_check_window, 1);
+  std::vector<double> hist;  // [W][nv], oldest first
+  std::vector<uint8_t> main_mask(nv, 1), mask_conv(nv, 1);
+  std::vector<int> flag_count(nv, 0);
+  int reprime = 0;
+  if (robust) {
+    hist.resize(size_t(W) * nv);
+    for (int k = 0; k < W; ++k) std::copy(pvl_init.begin(), pvl_init.end(), hist.begin() + size_t(k) * nv);
+  }
   if (trace) trace[0] = loss;
   double min_loss = loss, min_fid = fid, min_reg = reg;
   int min_iter = 0, epoch = 0;
@@ -332,16 +374,52 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
     if (min_loss > loss) {
       min_loss = loss; min_fid = fid; min_reg = reg; zmn = z; min_iter = epoch; elem_min = elem_cur;
     }
+    // 3.3 update_outlier_mask (:448-510)
+    double loss_c = loss;
+    if (robust) {
+      per_voxel_loss(pvl);
+      w_cur = w_orig;
+      if (cfg.weight_update_interval > 0 && epoch % cfg.weight_update_interval == 0) {
+        std::rotate(hist.begin(), hist.begin() + nv, hist.end());
+        std::copy(pvl.begin(), pvl.end(), hist.end() - nv);
+        std::vector<double> impA(nv), impB(nv);
+        for (int64_t v = 0; v < nv; ++v) {
+          const double h0 = hist[v], h1 = hist[size_t(W - 1) * nv + v];
+          impA[v] = (h0 - h1) / (h0 + 1e-8);
+          impB[v] = (pvl_init[v] - pvl[v]) / (pvl_init[v] + 1e-8);
+        }
+        const double medA = median_of(impA), medB = median_of(impB);
+        for (int64_t v = 0; v < nv; ++v) {
+          const bool failA = impA[v] < cfg.outlier_voxel_thres && medA > cfg.outlier_pop_thres;
+          const bool failB = impB[v] < cfg.outlier_init_thres && medB > cfg.outlier_init_pop_thres;
+          if (failA && failB) flag_count[v] += 1;
+          else flag_count[v] = std::max(flag_count[v] - 1, 0);
+          main_mask[v] = flag_count[v] == 0;
+          if (!main_mask[v])
+            for (int m = 0; m < M; ++m) w_cur[size_t(m) * nv + v] = float(w_orig[size_t(m) * nv + v] * cfg.outlier_weight);
+        }
+      }
+      // frozen convergence set, refreshed on convergenceWindow boundaries (:541-566)
+      if (cfg.convergence_window > 0 && epoch % cfg.convergence_window == 0 && main_mask != mask_conv) {
+        mask_conv = main_mask;
+        reprime = cfg.convergence_window;
+      }
+      double a = 0;
+      int64_t cnt = 0;
+      for (int64_t v = 0; v < nv; ++v)
+        if (mask_conv[v]) { a += pvl[v]; ++cnt; }
+      loss_c = cnt ? a / double(cnt) : std::nan("");
+    }
     // 3.4 update_convergence_signals (:512-632)
     if (cfg.convergence_model == GC_CONV_EMA) {
       // update_convergence_ema, :1192-1204
-      const double e_new = cfg.ema_decay * ema + (1 - cfg.ema_decay) * loss;
+      const double e_new = cfg.ema_decay * ema + (1 - cfg.ema_decay) * loss_c;
       conv = std::fabs(e_new - ema) / (std::fabs(ema) + 1e-8);
       ema = e_new;
     } else {
       // update_convergence, :1185-1190: minus the least-squares slope
       std::rotate(cbuf.begin(), cbuf.begin() + 1, cbuf.end());
-      cbuf.back() = loss;
+      cbuf.back() = loss_c;
       const int n = int(cbuf.size());
       double sx = 0, sy = 0, sxx = 0, sxy = 0;
       for (int i = 0; i < n; ++i) {
@@ -352,8 +430,9 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
       const double slope = (den != 0) ? (n * sxy - sx * sy) / den : 0.0;
       conv = -slope;
     }
-    if (conv > cfg.convergence_value || epoch <= cfg.convergence_window) pc = 0;
-    else if ((min_loss - loss) > cfg.convergence_value) pc = 0;
+    if (reprime > 0) { --reprime; pc = 0; }
+    else if (conv > cfg.convergence_value || epoch <= cfg.convergence_window) pc = 0;
+    else if (!robust && (min_loss - loss) > cfg.convergence_value) pc = 0;
     else pc += 1;
     if (cfg.step_tol > 0) {
       double num = 0, den = 0;
@@ -755,7 +834,6 @@ int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const
                 const float* extra, const int64_t* vox, const int32_t* dims, const float* z0, double* zfinal,
                 double* zmin, double* res_final, double* res_min, double* trace, double* scal_out, int32_t precision,
                 int32_t nthreads) {
-  if (cfg->robust) return GC_ERR_UNSUPPORTED;
   if (md->n_meas > 1024 || md->n_par > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
   orc::AskOut o{};
   int rc;

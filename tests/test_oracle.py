@@ -499,3 +499,64 @@ def test_mcrmwi_complex_phase_and_sum():
     ex2 = ex.copy()
     ex2[2] *= 2
     np.testing.assert_allclose(orc.fwd_value(mc, thc, ex2)[:, 0], 2 * sc, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# robustConvergence (utils/askadam.m:448-510, 541-566)
+def _robust_problem(nv=400, n_out=40, seed=9):
+    rng = np.random.default_rng(seed)
+    md = _r2s_md()
+    th = np.stack([rng.uniform(0.5, 1.5, nv), rng.uniform(10, 60, nv)])
+    y = orc.fwd_value(md, th) + 0.005 * rng.standard_normal((12, nv))
+    # outlier voxels: heavy noise around the starting point's own signal, so
+    # the fit cannot improve them while the other voxels improve a lot
+    start = orc.fwd_value(md, np.array([[1.0], [100.05]]))
+    y[:, :n_out] = start + 5.0 * rng.standard_normal((12, n_out))
+    return md, y.astype(np.float32), np.zeros((2, nv), np.float32)
+
+
+def _robust_cfg(**kw):
+    base = dict(modelParams=["M0", "R2star"], lb=[0, 0.1], ub=[2, 200], iteration=120, initialLearnRate=0.02,
+                tol=0, convergenceValue=-1, convergenceStepTol=0, convergenceGradTol=0, robustConvergence=True)
+    base.update(kw)
+    f = askadam.check_set_default_basic(base)
+    return askadam().make_cfg(f, None)
+
+
+def test_robust_without_flags_equals_plain_loop():
+    """Thresholds no voxel can fail and no convergence stop: the robust loop
+    takes exactly the plain loop's path."""
+    md, y, z0 = _robust_problem()
+    plain = orc.askadam(md, _robust_cfg(robustConvergence=False), y, z0, trace=True)
+    rob = orc.askadam(md, _robust_cfg(outlierVoxelThres=-1e9, outlierInitThres=-1e9), y, z0, trace=True)
+    np.testing.assert_array_equal(plain["trace"], rob["trace"])
+    np.testing.assert_array_equal(plain["z_final"], rob["z_final"])
+
+
+def test_robust_modulates_only_the_epoch_after_a_check():
+    """update_outlier_mask returns the modulated weights on check epochs only:
+    with every voxel flagged and outlierWeight 0, the (weighted) loss is 0
+    exactly on epochs k*weightUpdateInterval + 1 and positive elsewhere."""
+    md, y, z0 = _robust_problem()
+    cfg = _robust_cfg(outlierVoxelThres=1e9, outlierPopThres=-1e9, outlierInitThres=1e9, outlierInitPopThres=-1e9,
+                      outlierWeight=0.0, weightUpdateInterval=5, iteration=40)
+    o = orc.askadam(md, cfg, y, z0, trace=True)
+    tr = o["trace"]
+    zero = [e for e in range(1, 41) if tr[e] == 0]
+    assert zero == [6, 11, 16, 21, 26, 31, 36]
+    assert np.all(tr[[e for e in range(41) if e not in zero]] > 0)
+
+
+def test_robust_flags_the_unfittable_voxels():
+    """The unfittable voxels stagnate while the median improves: they fail both
+    criteria and are down-weighted, so the robust fit's final loss (weighted
+    on check-following epochs only) matches the plain fit on the good voxels."""
+    md, y, z0 = _robust_problem()
+    plain = orc.askadam(md, _robust_cfg(robustConvergence=False, iteration=200), y, z0, trace=True)
+    rob = orc.askadam(md, _robust_cfg(iteration=200, outlierWeight=0.0), y, z0, trace=True)
+    # once the median improvement passes outlierInitPopThres (the check at
+    # epoch 20), the epochs following each check see the outliers at weight 0
+    r = rob["trace"] / plain["trace"]
+    drops = [e for e in range(21, 200) if (e - 1) % 5 == 0]
+    assert np.all(r[drops] < 0.5), r[drops].max()
+    np.testing.assert_array_equal(r[:21], 1.0)

@@ -210,6 +210,52 @@ int gc_tv_fwd_adj(gc_ctx* ctx, const float* img, const uint8_t* mask, const int3
   return GC_OK;
 }
 
+int gc_posterior_stats(gc_ctx* ctx, const float* post, int32_t Q, int32_t n, int64_t nv, const float* lb,
+                       const float* ub, int32_t nbin, float* median, float* iqr, float* mode) {
+  if (!ctx) return GC_ERR_ARG;
+  if (!post || Q < 1 || Q > GC_MAX_PAR || n < 1 || n > 16384 || nv < 1)
+    return set_err(ctx, GC_ERR_ARG, "gc_posterior_stats: bad argument");
+  if (mode && (!lb || !ub || nbin < 2)) return set_err(ctx, GC_ERR_ARG, "gc_posterior_stats: mode needs lb, ub, nbin");
+  int n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  const int vpb = std::max(1, std::min(16, 8192 / n2));
+  int max_lds = 0;
+  GC_HIP(ctx, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  const size_t lds = size_t(vpb) * n2 * sizeof(float) + 2 * kBlock * sizeof(int);
+  if (lds > size_t(max_lds))
+    return set_err(ctx, GC_ERR_UNSUPPORTED, "gc_posterior_stats: %d samples per voxel need %zu B of LDS (max %d)", n, lds,
+                   max_lds);
+  DevBuf edges;
+  std::vector<double> he;
+  if (mode) {
+    // linspace(lb - 1e-8, ub + 1e-8, nbin) in double (MATLAB single lb/ub promoted)
+    he.resize(size_t(Q) * nbin);
+    for (int q = 0; q < Q; ++q) {
+      const double lo = double(lb[q]) - 1e-8, hi = double(ub[q]) + 1e-8;
+      for (int i = 0; i < nbin; ++i)
+        he[size_t(q) * nbin + i] = (i == nbin - 1) ? hi : lo + (hi - lo) * double(i) / double(nbin - 1);
+    }
+    GC_HIP(ctx, edges.alloc(sizeof(double) * he.size()));
+    GC_HIP(ctx, hipMemcpyAsync(edges.p, he.data(), sizeof(double) * he.size(), hipMemcpyHostToDevice, ctx->stream));
+  }
+  PostStatParams sp{};
+  sp.post = post;
+  sp.edges = edges.as<double>();
+  sp.median = median;
+  sp.iqr = iqr;
+  sp.mode = mode;
+  sp.nv = nv;
+  sp.n = n;
+  sp.n2 = n2;
+  sp.vpb = vpb;
+  sp.nbin = nbin;
+  const unsigned gx = unsigned((nv + vpb - 1) / vpb);
+  hipLaunchKernelGGL(k_posterior_stats, dim3(gx, Q), dim3(kBlock), size_t(vpb) * n2 * sizeof(float), ctx->stream, sp);
+  GC_HIP(ctx, hipGetLastError());
+  GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // edges freed on return
+  return GC_OK;
+}
+
 int gc_askadam_run(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int32_t n_vol) {
   if (!ctx) return GC_ERR_ARG;
   int rc = check_model(ctx, md);

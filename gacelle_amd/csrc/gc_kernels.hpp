@@ -724,6 +724,113 @@ __global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ md
   }
 }
 
+// ---- posterior statistics of res2out (utils/mcmc.m:823-857) ---------------
+// One workgroup sorts the n samples of `vpb` consecutive voxels of one
+// variable in LDS (bitonic network over n2 = next power of two >= n, padded
+// with +inf; the voxels' samples are read as vpb-wide segments), then takes
+// MATLAB's prctile order statistics (median = prctile 50; iqr = prctile 75
+// - prctile 25; sample k sits at 100 (k - 0.5) / n) and the mode of
+// discretize(x, edges) (bins [e_i, e_i+1), the last closed; the smallest bin
+// among the most frequent) from binary searches of the sorted samples.
+struct PostStatParams {
+  const float* post;     // [Q][n][nv]
+  const double* edges;   // [Q][nbin]
+  float* median;         // [Q][nv] or null
+  float* iqr;
+  float* mode;
+  int64_t nv;
+  int32_t n, n2, vpb, nbin;
+};
+
+GC_DEV double prctile_sorted(const float* s, int n, double p) {
+  const double k = p / 100.0 * n + 0.5;  // 1-based position
+  if (k <= 1.0) return double(s[0]);
+  if (k >= double(n)) return double(s[n - 1]);
+  const int i = int(floor(k));
+  const double f = k - double(i);
+  return double(s[i - 1]) + f * (double(s[i]) - double(s[i - 1]));
+}
+
+// first index in s[0, n) with s[i] >= x (or > x when upper)
+GC_DEV int bsearch_sorted(const float* s, int n, double x, bool upper) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const double v = double(s[mid]);
+    if (upper ? (v <= x) : (v < x)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_posterior_stats(const PostStatParams sp) {
+  extern __shared__ float buf[];  // [vpb][n2]
+  const int q = blockIdx.y;
+  const int64_t v0 = int64_t(blockIdx.x) * sp.vpb;
+  const int n = sp.n, n2 = sp.n2, vpb = sp.vpb;
+  const int total = vpb * n2;
+  for (int t = threadIdx.x; t < total; t += kBlock) {
+    const int j = t % vpb, i = t / vpb;  // voxel fastest: coalesced reads
+    const int64_t v = v0 + j;
+    float x = INFINITY;
+    if (i < n && v < sp.nv) x = sp.post[(int64_t(q) * n + i) * sp.nv + v];
+    buf[j * n2 + i] = x;
+  }
+  __syncthreads();
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int t = threadIdx.x; t < total; t += kBlock) {
+        const int vj = t / n2, i = t % n2;
+        const int l = i ^ jj;
+        if (l > i) {
+          float* a = buf + vj * n2;
+          const float x = a[i], y = a[l];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) { a[i] = y; a[l] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // order statistics: one thread per voxel
+  if (threadIdx.x < vpb) {
+    const int64_t v = v0 + threadIdx.x;
+    if (v < sp.nv) {
+      const float* a = buf + threadIdx.x * n2;
+      if (sp.median) sp.median[int64_t(q) * sp.nv + v] = float(prctile_sorted(a, n, 50.0));
+      if (sp.iqr) sp.iqr[int64_t(q) * sp.nv + v] = float(prctile_sorted(a, n, 75.0) - prctile_sorted(a, n, 25.0));
+    }
+  }
+  if (sp.mode == nullptr) return;
+  // mode: threads of a voxel split its bins; best (count, first index)
+  const int tpv = kBlock / vpb;
+  const int vj = threadIdx.x / tpv, lane_v = threadIdx.x % tpv;
+  const double* e = sp.edges + int64_t(q) * sp.nbin;
+  const int nb = sp.nbin - 1;
+  int best = -1, bestc = -1;
+  if (vj < vpb) {
+    const float* a = buf + vj * n2;
+    for (int b = lane_v; b < nb; b += tpv) {
+      const int lo = bsearch_sorted(a, n, e[b], false);
+      const int hi = (b == nb - 1) ? bsearch_sorted(a, n, e[b + 1], true) : bsearch_sorted(a, n, e[b + 1], false);
+      const int c = hi - lo;
+      if (c > bestc) { bestc = c; best = b; }
+    }
+  }
+  __shared__ int rc[kBlock], rb[kBlock];
+  rc[threadIdx.x] = bestc;
+  rb[threadIdx.x] = best;
+  __syncthreads();
+  if (vj < vpb && lane_v == 0) {
+    for (int t = 1; t < tpv; ++t) {
+      const int c = rc[threadIdx.x + t], b = rb[threadIdx.x + t];
+      if (c > bestc || (c == bestc && b < best)) { bestc = c; best = b; }
+    }
+    const int64_t v = v0 + vj;
+    if (v < sp.nv) sp.mode[int64_t(q) * sp.nv + v] = (bestc > 0) ? float(0.5 * (e[best] + e[best + 1])) : NAN;
+  }
+}
+
 // ---- MCMC -------------------------------------------------------------------
 struct McmcParams {
   const float* y;

@@ -187,6 +187,19 @@ class mcmc:
         }
         if trace_d is not None:
             res["trace"] = runtime.to_host(trace_d)
+        wanted = [m for m in ("median", "iqr", "mode") if m in [x.lower() for x in fitting["metric"]]]
+        if wanted:
+            if post_d is None:
+                raise GacelleError(_abi.GC_ERR_ARG, f"metric {wanted} needs fitting.storePosterior = True")
+            # res2out median / iqr / mode on the device (utils/mcmc.m:823-857)
+            outs = {m: ctx.empty((Q, nv)) for m in wanted}
+            lb = (ctypes.c_float * Q)(*[float(x) for x in np.atleast_1d(fitting["lb"])[:Q]])
+            ub = (ctypes.c_float * Q)(*[float(x) for x in np.atleast_1d(fitting["ub"])[:Q]])
+            ctx.check(ctx.lib.gc_posterior_stats(ctx.handle, runtime.ptr(post_d), Q, R * ns * W, nv, lb, ub,
+                                                 int(fitting.get("Nbin", 1001)), runtime.ptr(outs.get("median")),
+                                                 runtime.ptr(outs.get("iqr")), runtime.ptr(outs.get("mode"))))
+            for m, t in outs.items():
+                res[m] = runtime.to_host(t)
         posterior = {}
         if post_d is not None:
             p = runtime.to_host(post_d)  # [Q, R, Ns, W, nv]
@@ -198,52 +211,17 @@ class mcmc:
 
     # ------------------------------------------------------------------
     def res2out(self, res, fitting, mask) -> dict:
-        """utils/mcmc.m:806-862.  mean/std come from the device reduction;
-        median/iqr/mode are computed from the stored posterior."""
+        """utils/mcmc.m:806-862.  Every metric comes from the device: mean /
+        std from the sampler's reduction, median / iqr / mode from
+        gc_posterior_stats over the stored posterior."""
         out = {"posterior": res["posterior"]}
         names = res["names"]
         for metric in fitting["metric"]:
             m = metric.lower()
+            if m not in ("mean", "std", "median", "iqr", "mode"):
+                raise GacelleError(_abi.GC_ERR_ARG, f"unknown metric {metric}")
             out[m] = {}
             for q, n in enumerate(names):
-                if m == "mean":
-                    v = res["mean"][q]
-                elif m == "std":
-                    v = res["std"][q]
-                else:
-                    if n not in res["posterior"]:
-                        raise GacelleError(_abi.GC_ERR_ARG, f"metric '{m}' needs fitting.storePosterior = True")
-                    a = res["posterior"][n].reshape(res["posterior"][n].shape[0], -1, order="F")
-                    if m == "median":
-                        v = np.median(a, axis=1)
-                    elif m == "iqr":
-                        v = _matlab_iqr(a)
-                    elif m == "mode":
-                        k = list(fitting["modelParams"]).index(n)
-                        v = _matlab_mode_bins(a, fitting["lb"][k], fitting["ub"][k], int(fitting["Nbin"]))
-                    else:
-                        raise GacelleError(_abi.GC_ERR_ARG, f"unknown metric {metric}")
-                out[m][n] = utils.reshape_ND2image(np.asarray(v, np.float32), mask)
+                out[m][n] = utils.reshape_ND2image(np.asarray(res[m][q], np.float32), mask)
         out["acceptance"] = utils.reshape_ND2image(res["accept"], mask)
         return out
-
-
-def _matlab_iqr(a):
-    """iqr along rows with MATLAB's prctile convention."""
-    out = np.empty(a.shape[0])
-    for i in range(a.shape[0]):
-        out[i] = utils.prctile(a[i], 75) - utils.prctile(a[i], 25)
-    return out
-
-
-def _matlab_mode_bins(a, lb, ub, nbin):
-    """mode(discretize(x, edges)) with edges = linspace(lb-1e-8, ub+1e-8, Nbin)
-    and the bin centre returned (utils/mcmc.m:845-857)."""
-    edges = np.linspace(lb - 1e-8, ub + 1e-8, nbin)
-    idx = np.clip(np.searchsorted(edges, a, side="right") - 1, 0, nbin - 2)
-    out = np.empty(a.shape[0])
-    for i in range(a.shape[0]):
-        counts = np.bincount(idx[i], minlength=nbin - 1)
-        b = int(np.argmax(counts))  # smallest bin on ties, as MATLAB mode
-        out[i] = (edges[b] + edges[b + 1]) / 2
-    return out

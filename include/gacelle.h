@@ -326,6 +326,16 @@ int gc_tv_fwd_adj(gc_ctx* ctx, const float* img_dev, const uint8_t* mask_dev,
                   const int32_t dims[3], int32_t tv_mode, const double voxel_size[3],
                   double lambda, int64_t n_masked, double* loss, float* grad_img_dev);
 
+/* ---- posterior statistics ------------------------------------------------ */
+/* res2out 'median', 'iqr' and 'mode' (utils/mcmc.m:823-857) over the n
+ * samples of each voxel and variable of a posterior [Q][n][nv] (voxel
+ * fastest: gc_mcmc_io.posterior_dev with n = rep * Ns * W).  lb / ub are
+ * HOST arrays [Q]; mode bins are discretize(x, linspace(lb-1e-8, ub+1e-8,
+ * nbin)).  Outputs [Q][nv], each optional (NULL skips it).  n <= 16384. */
+int gc_posterior_stats(gc_ctx* ctx, const float* posterior_dev, int32_t n_var, int32_t n, int64_t nv,
+                       const float* lb, const float* ub, int32_t nbin, float* median_dev, float* iqr_dev,
+                       float* mode_dev);
+
 /* ---- solvers ------------------------------------------------------------ */
 /* askadam().optimisation (utils/askadam.m:121-365) for one model over a
  * batch of independent volumes, each with its own N, stopping state and

@@ -107,3 +107,26 @@ def replay_loss_stopping(trace, f):
         if loss < f["tol"]:
             return e, _abi.GC_STOP_TOL
     return len(trace) - 1, _abi.GC_STOP_NONE
+
+
+def matlab_iqr(a):
+    """iqr along rows with MATLAB's prctile convention (host reference)."""
+    from gacelle_amd import utils
+
+    out = np.empty(a.shape[0])
+    for i in range(a.shape[0]):
+        out[i] = utils.prctile(a[i], 75) - utils.prctile(a[i], 25)
+    return out
+
+
+def matlab_mode_bins(a, lb, ub, nbin):
+    """mode(discretize(x, edges)) with edges = linspace(lb-1e-8, ub+1e-8, Nbin)
+    and the bin centre returned (utils/mcmc.m:845-857), host reference."""
+    edges = np.linspace(lb - 1e-8, ub + 1e-8, nbin)
+    idx = np.clip(np.searchsorted(edges, a, side="right") - 1, 0, nbin - 2)
+    out = np.empty(a.shape[0])
+    for i in range(a.shape[0]):
+        counts = np.bincount(idx[i], minlength=nbin - 1)
+        b = int(np.argmax(counts))  # smallest bin on ties, as MATLAB mode
+        out[i] = (edges[b] + edges[b + 1]) / 2
+    return out

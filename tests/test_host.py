@@ -5,7 +5,9 @@ import pytest
 
 from gacelle_amd import _abi, models, utils
 from gacelle_amd.askadam import askadam
-from gacelle_amd.mcmc import _matlab_iqr, _matlab_mode_bins, mcmc
+from gacelle_amd.mcmc import mcmc
+
+from .helpers import matlab_iqr as _matlab_iqr, matlab_mode_bins as _matlab_mode_bins
 
 
 def test_find_mask_is_column_major():

@@ -8,9 +8,9 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint64, c_void_p
+from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 2
+GC_ABI_VERSION = 3
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
@@ -170,6 +170,8 @@ class gc_mcmc_cfg(ctypes.Structure):
         ("step_size", c_double),
         ("start_range", c_double),
         ("x_step", c_float * GC_MAX_PAR),
+        ("global_mask", c_uint32),
+        ("global_warmup", c_int32),
     ]
 
 
@@ -183,12 +185,17 @@ class gc_mcmc_io(ctypes.Structure):
         ("normals_dev", c_void_p),
         ("uniforms_dev", c_void_p),
         ("partners_dev", c_void_p),
+        ("g_uniforms_dev", c_void_p),
+        ("g_partners_dev", c_void_p),
         ("posterior_dev", c_void_p),
         ("mean_dev", c_void_p),
         ("std_dev", c_void_p),
         ("accept_dev", c_void_p),
         ("xfinal_dev", c_void_p),
         ("trace_dev", c_void_p),
+        ("g_posterior_dev", c_void_p),
+        ("g_xfinal_dev", c_void_p),
+        ("g_trace_dev", c_void_p),
         ("n_sample", c_int32),
         ("n_stored", c_int32),
         ("kernel_ms", c_double),

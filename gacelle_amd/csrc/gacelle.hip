@@ -290,8 +290,10 @@ static void host_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint
   o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
 }
 
-static void host_partners(uint64_t seed, int rep, int it, int W, int mode, int32_t* out) {
-  const uint64_t stream = (uint64_t(0xFFFFu) << 48) | (uint64_t(rep) << 32);
+// `tag` selects the stream: 0xFFFF for the voxelwise blocks, 0xFFFC for the
+// global-parameter block of goodman_weare_wglobal_constant.
+static void host_partners(uint64_t seed, int rep, int it, int W, int mode, int32_t* out, uint32_t tag = 0xFFFFu) {
+  const uint64_t stream = (uint64_t(tag) << 48) | (uint64_t(rep) << 32);
   auto draw = [&](uint64_t chain, uint32_t purpose, uint32_t o[4]) {
     host_philox(uint32_t(it), uint32_t(chain), uint32_t(chain >> 32), purpose, uint32_t(seed), uint32_t(seed >> 32), o);
   };
@@ -326,6 +328,200 @@ static void host_partners(uint64_t seed, int rep, int it, int W, int mode, int32
   }
 }
 
+// Partner table [rep][steps][W]: the native stream, or the replayed one
+// after a range check (red-black entries index the other half).
+static int partner_table(gc_ctx* ctx, const gc_mcmc_cfg* cfg, int W, int steps, uint32_t tag,
+                         const int32_t* replay_dev, DevBuf& buf, const int32_t** out) {
+  const size_t n = size_t(cfg->repetition) * std::max(steps, 1) * W;
+  std::vector<int32_t> tab(n, 0);
+  if (cfg->rng_mode == GC_RNG_NATIVE) {
+    for (int r = 0; r < cfg->repetition; ++r)
+      for (int k = 1; k <= steps; ++k)
+        host_partners(cfg->seed, r, k, W, cfg->ensemble_update, &tab[(size_t(r) * steps + (k - 1)) * W], tag);
+    GC_HIP(ctx, buf.alloc(n * sizeof(int32_t)));
+    GC_HIP(ctx, hipMemcpy(buf.p, tab.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    *out = buf.as<int32_t>();
+    return GC_OK;
+  }
+  if (steps > 0) GC_HIP(ctx, hipMemcpy(tab.data(), replay_dev, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int h = W / 2;
+  for (size_t i = 0; i < n && steps > 0; ++i) {
+    const int w = int(i % W);
+    int lo = 0, hi = W;
+    if (cfg->ensemble_update == GC_ENSEMBLE_REDBLACK) {
+      lo = (w < h) ? h : 0;
+      hi = (w < h) ? W : h;
+    }
+    if (tab[i] < lo || tab[i] >= hi)
+      return set_err(ctx, GC_ERR_ARG, "replayed partner index %d out of range at entry %zu", tab[i], i);
+  }
+  *out = replay_dev;
+  return GC_OK;
+}
+
+// Zero posterior slots [n_stored, ns) of a [nq][rep][ns][inner] buffer.
+static int zero_unfilled(gc_ctx* ctx, float* post, int nq, int rep, int ns, int n_stored, size_t inner) {
+  if (!post || n_stored >= ns) return GC_OK;
+  for (int q = 0; q < nq; ++q)
+    for (int r = 0; r < rep; ++r) {
+      float* p = post + ((size_t(q) * rep + r) * ns + n_stored) * inner;
+      GC_HIP(ctx, hipMemsetAsync(p, 0, size_t(ns - n_stored) * inner * sizeof(float), ctx->stream));
+    }
+  return GC_OK;
+}
+
+// goodman_weare_wglobal_constant (utils/mcmc.m:385-578): the launch
+// sequence of gc_gwglobal.hpp, one iteration at a time.
+static int mcmc_global_run(gc_ctx* ctx, const ModelOps* ops, const gc_model* md, const gc_mcmc_cfg* cfg,
+                           gc_mcmc_io* io, int ns, int n_stored) {
+  const int Q = cfg->n_var, W = cfg->n_walker, R = cfg->repetition;
+  const uint32_t all = (1u << Q) - 1u;
+  const uint32_t gm = cfg->global_mask;
+  const int NG = __builtin_popcount(gm & all), NND = Q - NG;
+  if ((gm & ~all) != 0)
+    return set_err(ctx, GC_ERR_ARG, "global_mask names variables beyond n_var (%d)", Q);
+  if (NND == 0 || NG == 0)
+    return set_err(ctx, GC_ERR_ARG,
+                   "Need at least one local and one global parameter (NND=%d, NGlobal=%d). Use goodman_weare for "
+                   "the all-local case.", NND, NG);
+  if (cfg->algorithm != GC_MCMC_GW)
+    return set_err(ctx, GC_ERR_UNSUPPORTED, "global parameters need the ensemble (Goodman-Weare) sampler");
+  if (cfg->global_warmup < 0) return set_err(ctx, GC_ERR_ARG, "globalWarmup must be >= 0");
+  if (cfg->rng_mode == GC_RNG_REPLAY && (!io->g_uniforms_dev || !io->g_partners_dev))
+    return set_err(ctx, GC_ERR_ARG, "replay mode needs the global-block draw buffers");
+  const int C = W * R;
+  const int64_t nv = io->nv;
+  const int T = cfg->global_warmup + cfg->iteration;
+  const int gpb = kBlock / C;
+  const int64_t ntile = (nv + gpb - 1) / gpb;
+  const int nblk = int(std::min<int64_t>(ntile, 2048));
+
+  DevBuf dlpart, dgpart, dxl, dlp, dlpg, dnacc, dg, dgprop, dgoob, dpend, dpartial, ds, dgs;
+  GwgParams P{};
+  int rc = partner_table(ctx, cfg, W, cfg->iteration, 0xFFFFu, io->partners_dev, dlpart, &P.partners);
+  if (rc) return rc;
+  rc = partner_table(ctx, cfg, W, T, 0xFFFCu, io->g_partners_dev, dgpart, &P.g_partners);
+  if (rc) return rc;
+  GC_HIP(ctx, dxl.alloc(size_t(NND) * nv * C * sizeof(float)));
+  GC_HIP(ctx, dlp.alloc(size_t(nv) * C * sizeof(float)));
+  GC_HIP(ctx, dlpg.alloc(size_t(nv) * C * sizeof(float)));
+  GC_HIP(ctx, dnacc.alloc(size_t(nv) * C * sizeof(uint32_t)));
+  GC_HIP(ctx, dg.alloc(size_t(C) * NG * sizeof(float)));
+  GC_HIP(ctx, dgprop.alloc(size_t(C) * NG * sizeof(float)));
+  GC_HIP(ctx, dgoob.alloc(size_t(C) * sizeof(int32_t)));
+  GC_HIP(ctx, dpend.alloc(size_t(C) * sizeof(int32_t)));
+  GC_HIP(ctx, dpartial.alloc(size_t(nblk) * C * sizeof(double)));
+  GC_HIP(ctx, ds.alloc(size_t(2) * NND * nv * sizeof(double)));
+  GC_HIP(ctx, dgs.alloc(size_t(2) * C * NG * sizeof(double)));
+  GC_HIP(ctx, hipMemsetAsync(dpend.p, 0, size_t(C) * sizeof(int32_t), ctx->stream));
+  GC_HIP(ctx, hipMemsetAsync(dgoob.p, 0, size_t(C) * sizeof(int32_t), ctx->stream));
+  GC_HIP(ctx, hipMemsetAsync(ds.p, 0, size_t(2) * NND * nv * sizeof(double), ctx->stream));
+  GC_HIP(ctx, hipMemsetAsync(dgs.p, 0, size_t(2) * C * NG * sizeof(double), ctx->stream));
+  P.y = io->y_dev;
+  P.w = io->w_dev;
+  P.ex = io->extra_dev;
+  P.x0 = io->x0_dev;
+  P.normals = io->normals_dev;
+  P.uniforms = io->uniforms_dev;
+  P.g_uniforms = io->g_uniforms_dev;
+  P.xl = dxl.as<float>();
+  P.lp = dlp.as<float>();
+  P.lpg = dlpg.as<float>();
+  P.nacc = dnacc.as<uint32_t>();
+  P.g = dg.as<float>();
+  P.gprop = dgprop.as<float>();
+  P.goob = dgoob.as<int32_t>();
+  P.pend = dpend.as<int32_t>();
+  P.part = dpartial.as<double>();
+  P.s1 = ds.as<double>();
+  P.s2 = P.s1 + size_t(NND) * nv;
+  P.gs1 = dgs.as<double>();
+  P.gs2 = P.gs1 + size_t(C) * NG;
+  P.posterior = io->posterior_dev;
+  P.g_posterior = io->g_posterior_dev;
+  P.mean = io->mean_dev;
+  P.stdv = io->std_dev;
+  P.accept = io->accept_dev;
+  P.xfinal = io->xfinal_dev;
+  P.g_xfinal = io->g_xfinal_dev;
+  P.trace = io->trace_dev;
+  P.g_trace = io->g_trace_dev;
+  P.nv = nv;
+  P.seed = cfg->seed;
+  P.iteration = cfg->iteration;
+  P.T = T;
+  P.rep = R;
+  P.n_walker = W;
+  P.nchain = C;
+  P.n_sample = ns;
+  P.n_stored = n_stored;
+  P.rng = cfg->rng_mode;
+  P.ens = cfg->ensemble_update;
+  P.nnd = NND;
+  P.ng = NG;
+  P.nblk = nblk;
+  P.gmask = gm;
+  P.step_size = float(cfg->step_size);
+  P.start_range = float(cfg->start_range);
+  const gc_model* dmd = nullptr;
+  rc = upload_model(ctx, md, &dmd);
+  if (rc) return rc;
+  const bool hw = io->w_dev != nullptr;
+  const bool rb = cfg->ensemble_update == GC_ENSEMBLE_REDBLACK;
+  const unsigned grid = unsigned(nblk);
+  const unsigned dgrid = unsigned((C + kBlock / 64 - 1) / (kBlock / 64));
+  hipEvent_t ev0, ev1;
+  GC_HIP(ctx, hipEventCreate(&ev0));
+  GC_HIP(ctx, hipEventCreate(&ev1));
+  auto run = [&]() -> int {
+    GwgStep S{0, 0, 0, -1, -1};
+    int r = ops->gwg(ctx, dmd, 0, hw, P, S, grid);
+    if (r) return r;
+    GC_HIP(ctx, hipEventRecord(ev0, ctx->stream));
+    // one global block: red-black runs its halves one after the other
+    auto global_block = [&](int k, int t, bool with_local, int keep) -> int {
+      const int nsub = rb ? 2 : 1;
+      for (int sub = 0; sub < nsub; ++sub) {
+        const GwgStep st{k, t, (with_local && sub == 0) ? 1 : 0, sub, (sub == 0) ? keep : -1};
+        int e = ops->gwg(ctx, dmd, 1, hw, P, st, grid);
+        if (e) return e;
+        const GwgStep sd{k, t, 0, sub, (sub == nsub - 1) ? keep : -1};
+        hipLaunchKernelGGL(k_gwg_decide, dim3(dgrid), dim3(kBlock), 0, ctx->stream, dmd, P, sd);
+        GC_HIP(ctx, hipGetLastError());
+      }
+      return GC_OK;
+    };
+    // global-only warm-up with the locals frozen (:503-507)
+    for (int t = 1; t <= cfg->global_warmup; ++t) {
+      r = global_block(0, t, false, -1);
+      if (r) return r;
+    }
+    int counter = 0;
+    for (int k = 1; k <= cfg->iteration; ++k) {
+      int keep = -1;
+      if (k > cfg->n_burnin && ((k - cfg->n_burnin + 1) % cfg->thinning) == 0) {  // :554
+        keep = (counter < ns) ? counter : -1;
+        ++counter;
+      }
+      r = global_block(k, cfg->global_warmup + k, true, keep);
+      if (r) return r;
+    }
+    r = ops->gwg(ctx, dmd, 2, hw, P, S, grid);
+    if (r) return r;
+    GC_HIP(ctx, hipEventRecord(ev1, ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    io->kernel_ms = ms;
+    return GC_OK;
+  };
+  rc = run();
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  return rc;
+}
+
 int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc_io* io) {
   if (!ctx) return GC_ERR_ARG;
   int rc = check_model(ctx, md);
@@ -347,6 +543,9 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
   if (gw && W < 2) return set_err(ctx, GC_ERR_NWALKER, "Goodman-Weare needs Nwalker >= 2");
   const int gsz = W * cfg->repetition;
   if (gsz > kBlock) return set_err(ctx, GC_ERR_UNSUPPORTED, "Nwalker*repetition must be <= %d", kBlock);
+  if (cfg->global_mask != 0) {
+    if (!gw) return set_err(ctx, GC_ERR_UNSUPPORTED, "global parameters need the ensemble (Goodman-Weare) sampler");
+  }
   if (cfg->rng_mode == GC_RNG_REPLAY && (!io->normals_dev || !io->uniforms_dev || (gw && !io->partners_dev)))
     return set_err(ctx, GC_ERR_ARG, "replay mode needs the draw buffers");
   const int ns = (cfg->iteration < cfg->n_burnin + 1) ? 0 : (cfg->iteration - cfg->n_burnin - 1) / cfg->thinning + 1;
@@ -357,6 +556,18 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
       if (((k - cfg->n_burnin + 1) % cfg->thinning) == 0) ++cnt;
     io->n_stored = std::min(cnt, ns);
   }
+  {
+    // Posterior slots that thinning leaves unfilled hold zeros, as in the
+    // reference's preallocated posterior (utils/mcmc.m:121, :245, :481-482).
+    const uint32_t all = (1u << cfg->n_var) - 1u;
+    const int ng = cfg->global_mask ? __builtin_popcount(cfg->global_mask & all) : 0;
+    rc = zero_unfilled(ctx, io->posterior_dev, cfg->n_var - ng, cfg->repetition, ns, io->n_stored,
+                       size_t(W) * size_t(io->nv));
+    if (rc) return rc;
+    if (ng) rc = zero_unfilled(ctx, io->g_posterior_dev, ng, cfg->repetition, ns, io->n_stored, size_t(W));
+    if (rc) return rc;
+  }
+  if (cfg->global_mask != 0) return mcmc_global_run(ctx, ops, md, cfg, io, ns, io->n_stored);
   McmcParams P{};
   P.y = io->y_dev;
   P.w = io->w_dev;

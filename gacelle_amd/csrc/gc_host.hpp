@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "../../include/gacelle.h"
-#include "gc_kernels.hpp"
+#include "gc_gwglobal.hpp"
 
 struct gc_ctx {
   int device = 0;
@@ -171,6 +171,10 @@ struct ModelOps {
   // one MH or GW launch (kernel-only timing through ev0/ev1)
   int (*mcmc)(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
               hipEvent_t ev0, hipEvent_t ev1);
+  // one launch of the GW sampler with global parameters: kind 0 = initial
+  // ensemble, 1 = iteration step (S), 2 = final statistics
+  int (*gwg)(gc_ctx* ctx, const gc_model* dmd, int kind, bool hw, const GwgParams& P, const GwgStep& S,
+             unsigned grid);
 };
 
 // one getter per model variant, each defined in its own ops_*.hip

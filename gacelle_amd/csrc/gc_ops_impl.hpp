@@ -84,12 +84,31 @@ struct OpsImpl {
       return GC_OK;
     });
   }
+
+  static int gwg(gc_ctx* ctx, const gc_model* dmd, int kind, bool hw, const GwgParams& P, const GwgStep& S,
+                 unsigned grid) {
+    if constexpr (Model::kStreamOnly) {
+      return set_err(ctx, GC_ERR_UNSUPPORTED, "MCMC is not available for this model");
+    } else {
+      return with_bool(hw, [&](auto hwc) -> int {
+        constexpr bool HW = decltype(hwc)::value;
+        if (kind == 0)
+          hipLaunchKernelGGL((k_gwg_init<Model, HW>), dim3(grid), dim3(kBlock), 0, ctx->stream, dmd, P);
+        else if (kind == 1)
+          hipLaunchKernelGGL((k_gwg_step<Model, HW>), dim3(grid), dim3(kBlock), 0, ctx->stream, dmd, P, S);
+        else
+          hipLaunchKernelGGL((k_gwg_final<Model>), dim3(grid), dim3(kBlock), 0, ctx->stream, dmd, P);
+        GC_HIP(ctx, hipGetLastError());
+        return GC_OK;
+      });
+    }
+  }
 };
 
 template <class Model>
 const ModelOps* ops_table() {
   static const ModelOps t{&OpsImpl<Model>::fwd, &OpsImpl<Model>::fwd_adj, &OpsImpl<Model>::askadam,
-                          &OpsImpl<Model>::mcmc};
+                          &OpsImpl<Model>::mcmc, &OpsImpl<Model>::gwg};
   return &t;
 }
 

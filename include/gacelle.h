@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 2
+#define GC_ABI_VERSION 3
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
@@ -233,6 +233,12 @@ typedef struct gc_mcmc_cfg {
   double step_size;        /* fitting.StepSize (GW stretch scale a)     */
   double start_range;      /* fitting.startRange (GW walker spread)     */
   float x_step[GC_MAX_PAR];/* fitting.xStepSize (MH proposal sd)        */
+  /* Goodman-Weare with global (shared) parameters,
+   * goodman_weare_wglobal_constant (utils/mcmc.m:385-578): bit q set =
+   * variable q is one value per walker shared by all voxels (a field of
+   * pars0 with one element, utils/mcmc.m:54-64, :760-770).  0 = none. */
+  uint32_t global_mask;
+  int32_t global_warmup;   /* fitting.globalWarmup: global-only steps   */
 } gc_mcmc_cfg;
 
 typedef struct gc_mcmc_io {
@@ -253,6 +259,18 @@ typedef struct gc_mcmc_io {
   const float* normals_dev;
   const float* uniforms_dev;
   const int32_t* partners_dev;
+  /* With global parameters (cfg.global_mask != 0) the local block keeps
+   * the GW layouts above over the NND local variables (normals
+   * [W][nv][NND]; uniforms and partners per local block, :507-545), and
+   * the global block (gw_global_block, :581-626) draws from
+   *  g_uniforms [W][NG]             (rand(size(ub_global)), :489)
+   *             then [rep][T][2][W]  (zz_g then acceptance, :587 and :597;
+   *             red-black: [rep][T][2 halves][2][hm])
+   *  g_partners [rep][T][W]          (find_partner; red-black anchors)
+   * where T = global_warmup + iteration, warm-up steps first, and
+   * hm = W - W/2.                                                      */
+  const float* g_uniforms_dev;
+  const int32_t* g_partners_dev;
   /* outputs */
   float* posterior_dev;      /* [Q][rep][Ns][W][nv] (MH: W = 1) or NULL  */
   float* mean_dev;           /* [Q x nv] posterior mean (res2out 'mean')  */
@@ -260,6 +278,14 @@ typedef struct gc_mcmc_io {
   float* accept_dev;         /* [nv] acceptance fraction, or NULL          */
   float* xfinal_dev;         /* [rep][W][Q][nv] final chain state, or NULL */
   uint8_t* trace_dev;        /* test hook: [rep][iter][W][nv] accept flags */
+  /* global-parameter outputs (cfg.global_mask != 0).  posterior_dev and
+   * xfinal_dev then hold the NND local variables only, in model order
+   * ([NND][rep][Ns][W][nv], [rep][W][NND][nv]); mean_dev / std_dev rows
+   * of global variables hold the scalar posterior statistic at every
+   * voxel.                                                             */
+  float* g_posterior_dev;    /* [NG][rep][Ns][W] or NULL                  */
+  float* g_xfinal_dev;       /* [rep][W][NG] or NULL                      */
+  uint8_t* g_trace_dev;      /* test hook: [rep][T][W] global accept flags */
   int32_t n_sample;          /* out: Ns = numel(Nburnin+1:thinning:iteration) */
   int32_t n_stored;          /* out: samples actually stored per chain    */
   double kernel_ms;          /* out: sampler kernel duration (HIP events) */
@@ -343,9 +369,11 @@ int gc_posterior_stats(gc_ctx* ctx, const float* posterior_dev, int32_t n_var, i
 int gc_askadam_run(gc_ctx* ctx, const gc_model* model, const gc_askadam_cfg* cfg,
                    gc_volume* vols, int32_t n_vol);
 
-/* mcmc().metropolis_hastings (utils/mcmc.m:82-190) and
- * mcmc().goodman_weare (utils/mcmc.m:192-383), with res2out mean/std
- * (utils/mcmc.m:806-862) computed on the device. */
+/* mcmc().metropolis_hastings (utils/mcmc.m:82-190),
+ * mcmc().goodman_weare (utils/mcmc.m:192-383) and, when cfg.global_mask
+ * is set, mcmc().goodman_weare_wglobal_constant with gw_global_block
+ * (utils/mcmc.m:385-626), with res2out mean/std (utils/mcmc.m:806-862)
+ * computed on the device. */
 int gc_mcmc_run(gc_ctx* ctx, const gc_model* model, const gc_mcmc_cfg* cfg,
                 gc_mcmc_io* io);
 

@@ -85,8 +85,15 @@ static inline double native_normal(uint64_t seed, uint32_t it, uint64_t chain, u
 // rejected until no fixed point (the randperm-until-derangement loop).
 // Red-black: for each half, na anchors drawn uniformly with replacement
 // from the other half.
-static void native_partners(uint64_t seed, int rep, int it, int W, int mode, int32_t* out) {
-  const uint64_t stream = (uint64_t(0xFFFFu) << 48) | (uint64_t(rep) << 32);
+// `tag` selects the stream: kTagLocal for the voxelwise blocks, kTagGlobal
+// for the global-parameter block of goodman_weare_wglobal_constant.
+enum : uint32_t { kTagLocal = 0xFFFFu, kTagGlobal = 0xFFFCu };
+// chain ids of the per-walker global draws (shared by all voxels)
+static inline uint64_t global_chain(int rep, int wk) { return (uint64_t(0xFFFEu) << 48) | (uint64_t(rep) << 32) | uint64_t(wk); }
+static inline uint64_t global_init_chain(int wk) { return (uint64_t(0xFFFDu) << 48) | uint64_t(wk); }
+
+static void native_partners(uint64_t seed, int rep, int it, int W, int mode, int32_t* out, uint32_t tag = kTagLocal) {
+  const uint64_t stream = (uint64_t(tag) << 48) | (uint64_t(rep) << 32);
   uint32_t attempt = 0;
   if (mode == GC_ENSEMBLE_SIMULTANEOUS) {
     for (;;) {
@@ -515,6 +522,11 @@ struct McmcArgs {
   const int32_t* partners;
   double *posterior, *mean, *stdv, *accept, *xfinal, *margin;
   uint8_t* trace;
+  // goodman_weare_wglobal_constant only
+  const float* g_uniforms;
+  const int32_t* g_partners;
+  double *g_posterior, *g_xfinal, *g_margin;
+  uint8_t* g_trace;
 };
 
 template <class S>
@@ -753,6 +765,305 @@ static void gw_run(const McmcArgs& a, int nthreads) {
   }
 }
 
+// Goodman-Weare with global parameters: utils/mcmc.m:385-578
+// (goodman_weare_wglobal_constant) and :581-626 (gw_global_block).  Walker
+// wk of repetition r carries NND local values per voxel and NG global values
+// shared by every voxel.  Each iteration runs a local GW block with the
+// globals fixed (exponent z^(NND-1)), then a global GW block with the locals
+// fixed, accepted on the FP64 sum over voxels of logP(proposal) - logPCurr
+// (exponent z^(NG-1)).  Repetitions restart from the same initial ensemble
+// and its logP0 (:497-501).
+template <class S>
+static void gwg_run(const McmcArgs& a, int nthreads) {
+  const gc_model& md = *a.md;
+  const gc_mcmc_cfg& c = *a.cfg;
+  const int Q = c.n_var, M = md.n_meas, E = md.n_extra, R = c.repetition, W = c.n_walker;
+  const int64_t nv = a.nv;
+  const int Ns = mcmc_ns(c);
+  const S aa = S(c.step_size);
+  const bool rb = c.ensemble_update == GC_ENSEMBLE_REDBLACK;
+  const int h = W / 2, hm = W - W / 2;
+  const int T = c.global_warmup + c.iteration;
+  std::vector<int> lq, gq;  // model index of each local / global variable
+  for (int q = 0; q < Q; ++q) (((c.global_mask >> q) & 1u) ? gq : lq).push_back(q);
+  const int NND = int(lq.size()), NG = int(gq.size());
+  const bool native = c.rng_mode == GC_RNG_NATIVE;
+
+  // partner tables (shared by all voxels): local block per iteration,
+  // global block per global step
+  std::vector<int32_t> ltab, gtab;
+  if (native) {
+    ltab.resize(size_t(R) * c.iteration * W);
+    gtab.resize(size_t(R) * T * W);
+    for (int r = 0; r < R; ++r) {
+      for (int k = 1; k <= c.iteration; ++k)
+        native_partners(c.seed, r, k, W, c.ensemble_update, &ltab[(size_t(r) * c.iteration + (k - 1)) * W], kTagLocal);
+      for (int t = 1; t <= T; ++t)
+        native_partners(c.seed, r, t, W, c.ensemble_update, &gtab[(size_t(r) * T + (t - 1)) * W], kTagGlobal);
+    }
+  }
+  const int32_t* lpart = native ? ltab.data() : a.partners;
+  const int32_t* gpart = native ? gtab.data() : a.g_partners;
+
+  std::vector<S> ex(size_t(nv) * (E > 0 ? E : 1));
+  for (int64_t v = 0; v < nv; ++v)
+    for (int e = 0; e < E; ++e) ex[size_t(v) * E + e] = S(a.extra[size_t(e) * nv + v]);
+  // full parameter vector of walker wk at voxel v from its local and global parts
+  auto assemble = [&](const S* xl, const S* g, S* x) {
+    for (int j = 0; j < NND; ++j) x[lq[j]] = xl[j];
+    for (int i = 0; i < NG; ++i) x[gq[i]] = g[i];
+  };
+  auto logp_at = [&](const S* xl, const S* g, int64_t v) {
+    S x[GC_MAX_PAR];
+    assemble(xl, g, x);
+    return logp_gaussian<S>(sos_of<S>(md, x, a.y, a.w, E > 0 ? &ex[size_t(v) * E] : nullptr, nv, v),
+                            x[c.noise_slot], M);
+  };
+
+  // ---- initial ensemble (:484-492) ----------------------------------------
+  // local: x0 + (ub-lb)*startRange.*randn([NND,Nv,W]), clamped
+  std::vector<S> xl0(size_t(W) * nv * NND), g0(size_t(W) * NG), lp0(size_t(W) * nv);
+  for (int wk = 0; wk < W; ++wk)
+    for (int64_t v = 0; v < nv; ++v)
+      for (int j = 0; j < NND; ++j) {
+        const int q = lq[j];
+        double n;
+        if (!native) n = a.normals[(size_t(wk) * nv + v) * NND + j];
+        else n = native_normal(c.seed, 0u, uint64_t(wk) * uint64_t(nv) + uint64_t(v), PURPOSE_INIT, j);
+        S xi = S(a.x0[size_t(q) * nv + v]) + S(md.ub[q] - md.lb[q]) * S(c.start_range) * S(n);
+        xl0[(size_t(wk) * nv + v) * NND + j] = std::min(std::max(xi, S(md.lb[q])), S(md.ub[q]));
+      }
+  // global: over-dispersed, lb + (ub-lb).*rand([NG,1,W]) (:489)
+  for (int wk = 0; wk < W; ++wk)
+    for (int i = 0; i < NG; ++i) {
+      const int q = gq[i];
+      double u;
+      if (!native) u = a.g_uniforms[size_t(wk) * NG + i];
+      else { uint32_t o[4]; draw(c.seed, 0u, global_init_chain(wk), PURPOSE_INIT + uint32_t(i / 4), o); u = u01(o[i % 4]); }
+      g0[size_t(wk) * NG + i] = S(md.lb[q]) + S(md.ub[q] - md.lb[q]) * S(u);
+    }
+  // logP0 at (local, global) of every walker (:492)
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t v = 0; v < nv; ++v)
+    for (int wk = 0; wk < W; ++wk)
+      lp0[size_t(wk) * nv + v] = logp_at(&xl0[(size_t(wk) * nv + v) * NND], &g0[size_t(wk) * NG], v);
+
+  std::vector<double> s1(size_t(NND) * nv, 0.0), s2(size_t(NND) * nv, 0.0), gs1(NG, 0.0), gs2(NG, 0.0);
+  std::vector<int64_t> nacc(nv, 0);
+  std::vector<S> xl, g, lp, xp(size_t(W) * nv * NND), lpp(size_t(W) * nv), zz(size_t(W) * nv);
+  std::vector<S> gp(size_t(W) * NG), lpg(size_t(W) * nv), zzg(W);
+  std::vector<uint8_t> oob(size_t(W) * nv), oobg(W);
+  std::vector<double> dlogp(W);
+  const size_t ubase_n = rb ? 2 * size_t(hm) : 2 * size_t(W);  // uniform rows per (sub-)step
+
+  for (int r = 0; r < R; ++r) {
+    xl = xl0;
+    g = g0;
+    lp = lp0;
+
+    // ---- global block, gw_global_block (:581-626) --------------------------
+    auto global_block = [&](int t) {
+      const size_t step = size_t(r) * T + (t - 1);
+      const int32_t* part = gpart + step * W;
+      const int nsub = rb ? 2 : 1;
+      for (int sub = 0; sub < nsub; ++sub) {
+        int a0 = 0, na = W, pofs = 0;
+        if (rb) { a0 = (sub == 0) ? 0 : h; na = (sub == 0) ? h : W - h; pofs = (sub == 0) ? 0 : h; }
+        // draws: zz_g then acceptance, per active walker
+        std::vector<double> uacc(na);
+        for (int j = 0; j < na; ++j) {
+          const int wk = a0 + j;
+          double u1, u2;
+          if (!native) {
+            const size_t b = size_t(W) * NG + (rb ? (step * 2 + sub) * 2 * size_t(hm) : step * 2 * size_t(W));
+            const size_t st = rb ? size_t(hm) : size_t(W);
+            u1 = a.g_uniforms[b + j];
+            u2 = a.g_uniforms[b + st + j];
+          } else {
+            uint32_t o[4];
+            draw(c.seed, uint32_t(t), global_chain(r, wk), PURPOSE_ZZ + sub, o);
+            u1 = u01(o[0]);
+            draw(c.seed, uint32_t(t), global_chain(r, wk), PURPOSE_ACCEPT + 0x100u * sub, o);
+            u2 = u01(o[0]);
+          }
+          uacc[j] = u2;
+          const S tt = (aa - S(1)) * S(u1) + S(1);
+          zzg[wk] = tt * tt / aa;  // :587
+          const int pw = rb ? part[pofs + j] : part[wk];
+          bool ob = false;
+          for (int i = 0; i < NG; ++i) {
+            const int q = gq[i];
+            const S xa = g[size_t(pw) * NG + i];
+            S val = xa + (g[size_t(wk) * NG + i] - xa) * zzg[wk];  // :588
+            ob = ob || val < S(md.lb[q]) || val > S(md.ub[q]);      // :589
+            gp[size_t(wk) * NG + i] = std::min(std::max(val, S(md.lb[q])), S(md.ub[q]));
+          }
+          oobg[wk] = ob;
+        }
+        // logP at (current local, proposed global) of every voxel (:592)
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+        for (int64_t v = 0; v < nv; ++v)
+          for (int j = 0; j < na; ++j) {
+            const int wk = a0 + j;
+            lpg[size_t(wk) * nv + v] = logp_at(&xl[(size_t(wk) * nv + v) * NND], &gp[size_t(wk) * NG], v);
+          }
+        for (int j = 0; j < na; ++j) {
+          const int wk = a0 + j;
+          // dlogP = sum(double(logP_g - logPCurr), 2), in voxel order (:594)
+          double d = 0.0;
+          for (int64_t v = 0; v < nv; ++v) d += double(S(lpg[size_t(wk) * nv + v] - lp[size_t(wk) * nv + v]));
+          dlogp[wk] = d;
+          S zpow = S(1);
+          for (int i = 0; i < NG - 1; ++i) zpow *= zzg[wk];
+          const S ratio = min1<S>(zpow * S(std::exp(d)));  // :595
+          const bool acc = ratio > S(uacc[j]) && !oobg[wk];  // :596-597
+          if (a.g_trace) a.g_trace[step * W + wk] = acc ? 1 : 0;
+          if (a.g_margin)
+            a.g_margin[step * W + wk] = oobg[wk] ? -INFINITY : double(NG - 1) * std::log(double(zzg[wk])) + d - std::log(uacc[j]);
+          if (acc) {  // :598-599
+            for (int i = 0; i < NG; ++i) g[size_t(wk) * NG + i] = gp[size_t(wk) * NG + i];
+            for (int64_t v = 0; v < nv; ++v) lp[size_t(wk) * nv + v] = lpg[size_t(wk) * nv + v];
+          }
+        }
+      }
+    };
+
+    // optional global-only warm-up, locals frozen (:503-507)
+    for (int t = 1; t <= c.global_warmup; ++t) global_block(t);
+
+    int counter = 0;
+    for (int k = 1; k <= c.iteration; ++k) {
+      // ---- local block, globals fixed (:512-545) ---------------------------
+      const size_t step = size_t(r) * c.iteration + (k - 1);
+      const int32_t* part = lpart + step * W;
+      const int nsub = rb ? 2 : 1;
+      for (int sub = 0; sub < nsub; ++sub) {
+        int a0 = 0, na = W, pofs = 0;
+        if (rb) { a0 = (sub == 0) ? 0 : h; na = (sub == 0) ? h : W - h; pofs = (sub == 0) ? 0 : h; }
+        const size_t ub0 = rb ? (step * 2 + sub) * ubase_n : step * ubase_n;
+        const size_t ust = rb ? size_t(hm) : size_t(W);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+        for (int64_t v = 0; v < nv; ++v) {
+          for (int j = 0; j < na; ++j) {
+            const int wk = a0 + j;
+            const int pw = rb ? part[pofs + j] : part[wk];
+            double u1;
+            if (!native) u1 = a.uniforms[(ub0 + j) * nv + v];
+            else {
+              uint32_t o[4];
+              draw(c.seed, uint32_t(k), (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v), PURPOSE_ZZ + sub, o);
+              u1 = u01(o[0]);
+            }
+            const S tt = (aa - S(1)) * S(u1) + S(1);
+            const size_t wv = size_t(wk) * nv + v;
+            zz[wv] = tt * tt / aa;  // :515
+            bool ob = false;
+            for (int jj = 0; jj < NND; ++jj) {
+              const int q = lq[jj];
+              const S xa = xl[(size_t(pw) * nv + v) * NND + jj];
+              S val = xa + (xl[wv * NND + jj] - xa) * zz[wv];  // :516
+              ob = ob || val < S(md.lb[q]) || val > S(md.ub[q]);  // :517
+              xp[wv * NND + jj] = std::min(std::max(val, S(md.lb[q])), S(md.ub[q]));
+            }
+            oob[wv] = ob;
+            lpp[wv] = logp_at(&xp[wv * NND], &g[size_t(wk) * NG], v);  // (new local, old global), :519-520
+          }
+          for (int j = 0; j < na; ++j) {
+            const int wk = a0 + j;
+            const size_t wv = size_t(wk) * nv + v;
+            double u2;
+            if (!native) u2 = a.uniforms[(ub0 + ust + j) * nv + v];
+            else {
+              uint32_t o[4];
+              draw(c.seed, uint32_t(k), (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v),
+                   PURPOSE_ACCEPT + 0x100u * sub, o);
+              u2 = u01(o[0]);
+            }
+            S zpow = S(1);
+            for (int i = 0; i < NND - 1; ++i) zpow *= zz[wv];
+            const S ratio = min1<S>(zpow * std::exp(lpp[wv] - lp[wv]));  // :521
+            const bool acc = ratio > S(u2) && !oob[wv];                  // :522-523
+            if (a.trace) a.trace[(step * W + wk) * nv + v] = acc ? 1 : 0;
+            if (a.margin)
+              a.margin[(step * W + wk) * nv + v] =
+                  oob[wv] ? -INFINITY : double(NND - 1) * std::log(double(zz[wv])) + double(lpp[wv] - lp[wv]) - std::log(u2);
+            if (acc) {  // :524-525
+              for (int jj = 0; jj < NND; ++jj) xl[wv * NND + jj] = xp[wv * NND + jj];
+              lp[wv] = lpp[wv];
+              ++nacc[v];
+            }
+          }
+        }
+      }
+      // ---- global block (:549-551) ------------------------------------------
+      global_block(c.global_warmup + k);
+
+      // thinning / burn-in (:554-558)
+      if (keep_sample(c, k)) {
+        if (counter < Ns) {
+          for (int wk = 0; wk < W; ++wk) {
+            for (int64_t v = 0; v < nv; ++v)
+              for (int j = 0; j < NND; ++j) {
+                const double xv = double(xl[(size_t(wk) * nv + v) * NND + j]);
+                if (a.posterior) a.posterior[(((size_t(j) * R + r) * Ns + counter) * W + wk) * nv + v] = xv;
+                const double d = xv - double(a.x0[size_t(lq[j]) * nv + v]);
+                s1[size_t(j) * nv + v] += d;
+                s2[size_t(j) * nv + v] += d * d;
+              }
+            for (int i = 0; i < NG; ++i) {
+              const double gv = double(g[size_t(wk) * NG + i]);
+              if (a.g_posterior) a.g_posterior[((size_t(i) * R + r) * Ns + counter) * W + wk] = gv;
+              const double d = gv - 0.5 * (md.lb[gq[i]] + md.ub[gq[i]]);
+              gs1[i] += d;
+              gs2[i] += d * d;
+            }
+          }
+        }
+        ++counter;
+      }
+    }
+    // unfilled posterior slots are zeros (:481-482) and enter res2out
+    const int unfilled = Ns - std::min(counter, Ns);
+    for (int j = 0; j < NND; ++j)
+      for (int64_t v = 0; v < nv; ++v) {
+        const double d = -double(a.x0[size_t(lq[j]) * nv + v]);
+        s1[size_t(j) * nv + v] += d * double(unfilled) * W;
+        s2[size_t(j) * nv + v] += d * d * double(unfilled) * W;
+      }
+    for (int i = 0; i < NG; ++i) {
+      const double d = -0.5 * (md.lb[gq[i]] + md.ub[gq[i]]);
+      gs1[i] += d * double(unfilled) * W;
+      gs2[i] += d * d * double(unfilled) * W;
+    }
+    if (a.xfinal)
+      for (int wk = 0; wk < W; ++wk)
+        for (int j = 0; j < NND; ++j)
+          for (int64_t v = 0; v < nv; ++v)
+            a.xfinal[((size_t(r) * W + wk) * NND + j) * nv + v] = double(xl[(size_t(wk) * nv + v) * NND + j]);
+    if (a.g_xfinal)
+      for (int wk = 0; wk < W; ++wk)
+        for (int i = 0; i < NG; ++i) a.g_xfinal[(size_t(r) * W + wk) * NG + i] = double(g[size_t(wk) * NG + i]);
+  }
+  const double n = double(Ns) * W * R;
+  auto stdv = [&](double a1, double a2) {
+    return (n > 1) ? std::sqrt(std::max(0.0, (a2 - a1 * a1 / n) / (n - 1))) : 0.0;
+  };
+  for (int j = 0; j < NND; ++j)
+    for (int64_t v = 0; v < nv; ++v) {
+      const size_t qv = size_t(lq[j]) * nv + v;
+      if (a.mean) a.mean[qv] = double(a.x0[qv]) + s1[size_t(j) * nv + v] / n;
+      if (a.stdv) a.stdv[qv] = stdv(s1[size_t(j) * nv + v], s2[size_t(j) * nv + v]);
+    }
+  for (int i = 0; i < NG; ++i)
+    for (int64_t v = 0; v < nv; ++v) {
+      const size_t qv = size_t(gq[i]) * nv + v;
+      if (a.mean) a.mean[qv] = 0.5 * (md.lb[gq[i]] + md.ub[gq[i]]) + gs1[i] / n;
+      if (a.stdv) a.stdv[qv] = stdv(gs1[i], gs2[i]);
+    }
+  if (a.accept)
+    for (int64_t v = 0; v < nv; ++v) a.accept[v] = double(nacc[v]) / (double(c.iteration) * R * W);
+}
+
 template <class F>
 static int with_np(int P, F&& f) {
   if (P <= 2) return f(std::integral_constant<int, 2>{});
@@ -778,6 +1089,12 @@ int orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 
 int orc_native_partners(uint64_t seed, int32_t rep, int32_t it, int32_t W, int32_t mode, int32_t* out) {
   orc::native_partners(seed, rep, it, W, mode, out);
+  return 0;
+}
+
+// partner table of the global block of goodman_weare_wglobal_constant
+int orc_native_partners_global(uint64_t seed, int32_t rep, int32_t it, int32_t W, int32_t mode, int32_t* out) {
+  orc::native_partners(seed, rep, it, W, mode, out, orc::kTagGlobal);
   return 0;
 }
 
@@ -866,13 +1183,34 @@ int orc_mcmc(const gc_model* md, const gc_mcmc_cfg* cfg, int64_t nv, const float
   if (cfg->algorithm == GC_MCMC_GW && cfg->ensemble_update == GC_ENSEMBLE_REDBLACK && cfg->n_walker < 2)
     return GC_ERR_NWALKER;
   orc::McmcArgs a{md, cfg, nv, y, w, extra, x0, normals, uniforms, partners, posterior, mean, stdv, accept,
-                  xfinal, margin, trace};
+                  xfinal, margin, trace, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   const int nt = std::max(1, nthreads);
   if (cfg->algorithm == GC_MCMC_MH) {
     if (precision == 0) orc::mh_run<double>(a, nt); else orc::mh_run<float>(a, nt);
   } else {
     if (precision == 0) orc::gw_run<double>(a, nt); else orc::gw_run<float>(a, nt);
   }
+  return 0;
+}
+
+// goodman_weare_wglobal_constant (cfg->global_mask != 0).  Layouts of the
+// draws and outputs as in gc_mcmc_io; g_margin receives the global block's
+// log(acceptance ratio) - log(u) per decision, [rep][T][W].
+int orc_mcmc_global(const gc_model* md, const gc_mcmc_cfg* cfg, int64_t nv, const float* y, const float* w,
+                    const float* extra, const float* x0, const float* normals, const float* uniforms,
+                    const int32_t* partners, const float* g_uniforms, const int32_t* g_partners, double* posterior,
+                    double* g_posterior, double* mean, double* stdv, double* accept, double* xfinal,
+                    double* g_xfinal, uint8_t* trace, uint8_t* g_trace, double* margin, double* g_margin,
+                    int32_t precision, int32_t nthreads) {
+  if (md->n_meas > 1024 || cfg->n_var > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
+  if (cfg->algorithm != GC_MCMC_GW || cfg->n_walker < 2) return GC_ERR_NWALKER;
+  const uint32_t all = (cfg->n_var >= 32) ? 0xFFFFFFFFu : ((1u << cfg->n_var) - 1u);
+  if ((cfg->global_mask & all) == 0 || (cfg->global_mask & all) == all || (cfg->global_mask & ~all) != 0)
+    return GC_ERR_ARG;
+  orc::McmcArgs a{md, cfg, nv, y, w, extra, x0, normals, uniforms, partners, posterior, mean, stdv, accept,
+                  xfinal, margin, trace, g_uniforms, g_partners, g_posterior, g_xfinal, g_margin, g_trace};
+  const int nt = std::max(1, nthreads);
+  if (precision == 0) orc::gwg_run<double>(a, nt); else orc::gwg_run<float>(a, nt);
   return 0;
 }
 

@@ -60,6 +60,8 @@ def lib():
         L.orc_mcmc.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_int32, c_int32]
+        L.orc_native_partners_global.argtypes = [c_uint64, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32)]
+        L.orc_mcmc_global.argtypes = [c_void_p, c_void_p, c_int64] + [c_void_p] * 20 + [c_int32, c_int32]
         _lib = L
     return _lib
 
@@ -87,6 +89,12 @@ def philox(ctr, key):
 def native_partners(seed, rep, it, W, mode):
     out = (c_int32 * W)()
     lib().orc_native_partners(seed, rep, it, W, mode, out)
+    return np.array(out, dtype=np.int32)
+
+
+def native_partners_global(seed, rep, it, W, mode):
+    out = (c_int32 * W)()
+    lib().orc_native_partners_global(seed, rep, it, W, mode, out)
     return np.array(out, dtype=np.int32)
 
 
@@ -181,3 +189,42 @@ def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partne
                         _p(post), _p(mean), _p(std), _p(acc), _p(xf), _p(tr), _p(mg), precision, nthreads)
     assert rc == 0, rc
     return dict(posterior=post, mean=mean, std=std, accept=acc, xfinal=xf, trace=tr, margin=mg, n_sample=ns)
+
+
+def mcmc_global(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partners=None, g_uniforms=None,
+                g_partners=None, store=True, trace=False, margin=False, precision=0, nthreads=1):
+    """goodman_weare_wglobal_constant; cfg.global_mask marks the global
+    variables.  Local outputs cover the NND local variables in model order."""
+    y = _f32(y)
+    M, nv = y.shape
+    x0 = _f32(x0)
+    Q = cfg.n_var
+    R = cfg.repetition
+    W = cfg.n_walker
+    it = cfg.iteration
+    T = cfg.global_warmup + it
+    NG = bin(cfg.global_mask & ((1 << Q) - 1)).count("1")
+    NND = Q - NG
+    ns = 0 if it < cfg.n_burnin + 1 else (it - cfg.n_burnin - 1) // cfg.thinning + 1
+    post = np.zeros((NND, R, ns, W, nv)) if store else None
+    gpost = np.zeros((NG, R, ns, W)) if store else None
+    mean = np.zeros((Q, nv))
+    std = np.zeros((Q, nv))
+    acc = np.zeros(nv)
+    xf = np.zeros((R, W, NND, nv))
+    gxf = np.zeros((R, W, NG))
+    tr = np.zeros((R, it, W, nv), dtype=np.uint8) if trace else None
+    gtr = np.zeros((R, T, W), dtype=np.uint8) if trace else None
+    mg = np.zeros((R, it, W, nv)) if margin else None
+    gmg = np.zeros((R, T, W)) if margin else None
+
+    def ip(a):
+        return None if a is None else np.ascontiguousarray(a, np.int32).ctypes.data_as(c_void_p)
+
+    rc = lib().orc_mcmc_global(ctypes.byref(md), ctypes.byref(cfg), nv, _p(y), _p(_f32(w)), _p(_f32(extra)), _p(x0),
+                               _p(_f32(normals)), _p(_f32(uniforms)), ip(partners), _p(_f32(g_uniforms)),
+                               ip(g_partners), _p(post), _p(gpost), _p(mean), _p(std), _p(acc), _p(xf), _p(gxf),
+                               _p(tr), _p(gtr), _p(mg), _p(gmg), precision, nthreads)
+    assert rc == 0, rc
+    return dict(posterior=post, g_posterior=gpost, mean=mean, std=std, accept=acc, xfinal=xf, g_xfinal=gxf,
+                trace=tr, g_trace=gtr, margin=mg, g_margin=gmg, n_sample=ns)

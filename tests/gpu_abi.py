@@ -100,7 +100,15 @@ def askadam(md, cfg, vols, trace=False):
     return outs
 
 
-def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partners=None, store=True, trace=False):
+def _prefilled(c, shape, fill):
+    t = c.empty(shape)
+    if fill is not None:
+        t.fill_(fill)
+    return t
+
+
+def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partners=None, store=True, trace=False,
+         post_fill=None):
     c = ctx()
     y = np.ascontiguousarray(y, np.float32)
     M, nv = y.shape
@@ -116,7 +124,7 @@ def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partne
     nd = c.to_device(normals) if normals is not None else None
     ud = c.to_device(uniforms) if uniforms is not None else None
     pd = c.to_device(np.asarray(partners, np.int32), np.int32) if partners is not None else None
-    post = c.empty((Q, R, ns, W, nv)) if store and ns > 0 else None
+    post = _prefilled(c, (Q, R, ns, W, nv), post_fill) if store and ns > 0 else None
     mean = c.empty((Q, nv))
     std = c.empty((Q, nv))
     acc = c.empty((nv,))
@@ -131,6 +139,39 @@ def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partne
     return dict(posterior=None if post is None else runtime.to_host(post), mean=runtime.to_host(mean),
                 std=runtime.to_host(std), accept=runtime.to_host(acc), xfinal=runtime.to_host(xf),
                 trace=None if tr is None else runtime.to_host(tr), n_sample=io.n_sample, n_stored=io.n_stored)
+
+
+def mcmc_global(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partners=None, g_uniforms=None,
+                g_partners=None, store=True, trace=False, post_fill=None):
+    """gc_mcmc_run with cfg.global_mask set (goodman_weare_wglobal_constant)."""
+    c = ctx()
+    y = np.ascontiguousarray(y, np.float32)
+    M, nv = y.shape
+    Q, R, W = cfg.n_var, cfg.repetition, cfg.n_walker
+    NG = bin(cfg.global_mask & ((1 << Q) - 1)).count("1")
+    NL = Q - NG
+    it = cfg.iteration
+    T = cfg.global_warmup + it
+    ns = 0 if it < cfg.n_burnin + 1 else (it - cfg.n_burnin - 1) // cfg.thinning + 1
+    io = _abi.gc_mcmc_io()
+    dev = lambda a, dt=np.float32: None if a is None else c.to_device(np.ascontiguousarray(a, dt), dt)
+    bufs = dict(y_dev=dev(y), x0_dev=dev(x0), w_dev=dev(w), extra_dev=dev(extra), normals_dev=dev(normals),
+                uniforms_dev=dev(uniforms), partners_dev=dev(partners, np.int32), g_uniforms_dev=dev(g_uniforms),
+                g_partners_dev=dev(g_partners, np.int32),
+                posterior_dev=_prefilled(c, (NL, R, ns, W, nv), post_fill) if store and ns > 0 else None,
+                g_posterior_dev=_prefilled(c, (NG, R, ns, W), post_fill) if store and ns > 0 else None,
+                mean_dev=c.empty((Q, nv)), std_dev=c.empty((Q, nv)), accept_dev=c.empty((nv,)),
+                xfinal_dev=c.empty((R, W, NL, nv)), g_xfinal_dev=c.empty((R, W, NG)),
+                trace_dev=c.zeros((R, it, W, nv), np.uint8) if trace else None,
+                g_trace_dev=c.zeros((R, T, W), np.uint8) if trace else None)
+    io.nv = nv
+    for name, t in bufs.items():
+        setattr(io, name, runtime.ptr(t))
+    c.check(c.lib.gc_mcmc_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), ctypes.byref(io)))
+    h = lambda k: None if bufs[k] is None else runtime.to_host(bufs[k])
+    return dict(posterior=h("posterior_dev"), g_posterior=h("g_posterior_dev"), mean=h("mean_dev"), std=h("std_dev"),
+                accept=h("accept_dev"), xfinal=h("xfinal_dev"), g_xfinal=h("g_xfinal_dev"), trace=h("trace_dev"),
+                g_trace=h("g_trace_dev"), n_sample=io.n_sample, n_stored=io.n_stored, kernel_ms=io.kernel_ms)
 
 
 def tv(img, mask, mode, vsize, lam, nmasked):

@@ -130,3 +130,24 @@ def matlab_mode_bins(a, lb, ub, nbin):
         b = int(np.argmax(counts))  # smallest bin on ties, as MATLAB mode
         out[i] = (edges[b] + edges[b + 1]) / 2
     return out
+
+
+def wglobal_draws(rng, R, it, warm, W, nv, NND, NG, redblack):
+    """Replay draws for goodman_weare_wglobal_constant in the C-ABI layouts:
+    (normals, uniforms, partners, g_uniforms, g_partners).  Partners are
+    derangements (simultaneous) or anchors from the other half (red-black)."""
+    T = warm + it
+    h, hm = W // 2, W - W // 2
+    normals = rng.standard_normal((W, nv, NND)).astype(np.float32)
+    uni = rng.uniform(0, 1, (R, it, 2, 2, hm, nv) if redblack else (R, it, 2, W, nv)).astype(np.float32)
+    gu = np.concatenate([rng.uniform(0, 1, W * NG),
+                         rng.uniform(0, 1, R * T * 2 * (2 * hm if redblack else W))]).astype(np.float32)
+
+    def table(steps):
+        if redblack:
+            return np.concatenate([rng.integers(h, W, (R, steps, h)), rng.integers(0, h, (R, steps, W - h))],
+                                  axis=2).astype(np.int32)
+        return np.stack([[np.roll(np.arange(W), 1 + rng.integers(0, W - 1)) for _ in range(steps)]
+                         for _ in range(R)]).astype(np.int32)
+
+    return normals, uni, table(it), gu, table(T)

@@ -106,7 +106,8 @@ def test_gw_replay_matches_oracle(redblack):
                         break
                 partners[r, k] = p
     x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
-    g = gpu_abi.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, partners=partners, trace=True)
+    g = gpu_abi.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, partners=partners, trace=True,
+                     post_fill=np.nan)
     o = orc.mcmc(md, cfg, y, x0, normals=normals, uniforms=uniforms, partners=partners, trace=True, margin=True)
     d = g["trace"] != o["trace"]  # [R, it, W, nv]
     for v in np.where(d.any(axis=(0, 1, 2)))[0]:
@@ -189,3 +190,21 @@ def test_mh_native_statistics_analytic_target():
     z = (g["mean"][0] - post_mean) / post_sd
     assert abs(np.mean(z)) < 0.05 and np.std(z) < 0.2, (np.mean(z), np.std(z))
     np.testing.assert_allclose(np.median(g["std"][0]), post_sd, rtol=0.05)
+
+
+@pytest.mark.parametrize("algo", [_abi.GC_MCMC_MH, _abi.GC_MCMC_GW])
+def test_unfilled_posterior_slots_are_zero(algo):
+    """iteration 30, burn-in 5, thinning 3: Ns = numel(6:3:30) = 9 slots but
+    only 8 iterations pass the thinning test (utils/mcmc.m:170); the last
+    slot keeps the zeros of the preallocated posterior and enters the mean."""
+    nv = 200
+    y, th, sigma = r2s_problem(nv, seed=3)
+    md = r2s_setup({}, solver="mcmc")[2]
+    W = 8 if algo == _abi.GC_MCMC_GW else 1
+    cfg = mcmc_cfg(algo, 30, 3, 5, rep=2, W=W, steps=(0.01, 1, 0.001))
+    x0 = np.stack([th[0], th[1], np.full(nv, sigma)]).astype(np.float32)
+    g = gpu_abi.mcmc(md, cfg, y, x0, post_fill=np.nan)
+    assert (g["n_sample"], g["n_stored"]) == (9, 8)
+    assert np.all(g["posterior"][:, :, 8] == 0) and np.all(np.isfinite(g["posterior"]))
+    p = g["posterior"].astype(np.float64)  # [Q, R, Ns, W, nv]
+    np.testing.assert_allclose(g["mean"], p.transpose(0, 4, 1, 2, 3).reshape(3, nv, -1).mean(-1), rtol=1e-5)

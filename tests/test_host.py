@@ -110,3 +110,22 @@ def test_r2star_model_flags_uniform_echo_spacing():
     md = models.gpuR2starMapping(np.array([2, 5, 9, 14]) * 1e-3).gc_model(
         askadam.check_set_default_basic(dict(modelParams=["M0", "R2star"], lb=[0, 0], ub=[2, 200])), "askadam", None)
     assert not md.flags & _abi.GC_FLAG_UNIFORM_TE
+
+
+def test_mcmc_global_detection_and_errors():
+    """utils/mcmc.m:53-72: a one-element field after the first marks global
+    parameters; they need the ensemble sampler."""
+    from gacelle_amd._abi import GacelleError
+
+    te = np.arange(2, 48, 4) * 1e-3
+    data = np.ones((2, 2, 1, 12), np.float32)
+    mask = np.ones((2, 2, 1))
+    f = dict(modelParams=["M0", "R2star", "noise"], lb=[0, 0, 0.001], ub=[2, 200, 0.1], algorithm="MH")
+    pars0 = dict(M0=np.ones((2, 2, 1)), R2star=np.ones((2, 2, 1)), noise=np.float32(0.01))
+    with pytest.raises(GacelleError, match="ensemble"):
+        mcmc().optimisation(data, mask, None, pars0, f, models.gpuR2starMapping(te))
+    bad = dict(M0=np.float32(1), R2star=np.ones((2, 2, 1)), noise=np.ones((2, 2, 1)))
+    with pytest.raises(GacelleError, match="global"):
+        mcmc().optimisation(data, mask, None, bad, dict(f, algorithm="GW"), models.gpuR2starMapping(te))
+    g = mcmc.check_global_constant(dict(a=np.ones(3), b=np.float32(1), c=np.ones(1)), ["a", "b", "c"])
+    assert list(g) == [False, True, True]

@@ -39,6 +39,7 @@ GC_FLAG_NEXI_RE_GUARD = 1 << 2
 GC_FLAG_MCRMWI_COMPLEX = 1 << 3
 GC_FLAG_MCRMWI_EXCHANGE = 1 << 4
 GC_FLAG_UNIFORM_TE = 1 << 5
+GC_FLAG_MCRMWI_PINI_FA = 1 << 6
 
 GC_QUAD_TRAPZ = 0
 GC_QUAD_TABLE = 1

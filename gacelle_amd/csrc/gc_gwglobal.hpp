@@ -135,7 +135,7 @@ GC_DEV void gwg_load_voxel(const gc_model& md, const GwgParams& P, int64_t vv, f
     }
   }
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * P.nv + vv] : 0.f;
+  for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * P.nv + vv] : 0.f;
 }
 
 // Initial ensemble and logP0 (utils/mcmc.m:484-492): locals
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_gwg_init(const gc_model* __restrict_
     const int64_t v = tile * gpb + gi;
     const bool active = lane_ok && v < nv;
     const int64_t vv = active ? v : 0;
-    float yr[Model::MR], wr[HAS_W ? Model::MR : 1], ex[8];
+    float yr[Model::MR], wr[HAS_W ? Model::MR : 1], ex[kMaxExtra];
     gwg_load_voxel<Model, HAS_W>(md, P, vv, yr, wr, ex);
     float x[Q];
 #pragma unroll
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_gwg_step(const gc_model* __restrict_
     const int64_t v = tile * gpb + gi;
     const bool active = lane_ok && v < nv;
     const int64_t vv = active ? v : 0;
-    float yr[Model::MR], wr[HAS_W ? Model::MR : 1], ex[8];
+    float yr[Model::MR], wr[HAS_W ? Model::MR : 1], ex[kMaxExtra];
     gwg_load_voxel<Model, HAS_W>(md, P, vv, yr, wr, ex);
     const int cc = lane_ok ? c : 0;
     float x[Q];

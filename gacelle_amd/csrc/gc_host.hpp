@@ -125,7 +125,8 @@ inline int model_np(const gc_model& md) {
 inline int check_model(gc_ctx* ctx, const gc_model* md) {
   if (!md) return set_err(ctx, GC_ERR_ARG, "model is NULL");
   if (md->n_par < 1 || md->n_par > GC_MAX_PAR) return set_err(ctx, GC_ERR_ARG, "n_par out of range");
-  if (md->n_extra < 0 || md->n_extra > 8) return set_err(ctx, GC_ERR_ARG, "n_extra out of range (0..8)");
+  if (md->n_extra < 0 || md->n_extra > kMaxExtra)
+    return set_err(ctx, GC_ERR_ARG, "n_extra out of range (0..%d)", kMaxExtra);
   const int np = model_np(*md);
   if (np < 0) return set_err(ctx, GC_ERR_UNSUPPORTED, "model %d is not supported", md->model_id);
   if (md->n_par < np) return set_err(ctx, GC_ERR_ARG, "n_par (%d) below the model's %d parameters", md->n_par, np);
@@ -144,9 +145,10 @@ inline int check_model(gc_ctx* ctx, const gc_model* md) {
   }
   if (md->model_id == GC_MODEL_MCRMWI) {
     if (md->i[0] < 1 || md->i[0] > 32 || md->i[1] < 1) return set_err(ctx, GC_ERR_ARG, "MCRMWI: 1..32 echoes");
-    if (md->n_extra < 3 + md->i[1])
-      return set_err(ctx, GC_ERR_ARG, "MCRMWI needs %d extra rows (B1, pini, ff, freqBKG per flip angle)",
-                     3 + md->i[1]);
+    const int need = 3 + md->i[1] * ((md->flags & GC_FLAG_MCRMWI_PINI_FA) ? 2 : 1);
+    if (md->n_extra < need)
+      return set_err(ctx, GC_ERR_ARG,
+                     "MCRMWI needs %d extra rows (B1, pini, ff, freqBKG per flip angle[, pini per flip angle])", need);
   }
   if (md->model_id == GC_MODEL_NEXI) {
     const int ns = md->i[0], nn = md->i[2];

@@ -20,6 +20,9 @@
 namespace gc {
 
 constexpr int kBlock = 256;
+// rows of per-voxel auxiliary data a kernel holds in registers (rows a model
+// never reads are dead code)
+constexpr int kMaxExtra = 12;
 
 // Kernels that are not templates have internal linkage (static): several
 // translation units include this header and launch them.
@@ -219,9 +222,9 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
       if (HAS_W) wr[HAS_W ? r : 0] = 0.f;
     }
   }
-  float ex[8];
+  float ex[kMaxExtra];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
+  for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
 
   float z[NP], m1[NP], m2[NP], lbp[NP], ubp[NP];
 #pragma unroll
@@ -434,9 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
   const int64_t vv = valid ? v : (nv - 1);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-  float ex[8];
+  float ex[kMaxExtra];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
+  for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
   float z[NP], m1[NP], m2[NP], th[NP], dth[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -672,9 +675,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ md
   const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const bool valid = v < nv;
   const int64_t vv = valid ? v : (nv - 1);
-  float ex[8];
+  float ex[kMaxExtra];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? ep.ex[int64_t(e) * nv + vv] : 0.f;
+  for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? ep.ex[int64_t(e) * nv + vv] : 0.f;
   float th[NP], dth[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p)
@@ -934,9 +937,9 @@ __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp,
       if (HAS_W) wr[HAS_W ? rr : 0] = 0.f;
     }
   }
-  float ex[8];
+  float ex[kMaxExtra];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * nv + vv] : 0.f;
+  for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * nv + vv] : 0.f;
   float x0[Q], x[Q], lbq[Q], ubq[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -1069,9 +1072,9 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
       if (HAS_W) wr[HAS_W ? rr : 0] = 0.f;
     }
   }
-  float ex[8];
+  float ex[kMaxExtra];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * nv + vv] : 0.f;
+  for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? P.ex[int64_t(e) * nv + vv] : 0.f;
   float xc0[Q], x[Q], lbq[Q], ubq[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {

@@ -483,7 +483,8 @@ struct McrmwiModel {
     float fbkg = 0.f, pini = 0.f;
     if constexpr (CPLX) {
       fbkg = th[NA + 5 + fi] + ex[3 + fi];
-      pini = th[NA + 5 + MF] + ex[1];
+      // one initial phase per flip angle (rows 3+MF.., n_fa == MF) or one per voxel
+      pini = th[NA + 5 + MF] + ((md.flags & GC_FLAG_MCRMWI_PINI_FA) ? ex[3 + MF + fi] : ex[1]);
     }
     const bool uni = (md.flags & GC_FLAG_UNIFORM_TE) != 0;
     constexpr float k2pi = 6.283185307179586f;

@@ -583,10 +583,229 @@ class gpuMCRMWI(_ModelBase):
     def _param_size(self, name) -> int:
         return self.fa.size if name == "dfreqBKG" else 1
 
+    # -- fit / estimate glue (MCRMWI/gpuMCRMWI.m:217-555, :873-967) ---------
+    thres_R2star = 2.0  # s^-1, lower bound of a fittable voxel
+    thres_T1 = 3.1      # s, upper bound
+
+    @staticmethod
+    def R2star_trapezoidal(img, te):
+        """Closed-form R2* and S0 (MCRMWI/gpuMCRMWI.m:1035-1058):
+        T2* = trapz(|img|, te) / (img_1 - img_end), S0 = img_1 exp(R2* te_1)."""
+        return gpuR2starMapping.R2star_trapezoidal(img, te)
+
+    def despot1(self, img, mask, b1):
+        """DESPOT1 T1 / M0 by linear regression (MCRMWI/private/despot1.m,
+        Deoni et al. MRM 2003): S/sin(a) = E1 S/tan(a) + M0 (1 - E1) with
+        a = FA b1, T1 = -TR / log(E1).  img [x,y,z,nfa]; returns (t1, m0,
+        mask_fitted) on the grid, zero outside the mask and where the fit is
+        not physical (T1 <= 0 or not finite, M0 < 0 or not finite)."""
+        img = np.abs(np.asarray(img, np.float64))
+        dims = img.shape[:3]
+        nfa = img.shape[3]
+        mask = np.asarray(mask) > 0
+        b1 = np.ones(dims) if b1 is None else np.broadcast_to(np.asarray(b1, np.float64).reshape(
+            dims if np.size(b1) > 1 else (1, 1, 1)), dims)
+        t1 = np.zeros(dims)
+        m0 = np.zeros(dims)
+        S = img[mask]                                  # [n, nfa]
+        alpha = np.deg2rad(self.fa.astype(np.float64))[None, :] * b1[mask][:, None]
+        ok = S.sum(axis=1) != 0                        # zero signal: T1 = M0 = 0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            yv = S / np.sin(alpha)
+            xv = S / np.tan(alpha)
+            # least squares of yv = b0 xv + b1 over the flip angles (x \ y)
+            xm, ym = xv.mean(axis=1), yv.mean(axis=1)
+            sxx = ((xv - xm[:, None]) ** 2).sum(axis=1)
+            sxy = ((xv - xm[:, None]) * (yv - ym[:, None])).sum(axis=1)
+            e1 = sxy / sxx
+            icpt = ym - e1 * xm
+            t1v = -float(self.tr) / np.log(e1)
+            m0v = icpt / (1 - np.exp(-float(self.tr) / t1v))
+        t1v = np.where(ok, t1v, 0.0)
+        m0v = np.where(ok, m0v, 0.0)
+        t1[mask], m0[mask] = t1v, m0v
+        fitted = (t1 > 0) & np.isfinite(t1) & (m0 >= 0) & np.isfinite(m0) & mask
+        t1 = np.where(fitted, t1, 0.0)
+        m0 = np.where(fitted, m0, 0.0)
+        return t1, m0, fitted
+
+    def _grid(self, a, dims):
+        """An extraData map [x,y,z] or [x,y,z,nfa] (or [x,y,z,1,nfa]) as
+        [x,y,z,k] float64."""
+        a = np.asarray(a, np.float64)
+        if a.size == 1:
+            return np.full(tuple(dims) + (1,), float(a))
+        return a.reshape(tuple(dims) + (-1,), order="F")
+
+    def validate_data(self, data, extraData, mask, fitting):
+        """MCRMWI/gpuMCRMWI.m:873-944 for the fitted-DIMWI path: default
+        freqBKG (zeros), pini (phase of the first echo of the first flip
+        angle after removing freqBKG), b1 (ones); the mask keeps voxels with
+        DESPOT1 T1 <= thres_T1, R2* >= thres_R2star and a non-zero first
+        echo at every flip angle."""
+        data = np.asarray(data)
+        dims = utils.dims3(data)
+        ex = dict(extraData or {})
+        if not fitting["DIMWI"].get("isFitIWF", True) and "IWF" not in ex:
+            raise GacelleError(_abi.GC_ERR_ARG, "Field IWF is missing in exraData structure variable for DIMWI model")
+        if ex.get("freqBKG") is None:
+            ex["freqBKG"] = np.zeros(dims)
+            if fitting["isComplex"]:
+                print("Warning: No total field map is provided for fitting complex-valued data.")
+        if ex.get("pini") is None:
+            fb = self._grid(ex["freqBKG"], dims)        # [x,y,z,1 or nfa]
+            d11 = data.reshape(dims + (self.te.size, -1), order="F")[:, :, :, 0, 0][..., None]
+            ex["pini"] = np.angle(d11 / np.exp(1j * 2 * np.pi * fb * (self.B0 * self.gyro) * float(self.te[0])))
+        if ex.get("b1") is None:
+            ex["b1"] = np.ones(dims)
+            print("Warning: Missing B1+ map. Assuming B1=1 everywhere.")
+        for k in list(ex.keys()):
+            if ex[k] is not None:
+                ex[k] = np.asarray(ex[k], np.float32)
+        mag = np.abs(data.reshape(dims + (self.te.size, -1), order="F"))   # [x,y,z,nte,nfa]
+        t1, _, fitted = self.despot1(mag[:, :, :, 0, :], mask, self._grid(ex["b1"], dims)[..., 0])
+        r2s, _ = self.R2star_trapezoidal(mag.mean(axis=4), self.te)
+        mask = (np.asarray(mask) > 0) & (t1 <= self.thres_T1) & fitted
+        with np.errstate(invalid="ignore"):
+            mask = mask & (r2s >= self.thres_R2star)
+        mask = mask & np.all(mag[:, :, :, 0, :] > 0, axis=3)
+        return ex, mask
+
+    def prepare_data(self, data, mask, extraData, fitting):
+        """MCRMWI/gpuMCRMWI.m:950-967: validate, then scale the data by the
+        98th percentile of the DESPOT1 M0 of the first echo."""
+        mask = np.asarray(mask) > 0
+        extraData, mask = self.validate_data(data, extraData, mask, fitting)
+        dims = utils.dims3(data)
+        mag1 = np.abs(np.asarray(data).reshape(dims + (self.te.size, -1), order="F")[:, :, :, 0, :])
+        _, m0, fitted = self.despot1(mag1, mask, self._grid(extraData["b1"], dims)[..., 0])
+        scale = utils.prctile(m0[fitted], 98)
+        data = np.asarray(data) / scale
+        data, mask = utils.remove_img_naninf(data, mask)
+        return data, mask, extraData, scale
+
+    def estimate_prior(self, data, mask, extraData):
+        """MCRMWI/gpuMCRMWI.m:514-555: S0 and R1IEW from DESPOT1 on the
+        per-flip-angle S0 of the closed-form R2* fit; R2sIW / R2sEW = R2*
+        of the flip-angle mean -/+ 3 s^-1, kept 2 s^-1 inside the bounds."""
+        data = np.asarray(data)
+        dims = utils.dims3(data)
+        nte, nfa = self.te.size, self.fa.size
+        mag = np.abs(data.reshape(dims + (nte, nfa), order="F"))
+        pars0 = utils.initialise_x0(dims, self.modelParams, self.startPoint)
+        s0 = np.zeros(dims + (nfa,))
+        for k in range(nfa):
+            _, s0[..., k] = self.R2star_trapezoidal(mag[..., k], self.te)
+        s0[~np.isfinite(s0)] = 0
+        t1, m0, _ = self.despot1(s0, mask, self._grid((extraData or {}).get("b1", 1.0), dims)[..., 0])
+        pars0[self.modelParams[0]] = m0.astype(np.float32)
+        if "R1IEW" in self.modelParams:
+            i = self.modelParams.index("R1IEW")
+            with np.errstate(divide="ignore"):
+                r1 = 1.0 / t1
+            r1[~np.isfinite(r1)] = 0
+            pars0["R1IEW"] = np.clip(r1, self.lb[i], self.ub[i]).astype(np.float32)
+        if "dfreqBKG" in pars0:
+            pars0["dfreqBKG"] = np.repeat(pars0["dfreqBKG"][..., None], nfa, axis=3)
+        r2s, _ = self.R2star_trapezoidal(mag.mean(axis=4), self.te)
+        for name, off in (("R2sIW", -3.0), ("R2sEW", 3.0)):
+            if name not in self.modelParams:
+                continue
+            i = self.modelParams.index(name)
+            v = r2s + off
+            v[~np.isfinite(v)] = self.startPoint[i]
+            pars0[name] = np.maximum(np.minimum(v, self.ub[i] - 2), self.lb[i] + 2).astype(np.float32)
+        return pars0
+
+    def determine_x0(self, y, fitting, mask, extraData):
+        """MCRMWI/gpuMCRMWI.m:472-511.  As in the reference, a numeric
+        fitting.start is reported but the fixed start points are used."""
+        dims = utils.dims3(y)
+        start = fitting.get("start", "prior")
+        if isinstance(start, str) and start.lower() == "prior":
+            x0 = self.estimate_prior(y, mask, extraData)
+        else:
+            x0 = utils.initialise_x0(dims, self.modelParams, self.startPoint)
+            if "dfreqBKG" in x0:
+                x0["dfreqBKG"] = np.repeat(x0["dfreqBKG"][..., None], self.fa.size, axis=3)
+        from .askadam import askadam as _ask
+
+        return _ask.set_boundary(x0, fitting["ub"], fitting["lb"])
+
+    def compute_optimisation_weights(self, data, fitting):
+        """MCRMWI/gpuMCRMWI.m:441-467: '1stecho' |s|^p / |s(TE1)|^p per flip
+        angle, or 'norm' sqrt|s|, clamped to [0, 1]; repeated for the real
+        and imaginary halves of complex data."""
+        data = np.asarray(data)
+        dims = utils.dims3(data)
+        mag = np.abs(data.reshape(dims + (self.te.size, -1), order="F"))
+        if fitting.get("isWeighted", True):
+            if str(fitting["weightMethod"]).lower() == "norm":
+                w = np.sqrt(mag)
+            else:
+                p = fitting["weightPower"]
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    w = mag ** p / mag[:, :, :, :1, :] ** p
+        else:
+            w = np.ones_like(mag)
+        w = np.where(w > 1, 1, w)
+        w = np.where(w < 0, 0, w)
+        if fitting.get("isComplex", True):
+            w = np.stack([w, w], axis=5)
+        return w.astype(np.float32)
+
+    def fit(self, data, mask=None, fitting=None, extraData=None):
+        """MCRMWI/gpuMCRMWI.m:320-438 (isEPG = false: no pre-trained network
+        is loaded).  data: complex [x,y,z,nte,nfa] (or magnitude)."""
+        data = np.asarray(data)
+        dims = utils.dims3(data)
+        if mask is None:
+            mask = np.ones(dims, bool)
+        fitting = self.check_set_default(fitting, data)
+        self.updateProperty(fitting)
+        fitting["modelParams"] = list(self.modelParams)
+        if len(np.atleast_1d(fitting.get("ub", []))) == 0:
+            fitting["ub"] = list(self.ub)
+        if len(np.atleast_1d(fitting.get("lb", []))) == 0:
+            fitting["lb"] = list(self.lb)
+        pars0 = self.determine_x0(data, fitting, mask, extraData)
+        w = self.compute_optimisation_weights(data, fitting)
+        # a per-flip-angle freqBKG gives a per-flip-angle default pini
+        # (MCRMWI/gpuMCRMWI.m:890-893)
+        pini = (extraData or {}).get("pini")
+        if fitting["isComplex"] and pini is not None and np.asarray(pini).size == int(np.prod(dims)) * self.fa.size:
+            fitting["piniPerFA"] = True
+        if fitting["isComplex"]:
+            d5 = data.reshape(dims + (self.te.size, -1), order="F")
+            data = np.stack([d5.real, d5.imag], axis=5).astype(np.float32)
+        else:
+            data = np.abs(data).astype(np.float32)
+        out = self._solve(data, mask, w, pars0, fitting, extraData)
+        return out
+
+    def estimate(self, data, mask, extraData=None, fitting=None):
+        """MCRMWI/gpuMCRMWI.m:217-317 (one segment: the whole volume fits in
+        one launch): prepare_data, fit, then S0 back to the data scale."""
+        fitting = self.check_set_default(fitting, data)
+        data, mask, extraData, scale = self.prepare_data(data, mask, extraData, fitting)
+        out = self.fit(data, mask, fitting, extraData)
+        out["mask"] = mask
+        if str(fitting["solver"]).lower() == "mcmc":
+            for k in fitting.get("metric", ["mean", "std"]):
+                if k in out and "S0" in out[k]:
+                    out[k]["S0"] = out[k]["S0"] * scale
+            out["posterior"]["S0"] = out["posterior"]["S0"] * scale
+        else:
+            out["final"]["S0"] = out["final"]["S0"] * scale
+            out["min"]["S0"] = out["min"]["S0"] * scale
+        return out
+
     def _flags(self, fitting):
         f = 0
         if fitting.get("isComplex", True):
             f |= _abi.GC_FLAG_MCRMWI_COMPLEX
+            if fitting.get("piniPerFA", False):
+                f |= _abi.GC_FLAG_MCRMWI_PINI_FA
         if fitting.get("isFitExchange", True):
             f |= _abi.GC_FLAG_MCRMWI_EXCHANGE
         return f
@@ -602,7 +821,8 @@ class gpuMCRMWI(_ModelBase):
         nte, nfa = self.te.size, self.fa.size
         md.i[0], md.i[1] = nte, nfa
         md.n_meas = nte * nfa * (2 if md.flags & _abi.GC_FLAG_MCRMWI_COMPLEX else 1)
-        md.n_extra = 3 + nfa
+        self._pini_fa = bool(md.flags & _abi.GC_FLAG_MCRMWI_PINI_FA)
+        md.n_extra = 3 + nfa * (2 if self._pini_fa else 1)
         md.f[0], md.f[1], md.f[2] = self.tr, self.B0, self.gyro
         md.f[3], md.f[4], md.f[5] = self.rho_mw, self.t1_mw, self.freq_EW
         off = _fill_table(md, self.te)
@@ -616,11 +836,13 @@ class gpuMCRMWI(_ModelBase):
         return md
 
     def extra_rows(self, extraData, mask, vox, nv):
-        """[B1, pini, ff, freqBKG x nfa] per voxel (extraData maps on the image
-        grid, masked to G-D; defaults 1, 0, 1, 0)."""
+        """[B1, pini, ff, freqBKG x nfa (, pini x nfa)] per voxel (extraData
+        maps on the image grid, masked to G-D; defaults 1, 0, 1, 0).  A pini
+        with one map per flip angle needs fitting.piniPerFA (set by fit)."""
         ex = extraData or {}
         nfa = self.fa.size
-        rows = np.zeros((3 + nfa, nv), np.float32)
+        pini_fa = getattr(self, "_pini_fa", False)
+        rows = np.zeros((3 + nfa * (2 if pini_fa else 1), nv), np.float32)
 
         def gd(key, default, k=None):
             if key not in ex or ex[key] is None:
@@ -637,4 +859,11 @@ class gpuMCRMWI(_ModelBase):
         rows[2] = gd("ff", 1.0)
         for k in range(nfa):
             rows[3 + k] = gd("freqBKG", 0.0, k)
+        per_fa = ex.get("pini") is not None and np.size(ex["pini"]) > 1 and \
+            np.asarray(ex["pini"]).size == int(np.prod(utils.dims3(mask))) * nfa
+        if pini_fa:
+            for k in range(nfa):
+                rows[3 + nfa + k] = gd("pini", 0.0, k if per_fa else None)
+        elif per_fa and np.any(gd("pini", 0.0, nfa - 1) != rows[1]):
+            raise GacelleError(_abi.GC_ERR_ARG, "pini has one map per flip angle: set fitting['piniPerFA'] = True")
         return rows

@@ -67,6 +67,8 @@ enum {
   GC_FLAG_NEXI_RE_GUARD = 1 << 2,  /* re = ra*fa/max(1-fa,eps) (gpuNEXIrice.m:678) */
   GC_FLAG_MCRMWI_COMPLEX = 1 << 3, /* complex data: re/im stacked, dfreqBKG/dpini fitted */
   GC_FLAG_MCRMWI_EXCHANGE = 1 << 4, /* BM two-pool exchange steady state (kIEWM fitted) */
+  GC_FLAG_MCRMWI_PINI_FA = 1 << 6,  /* one initial phase per flip angle: extra
+                                     * rows 3+n_fa..3+2n_fa replace row 1    */
   GC_FLAG_UNIFORM_TE = 1 << 5       /* echo times equally spaced: f[8] = TE spacing;
                                        exp(-TE_k R) is formed by recurrence */
 };
@@ -96,7 +98,10 @@ enum { GC_QUAD_TRAPZ = 0, GC_QUAD_TABLE = 1 };
  *   MCRMWI  : i[0]=n_te, i[1]=n_fa; f[0]=TR [s], f[1]=B0 [T], f[2]=gyro
  *             [MHz/T], f[3]=rho_mw, f[4]=t1_mw [s], f[5]=freq_EW [ppm];
  *             table[0:n_te]=TE [s], table[n_te:n_te+n_fa]=FA [rad];
- *             extra rows: 0=B1, 1=pini [rad], 2=ff, 3..3+n_fa=freqBKG [ppm].
+ *             extra rows: 0=B1, 1=pini [rad], 2=ff, 3..3+n_fa=freqBKG [ppm],
+ *             and with GC_FLAG_MCRMWI_PINI_FA 3+n_fa..3+2n_fa = pini per
+ *             flip angle (the default pini of a per-FA freqBKG,
+ *             MCRMWI/gpuMCRMWI.m:890-893).  At most 12 extra rows.
  *             P = {S0,MWF,IWF,R1IEW,[kIEWM],R2sMW,R2sIW,R2sEW,freqMW,freqIW,
  *                  [dfreqBKG x n_fa, dpini]}  ([] depends on flags)
  *

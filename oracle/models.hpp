@@ -245,7 +245,8 @@ void fwd_mcrmwi(const gc_model& md, const V* th, const S* extra, V* s) {
     V fbkg = V(S(0)), pini = V(S(0));
     if (cplx) {
       fbkg = th[kbkg + fi] + extra[3 + fi];
-      pini = th[kpini] + extra[1];
+      // pini per flip angle when freqBKG is (MCRMWI/gpuMCRMWI.m:890-893, :580)
+      pini = th[kpini] + ((md.flags & GC_FLAG_MCRMWI_PINI_FA) ? extra[3 + nfa + fi] : extra[1]);
     }
     for (int t = 0; t < nte; ++t) {
       const S te = S(md.table[t]);

@@ -501,6 +501,34 @@ def test_mcrmwi_complex_phase_and_sum():
     np.testing.assert_allclose(orc.fwd_value(mc, thc, ex2)[:, 0], 2 * sc, rtol=1e-12)
 
 
+def test_mcrmwi_pini_per_flip_angle():
+    """GC_FLAG_MCRMWI_PINI_FA: flip angle k uses pini row 3+nfa+k.  Each
+    flip-angle block equals the one-pini model evaluated with that row's
+    value; with equal rows the two forms agree everywhere."""
+    te = np.array([3e-3, 7e-3, 12e-3])
+    m, f, mc = _mcrmwi_md(True, cplx=True, te=te)
+    f2 = dict(f, piniPerFA=True)
+    mp = m.gc_model(f2, "askadam", None)
+    assert mp.flags & _abi.GC_FLAG_MCRMWI_PINI_FA and mp.n_extra == 3 + 2 * 4
+    th = np.array([1.3, 0.15, 0.6, 1.1, 2.5, 80.0, 20.0, 25.0, 0.08, -0.02, 0.01, -0.02, 0.03, 0.0, 0.1])[:, None]
+    base = np.array([[1.0], [0.0], [0.9], [0.05], [0.0], [-0.05], [0.02]])
+    pinis = [0.3, -1.2, 2.0, 0.7]
+    ex_fa = np.vstack([base, np.array(pinis)[:, None]])
+    s_fa = orc.fwd_value(mp, th, ex_fa)[:, 0]
+    nte, nfa = 3, 4
+    for k, pk in enumerate(pinis):
+        exk = base.copy()
+        exk[1] = pk
+        sk = orc.fwd_value(mc, th, exk)[:, 0]
+        for half in (0, 1):
+            sl = slice(half * nte * nfa + k * nte, half * nte * nfa + (k + 1) * nte)
+            np.testing.assert_allclose(s_fa[sl], sk[sl], rtol=1e-12, atol=1e-15)
+    ex_eq = np.vstack([base, np.full((4, 1), 0.4)])
+    ex_one = base.copy()
+    ex_one[1] = 0.4
+    np.testing.assert_allclose(orc.fwd_value(mp, th, ex_eq), orc.fwd_value(mc, th, ex_one), rtol=1e-12)
+
+
 # ---------------------------------------------------------------------------
 # robustConvergence (utils/askadam.m:448-510, 541-566)
 def _robust_problem(nv=400, n_out=40, seed=9):

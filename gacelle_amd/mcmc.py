@@ -83,7 +83,7 @@ class mcmc:
             # scalar fields stay scalars (reshape_ND2GD_struct, utils/utils.m:166-178)
             x0[name] = a.ravel() if a.size == 1 else (a.ravel(order="F")[vox] if a.size == mask.size else a.ravel())
         # a field with one element while the first has N marks global
-        # parameters (utils/mcmc.m:53-63)
+        # parameters (utils/mcmc.m:54-63)
         names = list(fitting["modelParams"])
         n_first = x0[names[0]].size
         is_global = any(x0[n].size != n_first and x0[n].size == 1 for n in names[1:])

@@ -135,7 +135,7 @@ def test_two_globals_match_grid_posterior():
 
 def test_mirror_optimisation_with_a_shared_noise():
     """mcmc().optimisation with pars0.noise a scalar dispatches to
-    goodman_weare_wglobal_constant (utils/mcmc.m:53-72); global metrics are
+    goodman_weare_wglobal_constant (utils/mcmc.m:54-72); global metrics are
     scalars, local ones images."""
     from gacelle_amd import models
     from gacelle_amd.mcmc import mcmc as M

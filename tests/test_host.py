@@ -113,7 +113,7 @@ def test_r2star_model_flags_uniform_echo_spacing():
 
 
 def test_mcmc_global_detection_and_errors():
-    """utils/mcmc.m:53-72: a one-element field after the first marks global
+    """utils/mcmc.m:54-72: a one-element field after the first marks global
     parameters; they need the ensemble sampler."""
     from gacelle_amd._abi import GacelleError
 

@@ -101,6 +101,42 @@ GC_DEV float wave_sum4(float v0, float v1, float v2, float v3, int lane) {
 }
 GC_DEV int wave_sum4_index(int lane) { return ((lane & 1) << 1) | ((lane >> 1) & 1); }
 
+// Byte addresses of the two cross-row partners (lane ^ 16, lane ^ 32) for
+// ds_bpermute, computed once per thread outside the loops that reduce.
+struct XRow {
+  int a16, a32;
+};
+GC_DEV XRow xrow_addr(int lane) { return XRow{(lane ^ 16) << 2, (lane ^ 32) << 2}; }
+GC_DEV float bperm(int addr, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+GC_DEV float sum_class4(float k, XRow x) {
+  k += dpp<kDppRor4>(k);
+  k += dpp<kDppRor8>(k);
+  k += bperm(x.a16, k);
+  return k + bperm(x.a32, k);
+}
+GC_DEV float wave_sum4(float v0, float v1, float v2, float v3, int lane, XRow x) {
+  const bool b0 = lane & 1;
+  const float k0 = b0 ? v2 : v0, k1 = b0 ? v3 : v1;
+  const float s0 = b0 ? v0 : v2, s1 = b0 ? v1 : v3;
+  const float a0 = k0 + dpp<kDppXor1>(s0);
+  const float a1 = k1 + dpp<kDppXor1>(s1);
+  const bool b1 = lane & 2;
+  float k = b1 ? a1 : a0;
+  const float s = b1 ? a0 : a1;
+  k += dpp<kDppXor2>(s);
+  return sum_class4(k, x);
+}
+
+// Sum over the 16 lanes of each row: every lane holds its row's total.
+GC_DEV float row_sum16(float v) {
+  v += dpp<kDppXor1>(v);
+  v += dpp<kDppXor2>(v);
+  v += dpp<kDppRor4>(v);
+  return v + dpp<kDppRor8>(v);
+}
+
 GC_DEV double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

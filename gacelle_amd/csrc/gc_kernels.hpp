@@ -246,14 +246,21 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
   // the constant address space lets the wave-uniform loads go scalar.
   using ConstEpochScal = __attribute__((address_space(4))) const EpochScal;
   ConstEpochScal* es_table = reinterpret_cast<ConstEpochScal*>(reinterpret_cast<uintptr_t>(V.es));
+  // Per-iteration block sums.  With the norms (step and gradient stopping
+  // criteria) each wavefront reduces its four sums with DPP and cross-row
+  // permutes; for the loss alone every thread parks its value in an LDS ring
+  // that the block reduces once per kRing iterations (one row of 16 lanes per
+  // iteration), which keeps the reduction off the per-iteration VALU path.
   __shared__ float red[kRing][kBlock / 64][4];
+  __shared__ float redl[kRing][kBlock];
   const float inv_count = V.inv_count;
-  const bool norms = ap.need_norms != 0;
+  const XRow xr = xrow_addr(lane);
 
-  // The epoch loop, instantiated per parameter transform so that the
-  // transform costs no per-iteration selection.
-  auto epochs = [&](auto tr) {
+  // The epoch loop, instantiated per parameter transform and per norms flag
+  // so that neither costs a per-iteration selection.
+  auto epochs = [&](auto tr, auto nm) {
     constexpr int TR = decltype(tr)::value;
+    constexpr bool norms = decltype(nm)::value;
     for (int k = 0; k < V.n_epochs; ++k) {
       const int e = V.epoch0 + k;
       float th[NP], dth[NP];
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
       float g[NP];
 #pragma unroll
       for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * inv_count;
-      if (norms) {
+      if constexpr (norms) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
           den += z[p] * z[p];
@@ -317,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
         for (int p = 0; p < NP; ++p) {
           const float dz = opt_step(ap.o, es, g[p], m1[p], m2[p]);
           const float zn = z[p] + dz;
-          if (norms) {
+          if constexpr (norms) {
             const float d = zn - z[p];
             step += d * d;
           }
@@ -326,34 +333,53 @@ __global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__
       }
       if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
       const int slot = k & (kRing - 1);
-      if (norms) {
-        const float tot = wave_sum4(lsum, step, den, gn, lane);
+      if constexpr (norms) {
+        const float tot = wave_sum4(lsum, step, den, gn, lane, xr);
         if (lane < 4) red[slot][wave][wave_sum4_index(lane)] = tot;
       } else {
-        const float tot = wave_sum(lsum);
-        if (lane < 4) red[slot][wave][lane] = (lane == 0) ? tot : 0.f;
+        redl[slot][threadIdx.x] = lsum;
       }
       if (slot == kRing - 1 || k == V.n_epochs - 1) {
         __syncthreads();
         const int cnt = slot + 1;
         const int kbase = k - slot;
-        // npv sums per iteration: 4 with the norms, else the loss alone
-        const int lg = norms ? 2 : 0;
-        if (threadIdx.x < (cnt << lg)) {
-          const int kk = threadIdx.x >> lg, j = threadIdx.x & ((1 << lg) - 1);
-          float s = 0.f;
+        if constexpr (norms) {
+          // four sums per iteration: loss, step^2, |z|^2, |g|^2
+          if (threadIdx.x < (cnt << 2)) {
+            const int kk = threadIdx.x >> 2, j = threadIdx.x & 3;
+            float s = 0.f;
 #pragma unroll
-          for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
-          V.partials[((int64_t(lblk) * V.n_epochs + kbase + kk) << lg) + j] = s;
+            for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
+            V.partials[((int64_t(lblk) * V.n_epochs + kbase + kk) << 2) + j] = s;
+          }
+        } else {
+          // row kk of 16 lanes sums iteration kk's 256 values, 16 per lane
+          static_assert(kRing * 16 == kBlock, "one row of 16 lanes per ring slot");
+          const int kk = threadIdx.x >> 4, part = threadIdx.x & 15;
+          float s = 0.f;
+          if (kk < cnt) {
+            const float4* p4 = reinterpret_cast<const float4*>(&redl[kk][part * 16]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const float4 q = p4[i];
+              s += (q.x + q.y) + (q.z + q.w);
+            }
+          }
+          s = row_sum16(s);
+          if (part == 0 && kk < cnt) V.partials[int64_t(lblk) * V.n_epochs + kbase + kk] = s;
         }
         __syncthreads();
       }
     }
   };
-  if (ap.transform == GC_TRANSFORM_LINEAR)
-    epochs(std::integral_constant<int, GC_TRANSFORM_LINEAR>{});
-  else
-    epochs(std::integral_constant<int, GC_TRANSFORM_SIGMOID>{});
+  const bool with_norms = ap.need_norms != 0;
+  if (ap.transform == GC_TRANSFORM_LINEAR) {
+    if (with_norms) epochs(std::integral_constant<int, GC_TRANSFORM_LINEAR>{}, std::true_type{});
+    else epochs(std::integral_constant<int, GC_TRANSFORM_LINEAR>{}, std::false_type{});
+  } else {
+    if (with_norms) epochs(std::integral_constant<int, GC_TRANSFORM_SIGMOID>{}, std::true_type{});
+    else epochs(std::integral_constant<int, GC_TRANSFORM_SIGMOID>{}, std::false_type{});
+  }
 
   if (valid) {
 #pragma unroll
@@ -552,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
 
   // block sums of this iteration
   if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; reg = 0.f; msum = 0.f; }
-  const float t4 = wave_sum4(lsum, step, den, gn, lane);
+  const float t4 = wave_sum4(lsum, step, den, gn, lane, xrow_addr(lane));
   const float tr = wave_sum(reg);
   const float tm = wave_sum(msum);
   __shared__ float red[kBlock / 64][kStepSums];

@@ -2,6 +2,8 @@
 // dispatch to the per-model operation tables, layout helpers, TV and the
 // MCMC set-up.  The askadam driver is in gc_askadam.hpp; model-dependent
 // launches are compiled per model variant in ops_*.hip.
+#include <atomic>
+
 #include "gc_host.hpp"
 
 using namespace gc;
@@ -60,6 +62,9 @@ extern "C" {
 
 int gc_version(void) { return GC_ABI_VERSION; }
 
+// live contexts: the block caches are released with the last one
+static std::atomic<int> g_live_contexts{0};
+
 int gc_ctx_create(int32_t device, gc_ctx** out) {
   if (!out) return GC_ERR_ARG;
   *out = nullptr;
@@ -71,6 +76,7 @@ int gc_ctx_create(int32_t device, gc_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) { delete c; return GC_ERR_HIP; }
   c->stream = nullptr;  // the device's default stream until gc_ctx_set_stream
   c->own_stream = false;
+  g_live_contexts.fetch_add(1);
   *out = c;
   return GC_OK;
 }
@@ -81,6 +87,7 @@ int gc_ctx_destroy(gc_ctx* ctx) {
   if (ctx->d_model) (void)hipFree(ctx->d_model);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  if (g_live_contexts.fetch_sub(1) == 1) trim_caches();
   return GC_OK;
 }
 
@@ -162,6 +169,7 @@ int gc_fwd_adj(gc_ctx* ctx, const gc_model* md, int32_t loss, int32_t transform,
                const float* w, const float* extra, int64_t nv, float scale, float* grad, double* loss_sum,
                float* residual) {
   if (!ctx) return GC_ERR_ARG;
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
   int rc = check_model(ctx, md);
   if (rc) return rc;
   if (!x || !y || nv <= 0 || (md->n_extra > 0 && !extra)) return set_err(ctx, GC_ERR_ARG, "gc_fwd_adj: bad argument");
@@ -190,6 +198,7 @@ int gc_tv_fwd_adj(gc_ctx* ctx, const float* img, const uint8_t* mask, const int3
                   const double vsize[3], double lambda, int64_t n_masked, double* loss, float* grad) {
   if (!ctx || !img || !mask || !dims || !vsize || !loss || n_masked <= 0)
     return set_err(ctx, GC_ERR_ARG, "gc_tv_fwd_adj: bad argument");
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
   const int64_t n = int64_t(dims[0]) * dims[1] * dims[2];
   if (n <= 0) return set_err(ctx, GC_ERR_ARG, "gc_tv_fwd_adj: empty grid");
   const unsigned nb = nblocks(n);
@@ -213,6 +222,7 @@ int gc_tv_fwd_adj(gc_ctx* ctx, const float* img, const uint8_t* mask, const int3
 int gc_posterior_stats(gc_ctx* ctx, const float* post, int32_t Q, int32_t n, int64_t nv, const float* lb,
                        const float* ub, int32_t nbin, float* median, float* iqr, float* mode) {
   if (!ctx) return GC_ERR_ARG;
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
   if (!post || Q < 1 || Q > GC_MAX_PAR || n < 1 || n > 16384 || nv < 1)
     return set_err(ctx, GC_ERR_ARG, "gc_posterior_stats: bad argument");
   if (mode && (!lb || !ub || nbin < 2)) return set_err(ctx, GC_ERR_ARG, "gc_posterior_stats: mode needs lb, ub, nbin");
@@ -258,6 +268,7 @@ int gc_posterior_stats(gc_ctx* ctx, const float* post, int32_t Q, int32_t n, int
 
 int gc_askadam_run(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int32_t n_vol) {
   if (!ctx) return GC_ERR_ARG;
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
   int rc = check_model(ctx, md);
   if (rc) return rc;
   if (!cfg || !vols || n_vol < 1 || n_vol > GC_MAX_VOL) return set_err(ctx, GC_ERR_ARG, "gc_askadam_run: bad argument");
@@ -524,6 +535,7 @@ static int mcmc_global_run(gc_ctx* ctx, const ModelOps* ops, const gc_model* md,
 
 int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc_io* io) {
   if (!ctx) return GC_ERR_ARG;
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
   int rc = check_model(ctx, md);
   if (rc) return rc;
   if (!cfg || !io || io->nv <= 0 || !io->y_dev || !io->x0_dev) return set_err(ctx, GC_ERR_ARG, "gc_mcmc_run: bad argument");

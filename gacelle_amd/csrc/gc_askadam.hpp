@@ -173,10 +173,10 @@ struct AskDriver {
     // ~2.5e8 voxel-iterations (at most 256 iterations) bound the replay that
     // resolves it.  Without stopping criteria the host has nothing to decide
     // between blocks: as many iterations per launch as the block partials
-    // (16 bytes per block and iteration) allow within 1 GiB.
+    // (4 bytes per block, iteration and sum) allow within 1 GiB.
     int64_t nblk_total = 0;
     for (int i = 0; i < n_vol; ++i) nblk_total += (vols[i].nv + kBlock - 1) / kBlock;
-    const int64_t row_bytes = 4 * int64_t(step_mode ? kStepSums : 4);
+    const int64_t row_bytes = 4 * int64_t(npv_count());
     const int k_mem =
         int(std::max<int64_t>(16, (int64_t(1) << 30) / (row_bytes * std::max<int64_t>(nblk_total, 1))));
     if (cfg.block_iters > 0) K = cfg.block_iters;
@@ -204,7 +204,7 @@ struct AskDriver {
       for (int s = 0; s < 3; ++s)
         for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.set[s][j].alloc(pbytes));
       for (int s = 0; s < 2; ++s) GC_HIP(ctx, R.zlast[s].alloc(pbytes));
-      GC_HIP(ctx, R.partials.alloc(size_t(K) * kStepSums * R.nblk * sizeof(float)));
+      GC_HIP(ctx, R.partials.alloc(size_t(K) * npv_count() * R.nblk * sizeof(float)));
       if (step_mode)
         for (int s = 0; s < 2; ++s)
           for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.pp[s][j].alloc(pbytes));

@@ -14,6 +14,8 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -50,34 +52,162 @@ inline int set_err(gc_ctx* ctx, int code, const char* fmt, ...) {
                      __FILE__, __LINE__);                                                      \
   } while (0)
 
-// RAII device buffer
+// ---- caching allocators ------------------------------------------------------
+// A fit allocates its working set (state sets, iteration partials, pinned
+// descriptors) on every call.  hipMalloc / hipHostMalloc of hundreds of MB
+// cost milliseconds, so released blocks are kept per device (device memory)
+// or per process (pinned host memory) and handed out again to requests of
+// the same size class.  Blocks are returned only when the stream that used
+// them has drained; cached blocks are freed when an allocation fails, when
+// the cache exceeds its cap and when the last context is destroyed.
+class BlockCache {
+ public:
+  using AllocFn = hipError_t (*)(void**, size_t);
+  using FreeFn = hipError_t (*)(void*);
+  BlockCache(AllocFn a, FreeFn f, size_t cap) : alloc_(a), free_(f), cap_(cap) {}
+  static size_t size_class(size_t bytes) {
+    const size_t g = bytes <= (size_t(1) << 20) ? 4096 : (size_t(2) << 20);
+    return (bytes + g - 1) / g * g;
+  }
+  hipError_t get(size_t bytes, void** out) {
+    const size_t sz = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = blocks_.lower_bound(sz);
+      if (it != blocks_.end() && it->first <= sz + sz / 4) {
+        *out = it->second.p;
+        cached_ -= it->first;
+        blocks_.erase(it);
+        return hipSuccess;
+      }
+    }
+    hipError_t e = alloc_(out, sz);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      e = alloc_(out, sz);
+    }
+    return e;
+  }
+  void put(void* p, size_t bytes) {
+    const size_t sz = size_class(bytes);
+    std::lock_guard<std::mutex> lk(mu_);
+    if (cached_ + sz > cap_) {
+      (void)free_(p);
+      return;
+    }
+    blocks_.emplace(sz, Block{p});
+    cached_ += sz;
+  }
+  void trim() {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto& b : blocks_) (void)free_(b.second.p);
+    blocks_.clear();
+    cached_ = 0;
+  }
+
+ private:
+  struct Block {
+    void* p;
+  };
+  AllocFn alloc_;
+  FreeFn free_;
+  size_t cap_;
+  size_t cached_ = 0;
+  std::mutex mu_;
+  std::multimap<size_t, Block> blocks_;
+};
+
+inline hipError_t dev_malloc(void** p, size_t n) { return hipMalloc(p, n); }
+inline hipError_t dev_free(void* p) { return hipFree(p); }
+inline hipError_t pinned_malloc(void** p, size_t n) { return hipHostMalloc(p, n, hipHostMallocDefault); }
+inline hipError_t pinned_free(void* p) { return hipHostFree(p); }
+
+// One cache per device (never destroyed: freeing at process exit would run
+// after the HIP runtime has shut down).
+inline BlockCache& device_cache(int dev) {
+  static std::mutex mu;
+  static std::map<int, BlockCache*>* caches = new std::map<int, BlockCache*>();
+  std::lock_guard<std::mutex> lk(mu);
+  BlockCache*& c = (*caches)[dev];
+  if (!c) c = new BlockCache(&dev_malloc, &dev_free, size_t(16) << 30);
+  return *c;
+}
+inline BlockCache& pinned_cache() {
+  static BlockCache* c = new BlockCache(&pinned_malloc, &pinned_free, size_t(256) << 20);
+  return *c;
+}
+inline void trim_caches() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) == hipSuccess)
+    for (int d = 0; d < n; ++d) device_cache(d).trim();
+  pinned_cache().trim();
+}
+
+// The stream the calling entry point issues its work on, for the buffers it
+// allocates (set by StreamScope; null stream when unset).
+inline hipStream_t& active_stream() {
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(active_stream()) { active_stream() = s; }
+  ~StreamScope() { active_stream() = prev; }
+};
+// Wait until work on `s` that may still use a block has finished.
+inline void drain(hipStream_t s) {
+  if (hipStreamQuery(s) != hipSuccess) (void)hipStreamSynchronize(s);
+  (void)hipGetLastError();
+}
+
+// RAII device buffer from the device's block cache
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  hipError_t alloc(size_t bytes) {
-    if (p) (void)hipFree(p);
+  ~DevBuf() { release(); }
+  void release() {
+    if (!p) return;
+    drain(stream);
+    device_cache(dev).put(p, n);
     p = nullptr;
+  }
+  hipError_t alloc(size_t bytes) {
+    release();
     n = bytes;
     if (bytes == 0) return hipSuccess;
-    return hipMalloc(&p, bytes);
+    (void)hipGetDevice(&dev);
+    stream = active_stream();
+    return device_cache(dev).get(bytes, &p);
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// RAII pinned host buffer from the process's block cache
 struct HostBuf {
   void* p = nullptr;
+  size_t n = 0;
+  hipStream_t stream = nullptr;
   HostBuf() = default;
   HostBuf(const HostBuf&) = delete;
   HostBuf& operator=(const HostBuf&) = delete;
-  ~HostBuf() { if (p) (void)hipHostFree(p); }
-  hipError_t alloc(size_t bytes) {
-    if (p) (void)hipHostFree(p);
+  ~HostBuf() { release(); }
+  void release() {
+    if (!p) return;
+    drain(stream);
+    pinned_cache().put(p, n);
     p = nullptr;
-    return hipHostMalloc(&p, std::max<size_t>(bytes, 64), hipHostMallocDefault);
+  }
+  hipError_t alloc(size_t bytes) {
+    release();
+    n = std::max<size_t>(bytes, 64);
+    stream = active_stream();
+    return pinned_cache().get(n, &p);
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };

@@ -35,7 +35,7 @@ GC_DEV void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, ui
 }
 
 // Native stream: counter = {iteration, chain_lo, chain_hi, purpose}, key = seed.
-enum : uint32_t { PURPOSE_NORMAL0 = 0, PURPOSE_ACCEPT = 15, PURPOSE_ZZ = 16, PURPOSE_INIT = 32 };
+enum : uint32_t { PURPOSE_NORMAL0 = 0, PURPOSE_ZZ = 16, PURPOSE_INIT = 32 };
 
 GC_DEV void draw4(uint64_t seed, uint32_t it, uint64_t chain, uint32_t purpose, uint32_t out[4]) {
   philox4x32_10(it, uint32_t(chain), uint32_t(chain >> 32), purpose, uint32_t(seed), uint32_t(seed >> 32), out);
@@ -44,13 +44,49 @@ GC_DEV void draw4(uint64_t seed, uint32_t it, uint64_t chain, uint32_t purpose, 
 // uint32 -> (0,1): ((x >> 8) + 0.5) * 2^-24, exact in float.
 GC_DEV float u01(uint32_t x) { return (float(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
 
-// Box-Muller pair from two uniforms.
+// Box-Muller pair from two uniforms: r = sqrt(-2 ln u1), angle 2 pi u2.
+// v_sin_f32 / v_cos_f32 take their argument in revolutions, so the angle
+// needs no range reduction.
+GC_DEV float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
+GC_DEV float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
 GC_DEV void box_muller(float u1, float u2, float& n0, float& n1) {
   const float r = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincosf(2.0f * kPiF * u2, &s, &c);
-  n0 = r * c;
-  n1 = r * s;
+  n0 = r * cos_rev(u2);
+  n1 = r * sin_rev(u2);
+}
+
+// Standard normal quantile sqrt(2) erfinv(2u - 1) for u in (0, 1), with the
+// single-precision erfinv of M. Giles, "Approximating the erfinv function"
+// (GPU Computing Gems Jade Edition, 2011); (1 - x)(1 + x) is formed as
+// 4 u (1 - u), exact near the tails.
+GC_DEV float norm_inv(float u) {
+  const float x = 2.f * u - 1.f;
+  float w = -logf(4.f * u * (1.f - u));
+  float p;
+  if (w < 5.f) {
+    w -= 2.5f;
+    p = 2.81022636e-08f;
+    p = 3.43273939e-07f + p * w;
+    p = -3.5233877e-06f + p * w;
+    p = -4.39150654e-06f + p * w;
+    p = 0.00021858087f + p * w;
+    p = -0.00125372503f + p * w;
+    p = -0.00417768164f + p * w;
+    p = 0.246640727f + p * w;
+    p = 1.50140941f + p * w;
+  } else {
+    w = sqrtf(w) - 3.f;
+    p = -0.000200214257f;
+    p = 0.000100950558f + p * w;
+    p = 0.00134934322f + p * w;
+    p = -0.00367342844f + p * w;
+    p = 0.00573950773f + p * w;
+    p = -0.0076224613f + p * w;
+    p = 0.00943887047f + p * w;
+    p = 1.00167406f + p * w;
+    p = 2.83297682f + p * w;
+  }
+  return 1.41421356237f * p * x;
 }
 
 // ---------------------------------------------------------------------------

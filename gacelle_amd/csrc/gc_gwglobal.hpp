@@ -113,11 +113,9 @@ GC_DEV void gwg_global_draws(const GwgParams& P, int r, int wk, int t, int gsub,
     u2 = P.g_uniforms[base + (rb ? hm : W) + j];
   } else {
     uint32_t o[4];
-    const uint64_t ch = gwg_global_chain(r, wk);
-    draw4(P.seed, uint32_t(t), ch, PURPOSE_ZZ + uint32_t(gsub), o);
+    draw4(P.seed, uint32_t(t), gwg_global_chain(r, wk), PURPOSE_ZZ + uint32_t(gsub), o);
     u1 = u01(o[0]);
-    draw4(P.seed, uint32_t(t), ch, PURPOSE_ACCEPT + 0x100u * uint32_t(gsub), o);
-    u2 = u01(o[0]);
+    u2 = u01(o[1]);
   }
 }
 
@@ -301,8 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_gwg_step(const gc_model* __restrict_
             uint32_t o[4];
             draw4(P.seed, uint32_t(S.k), chain, PURPOSE_ZZ + sub, o);
             u1 = u01(o[0]);
-            draw4(P.seed, uint32_t(S.k), chain, PURPOSE_ACCEPT + 0x100u * sub, o);
-            u2 = u01(o[0]);
+            u2 = u01(o[1]);
           }
         }
         const float zz = gwg_zz(u1, P.step_size);

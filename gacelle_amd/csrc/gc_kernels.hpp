@@ -986,22 +986,27 @@ __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp,
   for (int k = 1; k <= P.iteration; ++k) {
     const int64_t step = int64_t(r) * P.iteration + (k - 1);
     float n[Q];
+    float u_acc = 0.f;
     if (P.rng == GC_RNG_REPLAY) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) n[q] = P.normals[(step * nv + vv) * Q + q];
     } else {
+      // native stream: uniforms u[0..Q] from Philox groups of four; normals
+      // in Box-Muller pairs, an odd last one from the normal quantile, and
+      // the acceptance uniform u[Q]
+      constexpr int NU = Q + 1;
+      float uu[4 * ((NU + 3) / 4)];
 #pragma unroll
-      for (int g4 = 0; g4 < (Q + 3) / 4; ++g4) {
+      for (int g4 = 0; g4 < (NU + 3) / 4; ++g4) {
         uint32_t o[4];
         draw4(P.seed, uint32_t(k), chain, PURPOSE_NORMAL0 + g4, o);
-        float a0, a1, a2, a3;
-        box_muller(u01(o[0]), u01(o[1]), a0, a1);
-        if (4 * g4 + 2 < Q) box_muller(u01(o[2]), u01(o[3]), a2, a3);
-        if (4 * g4 + 0 < Q) n[4 * g4 + 0] = a0;
-        if (4 * g4 + 1 < Q) n[4 * g4 + 1] = a1;
-        if (4 * g4 + 2 < Q) n[4 * g4 + 2] = a2;
-        if (4 * g4 + 3 < Q) n[4 * g4 + 3] = a3;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) uu[4 * g4 + i] = u01(o[i]);
       }
+#pragma unroll
+      for (int q = 0; q + 1 < Q; q += 2) box_muller(uu[q], uu[q + 1], n[q], n[q + 1]);
+      if constexpr (Q % 2 == 1) n[Q - 1] = norm_inv(uu[Q - 1]);
+      u_acc = uu[Q];
     }
     float xp[Q];
     bool oob = false;
@@ -1012,14 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp,
       xp[q] = fminf(fmaxf(xp[q], lbq[q]), ubq[q]);         // :151
     }
     const float lpp = logp_gaussian(model_sos<Model, HAS_W>(md, xp, ex, yr, wr), xp[Q - 1], M);  // :158
-    float u;
-    if (P.rng == GC_RNG_REPLAY) {
-      u = P.uniforms[step * nv + vv];
-    } else {
-      uint32_t o[4];
-      draw4(P.seed, uint32_t(k), chain, PURPOSE_ACCEPT, o);
-      u = u01(o[0]);
-    }
+    const float u = (P.rng == GC_RNG_REPLAY) ? P.uniforms[step * nv + vv] : u_acc;
     const float ratio = min1(__expf(lpp - lp));  // :160
     const bool acc = (ratio > u) && !oob;        // :161-162
     if (P.trace != nullptr && active) P.trace[step * nv + v] = acc ? 1 : 0;
@@ -1170,11 +1168,11 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
           u1 = P.uniforms[(ub0 + j) * nv + vv];
           u2 = P.uniforms[(ub0 + ust + j) * nv + vv];
         } else {
+          // one draw: stretch and acceptance uniforms
           uint32_t o[4];
           draw4(P.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o);
           u1 = u01(o[0]);
-          draw4(P.seed, uint32_t(k), chain, PURPOSE_ACCEPT + 0x100u * sub, o);
-          u2 = u01(o[0]);
+          u2 = u01(o[1]);
         }
       }
       // stretch move (:299-304)

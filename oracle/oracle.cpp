@@ -60,7 +60,7 @@ static inline void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], u
 static inline double u01(uint32_t x) { return (double(x >> 8) + 0.5) * (1.0 / 16777216.0); }
 
 // Native stream layout.  Counter = {iteration, chain_lo, chain_hi, purpose}.
-enum : uint32_t { PURPOSE_NORMAL0 = 0, PURPOSE_ACCEPT = 15, PURPOSE_ZZ = 16, PURPOSE_INIT = 32 };
+enum : uint32_t { PURPOSE_NORMAL0 = 0, PURPOSE_ZZ = 16, PURPOSE_INIT = 32 };
 
 static inline void draw(uint64_t seed, uint32_t it, uint64_t chain, uint32_t purpose, uint32_t out[4]) {
   const uint32_t ctr[4] = {it, uint32_t(chain), uint32_t(chain >> 32), purpose};
@@ -77,6 +77,59 @@ static inline double native_normal(uint64_t seed, uint32_t it, uint64_t chain, u
   const double r = std::sqrt(-2.0 * std::log(u1));
   const double a = 2.0 * kPi * u2;
   return (q % 2 == 0) ? r * std::cos(a) : r * std::sin(a);
+}
+
+// Standard normal quantile sqrt(2) erfinv(2u - 1) with the single-precision
+// erfinv of M. Giles, "Approximating the erfinv function" (GPU Computing
+// Gems Jade Edition, 2011), evaluated in double with (1-x)(1+x) = 4u(1-u).
+static inline double norm_inv(double u) {
+  const double x = 2.0 * u - 1.0;
+  double w = -std::log(4.0 * u * (1.0 - u));
+  double p;
+  if (w < 5.0) {
+    w -= 2.5;
+    p = 2.81022636e-08;
+    p = 3.43273939e-07 + p * w;
+    p = -3.5233877e-06 + p * w;
+    p = -4.39150654e-06 + p * w;
+    p = 0.00021858087 + p * w;
+    p = -0.00125372503 + p * w;
+    p = -0.00417768164 + p * w;
+    p = 0.246640727 + p * w;
+    p = 1.50140941 + p * w;
+  } else {
+    w = std::sqrt(w) - 3.0;
+    p = -0.000200214257;
+    p = 0.000100950558 + p * w;
+    p = 0.00134934322 + p * w;
+    p = -0.00367342844 + p * w;
+    p = 0.00573950773 + p * w;
+    p = -0.0076224613 + p * w;
+    p = 0.00943887047 + p * w;
+    p = 1.00167406 + p * w;
+    p = 2.83297682 + p * w;
+  }
+  return std::sqrt(2.0) * p * x;
+}
+
+// The MH native stream at (iteration it, chain): uniforms u[0..Q] from
+// Philox groups of four (purposes PURPOSE_NORMAL0 + g); normals n[q] in
+// Box-Muller pairs (u[q], u[q+1]), an odd last normal from norm_inv(u[Q-1]);
+// the acceptance uniform is u[Q].
+static inline void native_mh_draws(uint64_t seed, uint32_t it, uint64_t chain, int Q, double* n, double* u_acc) {
+  double u[4 * ((GC_MAX_PAR + 1 + 3) / 4)];
+  for (int g = 0; g < (Q + 1 + 3) / 4; ++g) {
+    uint32_t o[4];
+    draw(seed, it, chain, PURPOSE_NORMAL0 + uint32_t(g), o);
+    for (int i = 0; i < 4; ++i) u[4 * g + i] = u01(o[i]);
+  }
+  for (int q = 0; q + 1 < Q; q += 2) {
+    const double r = std::sqrt(-2.0 * std::log(u[q]));
+    n[q] = r * std::cos(2.0 * kPi * u[q + 1]);
+    n[q + 1] = r * std::sin(2.0 * kPi * u[q + 1]);
+  }
+  if (Q % 2 == 1) n[Q - 1] = norm_inv(u[Q - 1]);
+  *u_acc = u[Q];
 }
 
 // Partner tables for the native GW stream (shared by all voxels, as the
@@ -580,19 +633,17 @@ static void mh_run(const McmcArgs& a, int nthreads) {
       for (int k = 1; k <= c.iteration; ++k) {
         const size_t step = size_t(r) * c.iteration + (k - 1);
         bool oob = false;
+        double nat[GC_MAX_PAR], u_nat = 0.0;
+        if (c.rng_mode != GC_RNG_REPLAY) native_mh_draws(c.seed, uint32_t(k), chain, Q, nat, &u_nat);
         for (int q = 0; q < Q; ++q) {
-          double n;
-          if (c.rng_mode == GC_RNG_REPLAY) n = a.normals[(step * nv + v) * Q + q];
-          else n = native_normal(c.seed, uint32_t(k), chain, PURPOSE_NORMAL0, q);
+          const double n = (c.rng_mode == GC_RNG_REPLAY) ? double(a.normals[(step * nv + v) * Q + q]) : nat[q];
           // xProposed = xCurr + xStepsize.*randn (:147)
           xp[q] = x[q] + S(c.x_step[q]) * S(n);
           oob = oob || (xp[q] < S(md.lb[q])) || (xp[q] > S(md.ub[q]));  // :149
           xp[q] = std::min(std::max(xp[q], S(md.lb[q])), S(md.ub[q]));  // :151
         }
         const S lpp = logp_gaussian<S>(sos_of<S>(md, xp, a.y, a.w, ex, nv, v), xp[c.noise_slot], M);  // :158
-        double u;
-        if (c.rng_mode == GC_RNG_REPLAY) u = a.uniforms[step * nv + v];
-        else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ACCEPT, o); u = u01(o[0]); }
+        const double u = (c.rng_mode == GC_RNG_REPLAY) ? double(a.uniforms[step * nv + v]) : u_nat;
         const S ratio = min1<S>(S(std::exp(lpp - lp)));  // :160
         const bool acc = (ratio > S(u)) && !oob;              // :161-162
         if (a.trace) a.trace[step * nv + v] = acc ? 1 : 0;
@@ -694,6 +745,7 @@ static void gw_run(const McmcArgs& a, int nthreads) {
             const int pw = rb ? part[(sub == 0 ? 0 : h) + j] : part[wk];
             double u1;
             const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v);
+            // native: one draw gives the stretch (o[0]) and acceptance (o[1]) uniforms
             if (c.rng_mode == GC_RNG_REPLAY) u1 = a.uniforms[(ubase + j) * nv + v];
             else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o); u1 = u01(o[0]); }
             // zz = ((a-1)*rand + 1).^2 / a (:299, :332)
@@ -716,7 +768,7 @@ static void gw_run(const McmcArgs& a, int nthreads) {
             const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v);
             double u2;
             if (c.rng_mode == GC_RNG_REPLAY) u2 = a.uniforms[(ubase + ustride + j) * nv + v];
-            else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ACCEPT + 0x100u * sub, o); u2 = u01(o[0]); }
+            else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o); u2 = u01(o[1]); }
             // min(zz.^(Nvar-1).*exp(logPProposed-logPCurr),1) > rand (:313-315)
             const S ratio = min1<S>(S(std::pow(zz[wk], S(Q - 1)) * std::exp(lpp[wk] - lp[wk])));
             const bool acc = ratio > S(u2) && !oob[wk];
@@ -883,8 +935,7 @@ static void gwg_run(const McmcArgs& a, int nthreads) {
             uint32_t o[4];
             draw(c.seed, uint32_t(t), global_chain(r, wk), PURPOSE_ZZ + sub, o);
             u1 = u01(o[0]);
-            draw(c.seed, uint32_t(t), global_chain(r, wk), PURPOSE_ACCEPT + 0x100u * sub, o);
-            u2 = u01(o[0]);
+            u2 = u01(o[1]);
           }
           uacc[j] = u2;
           const S tt = (aa - S(1)) * S(u1) + S(1);
@@ -975,9 +1026,8 @@ static void gwg_run(const McmcArgs& a, int nthreads) {
             if (!native) u2 = a.uniforms[(ub0 + ust + j) * nv + v];
             else {
               uint32_t o[4];
-              draw(c.seed, uint32_t(k), (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v),
-                   PURPOSE_ACCEPT + 0x100u * sub, o);
-              u2 = u01(o[0]);
+              draw(c.seed, uint32_t(k), (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v), PURPOSE_ZZ + sub, o);
+              u2 = u01(o[1]);
             }
             S zpow = S(1);
             for (int i = 0; i < NND - 1; ++i) zpow *= zz[wv];
@@ -1103,6 +1153,15 @@ double orc_native_normal(uint64_t seed, uint32_t it, uint64_t chain, uint32_t ba
 }
 
 double orc_logp_gaussian(double sos, double noise, int32_t nm) { return orc::logp_gaussian<double>(sos, noise, nm); }
+
+double orc_norm_inv(double u) { return orc::norm_inv(u); }
+
+// MH native draws: n[Q] normals and the acceptance uniform
+int orc_native_mh_draws(uint64_t seed, uint32_t it, uint64_t chain, int32_t Q, double* n, double* u_acc) {
+  if (Q < 1 || Q > GC_MAX_PAR) return GC_ERR_ARG;
+  orc::native_mh_draws(seed, it, chain, Q, n, u_acc);
+  return 0;
+}
 
 // s[M x nv] = FWD(theta[P x nv]); jac (optional) [M x P x nv] = ds/dtheta.
 int orc_fwd(const gc_model* md, int64_t nv, const double* theta, const double* extra, double* s, double* jac) {

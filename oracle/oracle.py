@@ -61,6 +61,9 @@ def lib():
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_int32, c_int32]
         L.orc_native_partners_global.argtypes = [c_uint64, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32)]
+        L.orc_norm_inv.argtypes = [c_double]
+        L.orc_norm_inv.restype = c_double
+        L.orc_native_mh_draws.argtypes = [c_uint64, c_uint32, c_uint64, c_int32, POINTER(c_double), POINTER(c_double)]
         L.orc_mcmc_global.argtypes = [c_void_p, c_void_p, c_int64] + [c_void_p] * 20 + [c_int32, c_int32]
         _lib = L
     return _lib
@@ -96,6 +99,18 @@ def native_partners_global(seed, rep, it, W, mode):
     out = (c_int32 * W)()
     lib().orc_native_partners_global(seed, rep, it, W, mode, out)
     return np.array(out, dtype=np.int32)
+
+
+def norm_inv(u):
+    return lib().orc_norm_inv(float(u))
+
+
+def native_mh_draws(seed, it, chain, Q):
+    """The MH native stream at (iteration, chain): (normals [Q], acceptance uniform)."""
+    n = (c_double * Q)()
+    u = c_double()
+    assert lib().orc_native_mh_draws(seed, it, chain, Q, n, ctypes.byref(u)) == 0
+    return np.array(n), u.value
 
 
 def native_normal(seed, it, chain, base, q):

@@ -32,6 +32,32 @@ def test_native_normals_are_standard_normal():
     assert stats.kstest(z, "norm").pvalue > 1e-3
 
 
+def test_norm_inv_matches_the_normal_quantile():
+    """Giles' single-precision erfinv behind the odd normal of the MH stream:
+    sqrt(2) erfinv(2u - 1) against scipy's ndtri, body and tails."""
+    from scipy import special
+
+    u = np.concatenate([np.linspace(1e-7, 1 - 1e-7, 20001), np.geomspace(3e-8, 1e-2, 400)])
+    u = np.concatenate([u, 1 - u[-400:]])
+    got = np.array([orc.norm_inv(x) for x in u])
+    ref = special.ndtri(u)
+    assert np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))) < 2e-6
+
+
+@pytest.mark.parametrize("Q", [3, 4, 5])
+def test_native_mh_draws_are_independent_normals_and_uniform(Q):
+    from scipy import stats
+
+    n, u = zip(*[orc.native_mh_draws(5, it, c, Q) for it in range(1, 400) for c in range(30)])
+    n = np.array(n)
+    u = np.array(u)
+    for q in range(Q):
+        assert stats.kstest(n[:, q], "norm").pvalue > 1e-3, q
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    c = np.corrcoef(np.column_stack([n, u]).T)
+    assert np.max(np.abs(c - np.eye(Q + 1))) < 0.05
+
+
 def test_native_partners_are_derangements():
     for it in range(1, 50):
         p = orc.native_partners(11, 0, it, 16, _abi.GC_ENSEMBLE_SIMULTANEOUS)

@@ -222,6 +222,8 @@ SIGNATURES = {
                        c_int64, c_void_p, c_void_p], c_int32),
     "gc_posterior_stats": ([c_void_p, c_void_p, c_int32, c_int32, c_int64, POINTER(c_float), POINTER(c_float), c_int32,
                             c_void_p, c_void_p, c_void_p], c_int32),
+    "gc_dict_search": ([c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_int64, POINTER(c_float), c_void_p,
+                        c_void_p], c_int32),
     "gc_askadam_run": ([c_void_p, POINTER(gc_model), POINTER(gc_askadam_cfg), POINTER(gc_volume), c_int32],
                        c_int32),
     "gc_mcmc_run": ([c_void_p, POINTER(gc_model), POINTER(gc_mcmc_cfg), POINTER(gc_mcmc_io)], c_int32),

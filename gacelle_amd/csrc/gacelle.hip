@@ -4,6 +4,7 @@
 // launches are compiled per model variant in ops_*.hip.
 #include <atomic>
 
+#include "gc_dict.hpp"
 #include "gc_host.hpp"
 
 using namespace gc;
@@ -264,6 +265,13 @@ int gc_posterior_stats(gc_ctx* ctx, const float* post, int32_t Q, int32_t n, int
   GC_HIP(ctx, hipGetLastError());
   GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // edges freed on return
   return GC_OK;
+}
+
+int gc_dict_search(gc_ctx* ctx, const float* y, int64_t nv, int32_t m, const float* dict, int64_t nd, const float* w,
+                   int32_t* best, double* sse) {
+  if (!ctx) return GC_ERR_ARG;
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
+  return dict_search(ctx, y, nv, m, dict, nd, w, best, sse);
 }
 
 int gc_askadam_run(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int32_t n_vol) {

@@ -16,7 +16,7 @@ import ctypes
 
 import numpy as np
 
-from . import _abi, runtime, utils
+from . import _abi, dwi, runtime, utils
 from ._abi import GacelleError
 
 
@@ -144,6 +144,46 @@ def _fill_table(md, values, offset=0):
     for k, v in enumerate(values):
         md.table[offset + k] = float(v)
     return offset + len(values)
+
+
+def despot1(tr, fa, img, mask, b1):
+    """DESPOT1 T1 / M0 by linear regression (MCRMWI/private/despot1.m:28-167,
+    Deoni et al. MRM 2003): S/sin(a) = E1 S/tan(a) + M0 (1 - E1) with
+    a = FA b1, T1 = -TR / log(E1).  img [x,y,z,nfa] (magnitude), fa in
+    degrees; returns (t1, m0, mask_fitted) on the grid, zero outside the mask
+    and where the fit is not physical (T1 <= 0 or not finite, M0 < 0 or not
+    finite).  The reference keeps NaN there (NaN times a zero mask); every
+    caller treats both alike (clamped to a bound, or excluded by
+    mask_fitted)."""
+    img = np.abs(np.asarray(img, np.float64))
+    dims = img.shape[:3]
+    mask = np.asarray(mask) > 0
+    b1 = np.ones(dims) if b1 is None else np.broadcast_to(np.asarray(b1, np.float64).reshape(
+        dims if np.size(b1) > 1 else (1, 1, 1)), dims)
+    t1 = np.zeros(dims)
+    m0 = np.zeros(dims)
+    S = img[mask]                                  # [n, nfa]
+    alpha = np.deg2rad(np.asarray(fa, np.float64).ravel())[None, :] * b1[mask][:, None]
+    ok = S.sum(axis=1) != 0                        # zero signal: T1 = M0 = 0
+    tr = float(tr)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        yv = S / np.sin(alpha)
+        xv = S / np.tan(alpha)
+        # least squares of yv = b0 xv + b1 over the flip angles (x \ y)
+        xm, ym = xv.mean(axis=1), yv.mean(axis=1)
+        sxx = ((xv - xm[:, None]) ** 2).sum(axis=1)
+        sxy = ((xv - xm[:, None]) * (yv - ym[:, None])).sum(axis=1)
+        e1 = sxy / sxx
+        icpt = ym - e1 * xm
+        t1v = -tr / np.log(e1)
+        m0v = icpt / (1 - np.exp(-tr / t1v))
+    t1v = np.where(ok, t1v, 0.0)
+    m0v = np.where(ok, m0v, 0.0)
+    t1[mask], m0[mask] = t1v, m0v
+    fitted = (t1 > 0) & np.isfinite(t1) & (m0 >= 0) & np.isfinite(m0) & mask
+    t1 = np.where(fitted, t1, 0.0)
+    m0 = np.where(fitted, m0, 0.0)
+    return t1, m0, fitted
 
 
 # ===========================================================================
@@ -324,11 +364,106 @@ class gpuJointR1R2starMapping(_ModelBase):
     def check_set_default(fitting):
         return gpuR2starMapping.check_set_default(fitting)
 
-    def estimate_prior(self, y, mask, fitting):
-        return utils.initialise_x0(utils.dims3(mask), self.modelParams, self.startPoint)
+    # -- estimate glue (R1R2s/gpuJointR1R2starMapping.m:105-192, :290-371, :427-491)
+    def _b1(self, extraData, dims):
+        ex = extraData or {}
+        if ex.get("b1") is None:
+            return np.ones(dims)
+        b1 = np.asarray(ex["b1"], np.float64)
+        return np.full(dims, float(b1)) if b1.size == 1 else b1.reshape(dims, order="F")
+
+    def validate_input(self, data, mask, extraData):
+        """R1R2s/gpuJointR1R2starMapping.m:427-471: TE / FA counts against the
+        4th / 5th data dimensions, mask dimensions (default: voxels whose
+        maximum |signal| over echoes and flip angles exceeds 5% of the
+        image maximum), a b1 map in extraData (ones when extraData is empty)."""
+        data = np.asarray(data)
+        shp = tuple(data.shape) + (1,) * max(0, 5 - data.ndim)
+        if self.te.size != shp[3]:
+            raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The size of TE does not match with the 4th dimension of the image.")
+        if self.fa.size != int(np.prod(shp[4:])):
+            raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH,
+                               "The size of flip angle does not match with the 5th dimension of the image.")
+        dims = shp[:3]
+        if mask is not None and np.size(mask) > 0:
+            if utils.dims3(mask) != dims:
+                raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The dimension of the mask does not match the inpt image.")
+            mask = np.asarray(mask).reshape(dims, order="F")
+        else:
+            a = np.abs(data).reshape(dims + (-1,), order="F")
+            mask = a.max(axis=3) / a.max() > 0.05
+        if extraData:
+            if "b1" not in extraData:
+                raise GacelleError(_abi.GC_ERR_ARG, "Cannot find B1 map in extradata structure. Please specify B1 map "
+                                   "to 'extradata.b1 = b1map' or set extradata to empty: 'extradata = []'")
+            if np.size(extraData["b1"]) != int(np.prod(dims)) or utils.dims3(extraData["b1"]) != dims:
+                raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The dimension of the B1 map does not match the inpt image.")
+        else:
+            extraData = {"b1": np.ones(dims)}
+        return mask, extraData
+
+    def prepare_data(self, img, mask, extraData):
+        """R1R2s/gpuJointR1R2starMapping.m:474-491: validate, then divide by
+        the 98th percentile of the DESPOT1 M0 of the first echo over the
+        fitted voxels; NaN / Inf voxels leave the mask."""
+        mask = None if mask is None or np.size(mask) == 0 else np.asarray(mask) > 0
+        mask, ex = self.validate_input(img, mask, extraData)
+        img = np.asarray(img)
+        dims = utils.dims3(img)
+        mag1 = np.abs(img.reshape(dims + (self.te.size, -1), order="F")[:, :, :, 0, :]).astype(np.float64)
+        _, m0, fitted = despot1(np.float64(self.tr), self.fa.astype(np.float64), mag1, mask, self._b1(ex, dims))
+        scale = utils.prctile(m0[fitted], 98)
+        img = img / scale
+        img, mask = utils.remove_img_naninf(img, mask)
+        return img, mask, scale
+
+    def estimate_prior(self, data, mask, extraData):
+        """R1R2s/gpuJointR1R2starMapping.m:330-371: R2* from the closed form
+        on the flip-angle mean, M0 and R1 from DESPOT1 of the first echo, each
+        with invalid values set to the lower bound and clamped to the bounds;
+        where R1 > R2*, R2* = R1 / 2.  The reference reads extraData.b1
+        unconditionally (an error without extraData); ones are used here."""
+        data = np.asarray(data)
+        dims = utils.dims3(data)
+        mag = np.abs(data.reshape(dims + (self.te.size, -1), order="F"))
+        pars0 = utils.initialise_x0(dims, self.modelParams, self.startPoint)
+        lb, ub = self.lb, self.ub
+
+        def clamp(v, k):
+            v = np.where(np.isfinite(v), v, lb[k])
+            return np.minimum(np.maximum(v, lb[k]), ub[k])
+
+        r2s0, _ = gpuR2starMapping.R2star_trapezoidal(mag.mean(axis=4), self.te)
+        r2s0 = clamp(r2s0, 2)
+        t10, m00, _ = despot1(np.float64(self.tr), self.fa.astype(np.float64), mag[:, :, :, 0, :], mask,
+                              self._b1(extraData, dims))
+        with np.errstate(divide="ignore"):
+            r10 = 1.0 / t10
+        m00 = clamp(m00, 0)
+        r10 = clamp(r10, 1)
+        sel = r10 > r2s0
+        r2s0[sel] = r10[sel] / 2
+        pars0[self.modelParams[0]] = m00.astype(np.float32)
+        pars0[self.modelParams[1]] = r10.astype(np.float32)
+        pars0[self.modelParams[2]] = r2s0.astype(np.float32)
+        return pars0
+
+    def determine_x0(self, y, mask, extraData, fitting):
+        """R1R2s/gpuJointR1R2starMapping.m:293-327.  A numeric fitting.start
+        is reported but the fixed start points are used, as in the reference."""
+        start = fitting.get("start", "prior")
+        if isinstance(start, str) and start.lower() == "prior":
+            x0 = self.estimate_prior(y, mask, extraData)
+        else:
+            x0 = utils.initialise_x0(utils.dims3(mask), self.modelParams, self.startPoint)
+        from .askadam import askadam as _ask
+
+        return _ask.set_boundary(x0, fitting["ub"], fitting["lb"])
 
     @staticmethod
     def compute_optimisation_weights(data, fitting):
+        """R1R2s/gpuJointR1R2starMapping.m:598-625 ('1stecho' divides by the
+        first echo of each flip angle)."""
         return gpuR2starMapping.compute_optimisation_weights(data, fitting)
 
     def fit(self, data, mask=None, fitting=None, extraData=None):
@@ -348,9 +483,27 @@ class gpuJointR1R2starMapping(_ModelBase):
             fitting["ub"] = list(self.ub)
         if len(np.atleast_1d(fitting["lb"])) == 0:
             fitting["lb"] = list(self.lb)
-        pars0 = self.determine_x0(data, mask, fitting)
+        pars0 = self.determine_x0(data, mask, extraData, fitting)
         w = self.compute_optimisation_weights(data, fitting)
         return self._solve(data, mask, w, pars0, fitting, extraData, fitting["solver"])
+
+    def estimate(self, data, mask=None, extraData=None, fitting=None):
+        """R1R2s/gpuJointR1R2starMapping.m:105-192 (one segment: a whole
+        volume fits in one launch): prepare_data, fit, then M0 back to the
+        data scale."""
+        fitting = self.check_set_default(fitting)
+        data, mask, scale = self.prepare_data(data, mask, extraData)
+        out = self.fit(np.asarray(data, np.float32), mask, fitting, extraData)
+        out["mask"] = mask
+        if str(fitting["solver"]).lower() == "mcmc":
+            for k in fitting.get("metric", ["mean", "std"]):
+                if k in out and "M0" in out[k]:
+                    out[k]["M0"] = out[k]["M0"] * scale
+            out["posterior"]["M0"] = out["posterior"]["M0"] * scale
+        else:
+            out["min"]["M0"] = out["min"]["M0"] * scale
+            out["final"]["M0"] = out["final"]["M0"] * scale
+        return out
 
 
 # ===========================================================================
@@ -387,16 +540,21 @@ class gpuNEXI(_ModelBase):
     model_id = _abi.GC_MODEL_NEXI
     epsilon = 1e-8
 
+    thres_similarity = 0.1
+    thres_impossible = 0.1
+    thres_bkg = 0.01
+    seed = 48463
+    # dictionary sampling intervals of the likelihood start (NEXI/gpuNEXI.m:763-768):
+    # fa, Da, De, exchange time (1 - fa) / ra, p2
+    dict_intervals = ((0.01, 0.99), (1.5, 3.0), (0.5, 1.5), (1.0, 100.0), (0.01, 0.99))
+
     def __init__(self, b, Delta, Nav=None, quad="trapz14", rician=False, re_guard=None):
-        b = np.asarray(b, dtype=np.float64).ravel()
-        Delta = np.asarray(Delta, dtype=np.float64).ravel()
-        # DWIutility.unique_shell_keepb0(..., discardB0=true): drop b=0, keep
-        # one entry per (b, Delta) shell in sorted order
-        keep = b > 0
-        shells = sorted(set(zip(np.round(b[keep], 6), np.round(Delta[keep], 6))), key=lambda t: (t[1], t[0]))
-        self.b = np.array([s[0] for s in shells], dtype=np.float32)
-        self.Delta = np.array([s[1] for s in shells], dtype=np.float32)
-        self.Nav = Nav
+        # DWIutility.unique_shell_keepb0(b, [], Delta, [], true) (NEXI/gpuNEXI.m:62-72):
+        # drop b=0, one entry per (Delta, b) shell, Delta then b ascending
+        bs, _, ds, _ = dwi.unique_shell_keepb0(b, None, Delta, None, True)
+        self.b = bs.astype(np.float32)
+        self.Delta = ds.astype(np.float32)
+        self.Nav = np.ones(self.b.size) if Nav is None else np.asarray(Nav, np.float64).ravel()
         self.quad = quad
         self.rician = bool(rician)
         self.re_guard = self.rician if re_guard is None else bool(re_guard)
@@ -483,12 +641,233 @@ class gpuNEXI(_ModelBase):
             f2["dataType"] = "volumetric"
         return f2
 
-    def fit(self, dwi, mask=None, fitting=None, extraData=None):
-        """NEXI/gpuNEXI.m:290-450 (start 'default' or user values; the
-        dictionary-likelihood start is not part of this path)."""
-        dwi = np.asarray(dwi)
-        dims = utils.dims3(dwi)
-        if mask is None:
+    # -- estimate glue (NEXI/gpuNEXI.m:148-288, :455-849; NEXI/gpuNEXIrice.m:104-448)
+    def compute_optimisation_weights(self, mask, lossFunction, lmax):
+        """NEXI/gpuNEXI.m:455-477: weight Nav(kb) / (2l + 1) per shell and
+        degree, square-rooted for the L1 loss, divided by the maximum."""
+        dims = utils.dims3(mask)
+        nb = self.b.size
+        if self.Nav.size != nb:
+            raise GacelleError(_abi.GC_ERR_ARG, f"Nav has {self.Nav.size} entries for {nb} shells")
+        ls = list(range(0, int(lmax) + 1, 2))
+        w = np.zeros(dims + (nb * len(ls),), np.float32)
+        for kl, l in enumerate(ls):
+            for kb in range(nb):
+                w[..., kl * nb + kb] = self.Nav[kb] / (2 * l + 1)
+        if str(lossFunction).lower() == "l1":
+            w = np.sqrt(w)
+        return (w / w.max()).astype(np.float32)
+
+    @staticmethod
+    def _zcorr(tmpl, d2):
+        """Pearson correlation of the z-scored template with each z-scored
+        voxel row of d2 [nv, M] (MATLAB corr; NaN propagates)."""
+        zt = tmpl - tmpl.mean()
+        zv = d2 - d2.mean(axis=1, keepdims=True)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return (zv @ zt) / np.sqrt((zv * zv).sum(axis=1) * (zt @ zt))
+
+    def prepare_dwi_data(self, data, mask, extradata, lmax, fitting):
+        """NEXI/gpuNEXI.m:480-636 (volumetric data): the rotationally
+        invariant signal when the input holds raw diffusion volumes
+        (extradata bval, bvec, BDELTA); voxels leave the mask when a
+        normalised Sl0 exceeds 1 + thres_impossible, the lowest shell is
+        below thres_bkg, the Sl0 curve correlates below thres_similarity
+        with the median curve of the valid voxels, or a value is NaN/Inf."""
+        if str(fitting.get("dataType", "volumetric")).lower() == "surface":
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "surface data (surface TV) are not part of this library")
+        data = np.asarray(data, np.float64)
+        dims = utils.dims3(data)
+        nb = self.b.size
+        nsl = int(lmax) // 2 + 1
+        if data.shape[3] / nsl > nb:
+            ex = extradata or {}
+            data = dwi.compute_rotationally_invariant_signal(data, ex["bval"], ex["bvec"], None, ex.get("BDELTA"),
+                                                             None, lmax)[0]
+        elif data.shape[3] < nb * nsl:
+            raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "Input has %d volumes but model expects %d. Check lmax or "
+                               "input data." % (data.shape[3], nb * nsl))
+        mask = np.asarray(mask).reshape(dims, order="F") > 0
+        sl0 = data[..., :nb]
+        with np.errstate(invalid="ignore"):
+            valid = ~np.any(np.abs(sl0) > 1 + self.thres_impossible, axis=3)
+            valid &= ~(sl0[..., 0] < self.thres_bkg)
+        vox = utils.find_mask(valid)
+        if vox.size > 0:
+            d2 = sl0.reshape(-1, nb, order="F")[vox]                       # [nv, nb]
+            tmpl = np.nanmedian(d2, axis=0)
+            tmpl = (tmpl - np.nanmean(tmpl)) / np.nanstd(tmpl, ddof=1)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                sd = d2.std(axis=1, ddof=1)
+                r = np.where(sd < np.finfo(np.float64).eps, 0.0, self._zcorr(tmpl, (d2 - d2.mean(axis=1, keepdims=True))
+                                                                            / sd[:, None]))
+                rc = np.zeros(int(np.prod(dims)))
+                rc[vox] = r
+                valid &= ~(rc.reshape(dims, order="F") < self.thres_similarity)
+        data, mask_naninf = utils.remove_img_naninf(data, mask)
+        valid &= mask_naninf
+        if mask.sum() - valid.sum() > 0:
+            mask = valid
+        return data, mask, extradata
+
+    def prepare_dwi_data_rice(self, data, mask, extradata, lmax):
+        """NEXI/gpuNEXIrice.m:338-412: sigma divided by the mean b=0 volume
+        (floored at epsilon); the b0-normalised invariants (get_Sl_all) of
+        raw diffusion volumes; voxels leave the mask when a value exceeds
+        1.01, is NaN/Inf, sigma > 1 (SNR below 1) or the curve correlates
+        below 0.1 with the mean curve of the valid voxels."""
+        ex = dict(extradata or {})
+        data = np.asarray(data, np.float64)
+        dims = utils.dims3(data)
+        bval = np.asarray(ex["bval"], np.float64).ravel()
+        b0 = bval == 0
+        dwi_b0 = data[..., b0].mean(axis=3) if np.any(b0) else np.ones(dims)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sig = np.asarray(ex["sigma"], np.float64) / dwi_b0
+        sig, _ = utils.set_nan_inf_zero(sig)
+        ex["sigma"] = np.maximum(sig, dwi.EPSILON)
+        nb = self.b.size
+        if data.shape[3] / (int(lmax) // 2 + 1) > nb:
+            ld = np.asarray(ex["ldelta"], np.float64).ravel()
+            if ld.size == 1:
+                ld = np.full(bval.size, ld[0])
+            data = dwi.get_Sl_all(data, bval, ex["bvec"], ld, ex["BDELTA"], lmax)[0]
+        elif data.shape[3] < nb:
+            raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "There are more b-shells in the class object than available "
+                               "in the input data. Please check your input data.")
+        data, bad = utils.set_nan_inf_zero(data)
+        bad = bad.any(axis=3)
+        mask = np.asarray(mask).reshape(dims, order="F") > 0
+        valid = mask & (data.max(axis=3) <= 1.01) & ~bad
+        valid &= ~(np.broadcast_to(ex["sigma"], dims) > 1)
+        vox = utils.find_mask(valid)
+        d2 = data.reshape(-1, data.shape[3], order="F")[vox]
+        tmpl = np.nanmean(d2, axis=0)
+        tmpl = (tmpl - tmpl.mean()) / tmpl.std(ddof=1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            zv = (d2 - d2.mean(axis=1, keepdims=True)) / d2.std(axis=1, ddof=1)[:, None]
+            rc = np.zeros(int(np.prod(dims)))
+            rc[vox] = self._zcorr(tmpl, zv)
+            valid &= ~(rc.reshape(dims, order="F") < 0.1)
+        if valid.sum() != mask.sum():
+            mask = valid
+        return data, mask, ex
+
+    # -- likelihood start (NEXI/gpuNEXI.m:675-849, NEXI/gpuNEXIrice.m:449-625)
+    def traindata(self, N_samples=10000, lmax=0, intervals=None, sigma=None):
+        """Dictionary of the likelihood start (NEXI/gpuNEXI.m:761-820):
+        batches of 1000 uniform draws of the five sampling intervals, the
+        exchange time turned into ra = (1 - fa) / tex; signals of the
+        trapezoid-14 model (Sl0, and Sl2 when lmax = 2) computed by the
+        library's forward model, Rician first moment at sigma when given
+        (NEXI/gpuNEXIrice.m:466).  The draws are MATLAB's rand after
+        rng(seed, 'twister'): MT19937 seeded by init_genrand, 53-bit doubles
+        (numpy's RandomState produces the same stream), rand(5, 1000)
+        filled column-major.  Returns (x_train [P, N], S_train [M, N])."""
+        iv = np.asarray(self.dict_intervals if intervals is None else intervals, np.float64)
+        reps = int(np.ceil(N_samples / 1000))
+        u = np.random.RandomState(self.seed).random_sample(iv.shape[0] * 1000 * reps)
+        u = u.reshape(reps * 1000, iv.shape[0]).T
+        pars = iv[:, :1] + np.diff(iv, axis=1) * u
+        pars[3] = 1.0 / pars[3] * (1 - pars[0])
+        if int(lmax) == 0:
+            pars = pars[:4]
+        names = ["fa", "Da", "De", "ra", "p2"][:pars.shape[0]]
+        m = gpuNEXI(self.b, self.Delta, self.Nav, quad="trapz14", rician=sigma is not None, re_guard=False)
+        extra = {"sigma": float(sigma)} if sigma is not None else None
+        S = m.FWD({n: pars[k].astype(np.float32) for k, n in enumerate(names)}, extra)
+        return pars, S
+
+    def likelihood(self, S0, x_train, S_train, lmax=0):
+        """Best dictionary entry per voxel (NEXI/gpuNEXI.m:822-849): the
+        library's gc_dict_search with weights kron(Nav, 1 / (2l + 1)).
+        S0 [M', nv] (the first nb (lmax/2 + 1) rows are used); returns
+        pars_best [P, nv] (zeros where no entry scores)."""
+        nb = self.b.size
+        M = nb * (int(lmax) // 2 + 1)
+        wt = np.kron(self.Nav.reshape(-1, 1), 1.0 / (2 * np.arange(0, int(lmax) + 1, 2) + 1)).ravel(order="F")
+        y = np.ascontiguousarray(np.asarray(S0)[:M], np.float32)
+        nv = y.shape[1]
+        ctx = runtime.context()
+        y_d = ctx.to_device(y)
+        d_d = ctx.to_device(np.asarray(S_train, np.float32)[:M])
+        best_d = ctx.empty((nv,), np.int32)
+        w = None if np.all(wt == 1) else (ctypes.c_float * M)(*wt.astype(np.float32))
+        ctx.check(ctx.lib.gc_dict_search(ctx.handle, runtime.ptr(y_d), nv, M, runtime.ptr(d_d), d_d.shape[1], w,
+                                         runtime.ptr(best_d), None))
+        best = runtime.to_host(best_d).astype(np.int64)
+        out = np.zeros((x_train.shape[0], nv))
+        ok = best >= 0
+        out[:, ok] = x_train[:, best[ok]]
+        return out
+
+    def estimate_prior(self, dwi_, mask, Nsample=None, lmax=0, sigma=None):
+        """NEXI/gpuNEXI.m:675-759 (gpuNEXIrice.m:449-520 with sigma): best
+        dictionary entry per masked voxel; where the low-b diffusivity D0
+        (least squares of -log Sl0 on the shells with b <= max(min b, 1.1))
+        exceeds 1.5 (CSF), the parameters are multiplied by
+        (0.01, 1, 1, 0.01, 0.01)."""
+        data = np.asarray(dwi_)
+        dims = utils.dims3(data)
+        x_train, S_train = self.traindata(10000 if Nsample is None else Nsample, lmax, sigma=sigma)
+        nb = self.b.size
+        flat = data.reshape(int(np.prod(dims)), -1, order="F").astype(np.float64)
+        vox = utils.find_mask(mask)
+        P = x_train.shape[0]
+        pars = np.zeros((int(np.prod(dims)), P))
+        if vox.size:
+            pars[vox] = self.likelihood(flat[vox].T, x_train, S_train, lmax).T
+        bthr = max(float(self.b.min()), 1.1)
+        idx = self.b <= bthr
+        bb = self.b[idx].astype(np.float64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            d0 = (-np.log(np.abs(flat[:, :nb][:, idx]))) @ bb / (bb @ bb)
+        d0, _ = utils.set_nan_inf_zero(d0)
+        csf = d0 > 1.5
+        pars[csf] *= np.array([0.01, 1, 1, 0.01, 0.01])[:P]
+        pars0 = {}
+        for k in range(P):
+            pars0[self.modelParams[k]] = pars[:, k].reshape(dims, order="F").astype(np.float32)
+        if self.modelParams[-1] == "noise":
+            pars0["noise"] = np.full(dims, self.startPoint[-1], np.float32)
+        return pars0
+
+    def determine_x0(self, y, mask, fitting, extradata=None):
+        """NEXI/gpuNEXI.m:638-673 (gpuNEXIrice.m:414-447): 'likelihood' (the
+        dictionary start; the Rician model at the median normalised sigma
+        of the mask), 'default' (fixed start points).  A numeric
+        fitting.start is reported but the fixed start points are used, as in
+        the reference."""
+        start = fitting.get("start", "default")
+        dims = utils.dims3(mask)
+        if isinstance(start, str) and start.lower() == "likelihood":
+            if self.rician:
+                sig = np.broadcast_to(np.asarray((extradata or {})["sigma"], np.float64), dims)
+                x0 = self.estimate_prior(y, mask, None, 0, float(np.median(sig[np.asarray(mask) > 0])))
+            else:
+                x0 = self.estimate_prior(y, mask, None, int(fitting.get("lmax", 0)))
+        elif isinstance(start, str) and start.lower() != "default":
+            raise GacelleError(_abi.GC_ERR_ARG, f"fitting.start '{start}' is not 'likelihood' or 'default'")
+        else:
+            x0 = utils.initialise_x0(dims, self.modelParams, self.startPoint)
+        from .askadam import askadam as _ask
+
+        return _ask.set_boundary(x0, fitting["ub"], fitting["lb"])
+
+    def fit(self, dwi_, mask=None, fitting=None, *args, pars0=None, extraData=None):
+        """NEXI/gpuNEXI.m:290-450: fit(dwi, mask, fitting, pars0, extradata);
+        the Rician model keeps gpuNEXIrice's order fit(dwi, mask, fitting,
+        extraData, pars0) (NEXI/gpuNEXIrice.m:203-336).  The 'prior'
+        regulariser and surface data are not part of this library."""
+        if args:
+            first, rest = args[0], (args[1] if len(args) > 1 else None)
+            if self.rician:
+                extraData, pars0 = first if extraData is None else extraData, rest if pars0 is None else pars0
+            else:
+                pars0, extraData = first if pars0 is None else pars0, rest if extraData is None else extraData
+        dwi_ = np.asarray(dwi_)
+        dims = utils.dims3(dwi_)
+        if mask is None or np.size(mask) == 0:
             mask = np.ones(dims, bool)
         fitting = self.check_set_default(fitting)
         self.updateProperty(fitting)
@@ -497,8 +876,32 @@ class gpuNEXI(_ModelBase):
             fitting["ub"] = list(self.ub)
         if len(np.atleast_1d(fitting["lb"])) == 0:
             fitting["lb"] = list(self.lb)
-        pars0 = self.determine_x0(dwi, mask, fitting)
-        return self._solve(dwi, mask, None, pars0, fitting, extraData)
+        lam = np.atleast_1d(fitting.get("lambda", [0]))
+        if str(fitting.get("regularisationType", "TV")).lower() == "prior" and lam.size and float(lam[0]) > 0:
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "the 'prior' regulariser is not part of this library")
+        if str(fitting.get("dataType", "volumetric")).lower() == "surface":
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "surface data (surface TV) are not part of this library")
+        w = self.compute_optimisation_weights(mask, fitting["lossFunction"], int(fitting["lmax"]))
+        if pars0 is None or (isinstance(pars0, dict) and not pars0):
+            pars0 = self.determine_x0(dwi_, mask, fitting, extraData)
+        else:
+            pars0 = {k: np.asarray(pars0[k], np.float32) for k in self.modelParams}
+        return self._solve(dwi_, mask, w, pars0, fitting, extraData)
+
+    def estimate(self, dwi_, mask, extraData=None, fitting=None, pars0=None):
+        """NEXI/gpuNEXI.m:148-257 (gpuNEXIrice.m:104-201 for the Rician
+        model), one segment: prepare the diffusion data, then fit."""
+        fitting = self.check_set_default(fitting)
+        lmax = int(fitting["lmax"])
+        if self.rician:
+            data, mask, extraData = self.prepare_dwi_data_rice(dwi_, mask, extraData, lmax)
+            data, mask = utils.remove_img_naninf(data, mask)
+        else:
+            data, mask, extraData = self.prepare_dwi_data(dwi_, mask, extraData, lmax, fitting)
+        mask = np.asarray(mask) > 0
+        out = self.fit(np.asarray(data, np.float32), mask, fitting, pars0=pars0, extraData=extraData)
+        out["mask"] = mask
+        return out
 
 
 gpuNEXIrice = gpuNEXI  # the Rician variant is gpuNEXI(..., rician=True)
@@ -594,40 +997,8 @@ class gpuMCRMWI(_ModelBase):
         return gpuR2starMapping.R2star_trapezoidal(img, te)
 
     def despot1(self, img, mask, b1):
-        """DESPOT1 T1 / M0 by linear regression (MCRMWI/private/despot1.m,
-        Deoni et al. MRM 2003): S/sin(a) = E1 S/tan(a) + M0 (1 - E1) with
-        a = FA b1, T1 = -TR / log(E1).  img [x,y,z,nfa]; returns (t1, m0,
-        mask_fitted) on the grid, zero outside the mask and where the fit is
-        not physical (T1 <= 0 or not finite, M0 < 0 or not finite)."""
-        img = np.abs(np.asarray(img, np.float64))
-        dims = img.shape[:3]
-        nfa = img.shape[3]
-        mask = np.asarray(mask) > 0
-        b1 = np.ones(dims) if b1 is None else np.broadcast_to(np.asarray(b1, np.float64).reshape(
-            dims if np.size(b1) > 1 else (1, 1, 1)), dims)
-        t1 = np.zeros(dims)
-        m0 = np.zeros(dims)
-        S = img[mask]                                  # [n, nfa]
-        alpha = np.deg2rad(self.fa.astype(np.float64))[None, :] * b1[mask][:, None]
-        ok = S.sum(axis=1) != 0                        # zero signal: T1 = M0 = 0
-        with np.errstate(divide="ignore", invalid="ignore"):
-            yv = S / np.sin(alpha)
-            xv = S / np.tan(alpha)
-            # least squares of yv = b0 xv + b1 over the flip angles (x \ y)
-            xm, ym = xv.mean(axis=1), yv.mean(axis=1)
-            sxx = ((xv - xm[:, None]) ** 2).sum(axis=1)
-            sxy = ((xv - xm[:, None]) * (yv - ym[:, None])).sum(axis=1)
-            e1 = sxy / sxx
-            icpt = ym - e1 * xm
-            t1v = -float(self.tr) / np.log(e1)
-            m0v = icpt / (1 - np.exp(-float(self.tr) / t1v))
-        t1v = np.where(ok, t1v, 0.0)
-        m0v = np.where(ok, m0v, 0.0)
-        t1[mask], m0[mask] = t1v, m0v
-        fitted = (t1 > 0) & np.isfinite(t1) & (m0 >= 0) & np.isfinite(m0) & mask
-        t1 = np.where(fitted, t1, 0.0)
-        m0 = np.where(fitted, m0, 0.0)
-        return t1, m0, fitted
+        """DESPOT1 T1 / M0 of this acquisition (module function despot1)."""
+        return despot1(self.tr, self.fa, img, mask, b1)
 
     def _grid(self, a, dims):
         """An extraData map [x,y,z] or [x,y,z,nfa] (or [x,y,z,1,nfa]) as

@@ -367,6 +367,20 @@ int gc_posterior_stats(gc_ctx* ctx, const float* posterior_dev, int32_t n_var, i
                        const float* lb, const float* ub, int32_t nbin, float* median_dev, float* iqr_dev,
                        float* mode_dev);
 
+/* ---- dictionary search ------------------------------------------------- */
+/* The likelihood start of NEXI (gpuNEXI.likelihood, NEXI/gpuNEXI.m:822-849;
+ * gpuNEXIrice.likelihood, NEXI/gpuNEXIrice.m:602-625): for each voxel v of
+ * y_dev [m][nv] (voxel fastest) the entry j of dict_dev [m][nd] of least
+ *   sse_j = sum_k w_k (y_kv - c_j d_kj)^2,  c_j = (y_v . d_j) / (d_j . d_j),
+ * the first one on ties; entries with a NaN score never win (MATLAB's min
+ * skips NaN).  w is a HOST array [m] of non-negative weights (NULL: ones).
+ * best_dev [nv] receives the 0-based entry (-1 when no entry scores), and
+ * sse_dev [nv] (FP64, NULL skips it) its sse.  Scores are formed in FP64
+ * from the FP32 inputs.  1 <= m <= 32.  Synchronous on the context's
+ * stream. */
+int gc_dict_search(gc_ctx* ctx, const float* y_dev, int64_t nv, int32_t m, const float* dict_dev, int64_t nd,
+                   const float* w, int32_t* best_dev, double* sse_dev);
+
 /* ---- solvers ------------------------------------------------------------ */
 /* askadam().optimisation (utils/askadam.m:121-365) for one model over a
  * batch of independent volumes, each with its own N, stopping state and

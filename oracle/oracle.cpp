@@ -1154,6 +1154,54 @@ double orc_native_normal(uint64_t seed, uint32_t it, uint64_t chain, uint32_t ba
 
 double orc_logp_gaussian(double sos, double noise, int32_t nm) { return orc::logp_gaussian<double>(sos, noise, nm); }
 
+// gpuNEXI.likelihood (NEXI/gpuNEXI.m:822-849), the dictionary search of the
+// likelihood start: for each voxel column y of y[m][nv], over the entries
+// d_j of dict[m][nd] in order, c = (y' d_j) / dot(d_j, d_j) and
+// sse = sum(wt .* (y - c d_j).^2); MATLAB's min skips NaN and returns the
+// first minimum of a batch, and a batch's minimum replaces the running
+// best only when strictly smaller, so the first global minimum wins.  w
+// NULL: wt = 1.  best[v] = -1 when every sse is NaN (pars_best stays 0).
+// Threads over voxels (each voxel's scan is sequential).
+int orc_dict_search(const float* y, int64_t nv, int32_t m, const float* dict, int64_t nd, const double* w,
+                    int32_t* best, double* sse_out, int32_t threads) {
+  if (m < 1 || nd < 1 || nv < 0) return GC_ERR_ARG;
+  std::vector<double> dd(size_t(m) * nd), nn(nd);
+  for (int64_t j = 0; j < nd; ++j) {
+    double n = 0;
+    for (int k = 0; k < m; ++k) {
+      const double v = dict[size_t(k) * nd + j];
+      dd[size_t(j) * m + k] = v;
+      n += v * v;
+    }
+    nn[j] = n;
+  }
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(static)
+  for (int64_t v = 0; v < nv; ++v) {
+    double yy[64];
+    for (int k = 0; k < m; ++k) yy[k] = y[size_t(k) * nv + v];
+    double bs = std::numeric_limits<double>::infinity();
+    int32_t bi = -1;
+    for (int64_t j = 0; j < nd; ++j) {
+      const double* d = &dd[size_t(j) * m];
+      double a = 0;
+      for (int k = 0; k < m; ++k) a += yy[k] * d[k];
+      const double c = a / nn[j];
+      double s = 0;
+      for (int k = 0; k < m; ++k) {
+        const double r = yy[k] - c * d[k];
+        s += (w ? w[k] : 1.0) * r * r;
+      }
+      if (s < bs) {  // false for NaN
+        bs = s;
+        bi = int32_t(j);
+      }
+    }
+    best[v] = bi;
+    if (sse_out) sse_out[v] = bi >= 0 ? bs : std::numeric_limits<double>::quiet_NaN();
+  }
+  return 0;
+}
+
 double orc_norm_inv(double u) { return orc::norm_inv(u); }
 
 // MH native draws: n[Q] normals and the acceptance uniform

@@ -64,6 +64,8 @@ def lib():
         L.orc_norm_inv.argtypes = [c_double]
         L.orc_norm_inv.restype = c_double
         L.orc_native_mh_draws.argtypes = [c_uint64, c_uint32, c_uint64, c_int32, POINTER(c_double), POINTER(c_double)]
+        L.orc_dict_search.argtypes = [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                      c_int32]
         L.orc_mcmc_global.argtypes = [c_void_p, c_void_p, c_int64] + [c_void_p] * 20 + [c_int32, c_int32]
         _lib = L
     return _lib
@@ -103,6 +105,22 @@ def native_partners_global(seed, rep, it, W, mode):
 
 def norm_inv(u):
     return lib().orc_norm_inv(float(u))
+
+
+def dict_search(y, dict_, w=None, threads=1):
+    """gpuNEXI.likelihood's search (NEXI/gpuNEXI.m:822-849) in FP64:
+    y [m, nv], dict_ [m, nd] (float32 values), w [m] or None.  Returns
+    (best [nv] int32, 0-based, -1 when no entry scores; sse [nv])."""
+    y = np.ascontiguousarray(y, np.float32)
+    d = np.ascontiguousarray(dict_, np.float32)
+    m, nv = y.shape
+    assert d.shape[0] == m and m <= 64
+    best = np.empty(nv, np.int32)
+    sse = np.empty(nv, np.float64)
+    wv = None if w is None else np.ascontiguousarray(w, np.float64)
+    rc = lib().orc_dict_search(_p(y), nv, m, _p(d), d.shape[1], _p(wv), _p(best), _p(sse), int(threads))
+    assert rc == 0
+    return best, sse
 
 
 def native_mh_draws(seed, it, chain, Q):

@@ -44,6 +44,21 @@ def fwd_adj(md, loss, transform, x, y, w=None, extra=None, scale=1.0, residual=F
     return out
 
 
+def dict_search(y, d, w=None, with_sse=True):
+    """gc_dict_search: y [m, nv], d [m, nd] (float32), w [m] host or None."""
+    c = ctx()
+    y = np.ascontiguousarray(y, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    m, nv = y.shape
+    yd, dd = c.to_device(y), c.to_device(d)
+    best = c.empty((max(nv, 1),), np.int32)
+    sse = c.empty((max(nv, 1),), np.float64) if with_sse else None
+    wp = None if w is None else (ctypes.c_float * m)(*np.asarray(w, np.float32))
+    c.check(c.lib.gc_dict_search(c.handle, runtime.ptr(yd), nv, m, runtime.ptr(dd), d.shape[1], wp,
+                                 runtime.ptr(best), runtime.ptr(sse)))
+    return runtime.to_host(best)[:nv], (runtime.to_host(sse)[:nv] if with_sse else None)
+
+
 def askadam(md, cfg, vols, trace=False):
     """vols: list of dict(y, z0, w=None, extra=None).  Returns per-volume dicts."""
     c = ctx()

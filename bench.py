@@ -34,6 +34,7 @@ TRANS_PER_VOXEL_ITER = 18.0      # M exp + P (exp + sqrt + div)
 PEAK_FP32_TFLOPS = 157.3         # MI355X published FP32 vector (packed)
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # half of it; v_fma_f32 alone measures 77.3 on this GPU
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_MFMA_TFLOPS = 78.6     # MI355X published FP64 matrix
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -392,6 +393,72 @@ def bench_c5(ctx, device, iterations, dims=(128, 128, 64)):
                 loss=v.loss, loss_reg=v.loss_reg)
 
 
+def bench_dict(ctx, device, steps, nv=1_000_000, nd=10_000):
+    """NEXI likelihood start (NEXI/gpuNEXI.m:675-849): the reference's
+    10,000-entry dictionary (4 b-values x 2 diffusion times; lmax 0 and 2)
+    searched for 1M voxels of scaled, noisy dictionary signals; the FP64
+    oracle on a 200,000-voxel sample beside it (cpu_agreement: share of that
+    sample where both pick the same entry)."""
+    import torch
+
+    from gacelle_amd import models, runtime
+    from oracle import oracle as orc
+
+    b = np.array([0, 1, 2.5, 4, 6] * 2, float)
+    D = np.array([20] * 5 + [30] * 5, float)
+    out = {}
+    for lmax in (0, 2):
+        m = models.gpuNEXI(b, D)
+        x, S = m.traindata(nd, lmax)
+        M = S.shape[0]
+        g = torch.Generator(device=device)
+        g.manual_seed(17 + lmax)
+        Sd = torch.from_numpy(np.ascontiguousarray(S, np.float32)).to(device)
+        k = torch.randint(0, nd, (nv,), generator=g, device=device)
+        y = (Sd[:, k] * (torch.rand(nv, generator=g, device=device) * 0.2 + 0.8)
+             + 0.01 * torch.randn((M, nv), generator=g, device=device)).contiguous()
+        best = torch.empty(nv, dtype=torch.int32, device=device)
+        wt = np.kron(m.Nav.reshape(-1, 1), 1.0 / (2 * np.arange(0, lmax + 1, 2) + 1)).ravel(order="F")
+        w = None if np.all(wt == 1) else (ctypes.c_float * M)(*wt.astype(np.float32))
+
+        def call():
+            ctx.check(ctx.lib.gc_dict_search(ctx.handle, runtime.ptr(y), nv, M, runtime.ptr(Sd), nd, w,
+                                             runtime.ptr(best), None))
+
+        call()
+        ts = []
+        for _ in range(max(steps, 1)):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            call()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        t = float(np.median(ts))
+        # MFMA products: a_j (and 2 b_j when weighted), 2 M flop per voxel-entry each
+        flop = 2.0 * M * nv * nd * (1 if w is None else 2)
+        ncpu = 200_000
+        yc = y[:, :ncpu].cpu().numpy()
+        nthreads = min(len(os.sched_getaffinity(0)), 16)
+        t0 = time.perf_counter()
+        bo, _ = orc.dict_search(yc, S, None if w is None else wt, threads=nthreads)
+        t_cpu = time.perf_counter() - t0
+        agree = float(np.mean(bo == best[:ncpu].cpu().numpy()))
+        out[f"lmax{lmax}"] = dict(
+            workload=f"{nd} entries x {nv} voxels, {M} measurements per voxel"
+                     + (", weights 1 and 1/5" if w is not None else ""),
+            seconds=t, voxels_per_s=nv / t, voxel_entries_per_s=nv * nd / t,
+            mfma_tflops=flop / t * 1e-12, frac_fp64_mfma_peak=flop / t * 1e-12 / PEAK_FP64_MFMA_TFLOPS,
+            cpu_agreement=agree,
+            cpu_baseline=dict(voxels_per_s=ncpu / t_cpu, cores=nthreads, kind="port",
+                              sample=f"oracle FP64 search of {ncpu} voxels, OpenMP {nthreads} threads: "
+                                     f"{t_cpu:.2f} s"))
+        log(f"dict lmax{lmax}: {t * 1e3:.2f} ms per call ({nv / t:.3e} voxels/s), CPU {ncpu / t_cpu:.3e} voxels/s")
+        del y, Sd, best
+    return out
+
+
 def cpu_baseline_askadam(s0, r2, sigma, iterations, nthreads, nsample, seed=7):
     """The CPU path: the oracle's FP32 restatement, OpenMP over host cores."""
     from oracle import oracle as orc
@@ -444,7 +511,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mh", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--configs", default="c3,c4,c5", help="further workloads to measure (c3, c4, c5 or none)")
+    ap.add_argument("--configs", default="c3,c4,c5,dict",
+                    help="further workloads to measure (c3, c4, c5, dict or none)")
     ap.add_argument("--c3-iterations", type=int, default=2000)
     ap.add_argument("--c4-iterations", type=int, default=500)
     ap.add_argument("--c5-iterations", type=int, default=200)
@@ -550,11 +618,11 @@ def main():
                                                    R2star=float(np.median(np.abs(m[1] - r2) / r2)))
         log(f"MH: {t_mh:.2f} s, {mh['samples_per_s']:.3e} chain-steps/s")
 
-    # ---- further workloads (SURVEY configs 3-5), reported beside the line ------
+    # ---- further workloads, reported beside the line ---------------------------
     extra = {}
     wanted = [c.strip() for c in args.configs.split(",") if c.strip() and c.strip() != "none"]
     for name, fn, it in (("c3", bench_c3, args.c3_iterations), ("c4", bench_c4, args.c4_iterations),
-                         ("c5", bench_c5, args.c5_iterations)):
+                         ("c5", bench_c5, args.c5_iterations), ("dict", bench_dict, args.steps)):
         if name in wanted:
             extra[name] = fn(ctx, device, it)
             log(f"{name}: {json.dumps(extra[name])}")

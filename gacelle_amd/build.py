@@ -30,6 +30,14 @@ HIPCC_FLAGS = [
 ]
 
 
+# per translation unit: the dictionary search keeps its FP64 MFMA
+# accumulators in VGPRs, so the epilogue reads them without
+# v_accvgpr_read copies
+EXTRA_FLAGS = {
+    "dict.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def sources() -> list[str]:
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
@@ -46,7 +54,9 @@ def build_library(verbose: bool = False, jobs: int | None = None) -> str:
     objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
     jobs = jobs or min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda so: (so[0], *_run([hipcc, *HIPCC_FLAGS, "-c", "-o", so[1], so[0]])),
+        results = list(ex.map(lambda so: (so[0], *_run([hipcc, *HIPCC_FLAGS,
+                                                                       *EXTRA_FLAGS.get(os.path.basename(so[0]), []),
+                                                                       "-c", "-o", so[1], so[0]])),
                               zip(srcs, objs)))
     failed = [(s, log) for s, rc, log in results if rc != 0]
     if failed:

@@ -195,3 +195,27 @@ def test_joint_validate_input():
     f["modelParams"], f["ub"], f["lb"] = m.modelParams, m.ub, m.lb
     x0 = m.determine_x0(data, np.ones(m0.shape, bool), None, f)
     assert np.all(x0["R2star"] == 30) and np.all(x0["R1"] == 1)
+
+
+def test_nexi_rician_prepare_masks():
+    """gpuNEXIrice.prepare_dwi_data (NEXI/gpuNEXIrice.m:338-412): sigma / b0,
+    and voxels leave the mask above 1.01, at NaN, at SNR below 1 (sigma > 1
+    after normalisation) or when their curve runs against the mean curve."""
+    m = models.gpuNEXI([1, 2, 3, 4], [20] * 4, rician=True)
+    rng = np.random.default_rng(8)
+    dims = (4, 3, 1)
+    base = np.exp(-0.6 * np.array([1, 2, 3, 4]))
+    data = base * rng.uniform(0.9, 1.0, dims + (1,))
+    data[0, 0, 0] = [1.2, 0.5, 0.3, 0.2]          # above 1.01
+    data[1, 0, 0, 1] = np.nan
+    data[2, 0, 0] = base[::-1] * 0.9              # rising with b
+    sigma = np.full(dims, 0.02)
+    sigma[3, 0, 0] = 2.0                          # SNR below 1
+    ex = dict(bval=np.array([1, 2, 3, 4.0]), sigma=sigma)
+    mask = np.ones(dims, bool)
+    d2, mk, exn = m.prepare_dwi_data_rice(data, mask, ex, 0)
+    bad = {(0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0)}
+    for idx in np.ndindex(*dims):
+        assert mk[idx] == (idx not in bad), idx
+    np.testing.assert_allclose(exn["sigma"], sigma)    # no b=0 volume: b0 = 1
+    assert np.all(np.isfinite(d2))

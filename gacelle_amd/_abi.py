@@ -10,7 +10,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 3
+GC_ABI_VERSION = 4
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
@@ -50,6 +50,7 @@ GC_TRANSFORM_SIGMOID, GC_TRANSFORM_LINEAR, GC_TRANSFORM_NONE = 0, 1, 2
 GC_CONV_EMA, GC_CONV_LINEAR = 0, 1
 GC_ADAM_EFFECTIVE_LR, GC_ADAM_KINGMA_BA = 0, 1
 GC_TV_2D, GC_TV_3D = 0, 1
+GC_REG_TV, GC_REG_PRIOR = 0, 1
 GC_STOP_NONE, GC_STOP_CONVERGENCE, GC_STOP_TOL, GC_STOP_STEP, GC_STOP_GRAD = 0, 1, 2, 3, 4
 GC_MCMC_MH, GC_MCMC_GW = 0, 1
 GC_ENSEMBLE_SIMULTANEOUS, GC_ENSEMBLE_REDBLACK = 0, 1
@@ -96,7 +97,7 @@ class gc_askadam_cfg(ctypes.Structure):
         ("n_reg", c_int32),
         ("regmap", c_int32 * GC_MAX_REG),
         ("robust", c_int32),
-        ("reserved", c_int32),
+        ("reg_type", c_int32),
         ("lr0", c_double),
         ("decay", c_double),
         ("beta1", c_double),
@@ -152,6 +153,8 @@ class gc_volume(ctypes.Structure):
         ("kernel_ms", c_double),
         ("kernel_launches", c_int32),
         ("iters_per_launch", c_int32),
+        ("prior_dev", c_void_p),
+        ("reg_offset", c_double),
     ]
 
 

@@ -220,6 +220,12 @@ class askadam:
             v.residual_dev = runtime.ptr(res_d)
             v.residual_min_dev = runtime.ptr(resmin_d)
             v.loss_trace_host = ctypes.cast(trace, ctypes.POINTER(ctypes.c_double)) if trace is not None else None
+            prior = _prior_rows(fitting, extra_data, mask, vox)
+            if prior is not None:
+                rows, v.reg_offset = prior
+                pr_d = ctx.to_device(rows)
+                v.prior_dev = runtime.ptr(pr_d)
+                keep.append(pr_d)
             keep += [y_d, w_d, ex_d, vox_d, z_d, zmin_d, res_d, resmin_d, rl_d, rlmin_d]
             metas.append(dict(mask=mask, vox=vox, nv=nv, nmeas=nmeas, z=z_d, zmin=zmin_d, res=res_d, resmin=resmin_d,
                               rl=rl_d, rlmin=rlmin_d, trace=trace, theta0=theta0))
@@ -272,6 +278,8 @@ class askadam:
                 # the parameter slot of the map (multi-slot maps precede it)
                 c.regmap[k] = slots[name].start if slots is not None else names.index(name)
                 c.lambda_[k] = lam[k]
+        c.reg_type = _abi.GC_REG_PRIOR if str(fitting.get("regularisationType", "TV")).lower() == "prior" \
+            else _abi.GC_REG_TV
         c.robust = int(bool(fitting["robustConvergence"]))
         c.lr0 = float(fitting["initialLearnRate"])
         c.decay = float(fitting["decayRate"])
@@ -366,6 +374,48 @@ def _resolve_model(userfuncCell):
         return f.__self__
     raise GacelleError(_abi.GC_ERR_UNSUPPORTED,
                        "the fused GPU path needs one of this package's model classes as the forward model")
+
+
+def _prior_rows(fitting, extra_data, mask, vox):
+    """The normal prior of gpuNEXI (NEXI/gpuNEXI.m:1256-1303, checks of
+    :333-349): mu / sigma maps of every regmap name from
+    extradata['mu'][name] / extradata['sigma'][name] on the image grid.
+    Returns ([mu rows; sigma rows] at the masked voxels, the constant the
+    grid voxels outside the mask add to loss_reg), or None without a prior."""
+    if str(fitting.get("regularisationType", "TV")).lower() != "prior":
+        return None
+    lam = [float(x) for x in fitting["lambda"]]
+    if not lam or lam[0] <= 0:
+        return None
+    regmap = fitting["regmap"]
+    regmap = [regmap] if isinstance(regmap, str) else list(regmap)
+    ex = extra_data if isinstance(extra_data, dict) else {}
+    mu, sd = ex.get("mu"), ex.get("sigma")
+    if not isinstance(mu, dict) or not isinstance(sd, dict):
+        raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:missingPriorDistribution: fitting.regularisationType is 'prior' "
+                           "and fitting.lambda > 0, but extradata.mu/extradata.sigma were not provided to fit().")
+    miss_mu = [n for n in regmap if n not in mu]
+    miss_sd = [n for n in regmap if n not in sd]
+    if miss_mu or miss_sd:
+        raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:incompletePriorDistribution: extradata.mu/extradata.sigma must "
+                           "have a field for every name in fitting.regmap. Missing from mu: {%s}. Missing from sigma: "
+                           "{%s}." % (", ".join(miss_mu), ", ".join(miss_sd)))
+    names = regmap[: len(lam)]
+    ngrid = int(np.prod(utils.dims3(mask)))
+    inside = np.zeros(ngrid, bool)
+    inside[vox] = True
+    rows = np.zeros((2 * len(names), vox.size), np.float32)
+    offset = 0.0
+    for k, n in enumerate(names):
+        m = np.asarray(mu[n], np.float32).reshape(-1, order="F")
+        s = np.asarray(sd[n], np.float32).reshape(-1, order="F")
+        if m.size != ngrid or s.size != ngrid:
+            raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, f"prior maps of '{n}' must match the mask grid")
+        rows[k], rows[len(names) + k] = m[vox], s[vox]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            zo = np.abs((np.float32(0) - m[~inside]) / s[~inside])
+        offset += lam[k] / vox.size * float(zo[np.isfinite(zo)].astype(np.float64).sum())
+    return rows, offset
 
 
 def _to_gd(data, mask, vox) -> np.ndarray:

@@ -37,6 +37,7 @@ struct VolRun {
   double carry_step = 0;
   double loss = 0, fid = 0, reg = 0;
   double min_loss = 0, min_fid = 0, min_reg = 0;
+  double reg_offset = 0;  // the prior's out-of-mask loss (constant)
   int min_iter = 0, last_epoch = 0, stop = GC_STOP_NONE;
   bool have_init = false;
   int final_from = -1;  // z_last buffer holding the final z, or -1 (replay)
@@ -87,7 +88,9 @@ struct AskDriver {
   std::vector<double> row_sums;  // [vol][K*kSum] after fold_sums
   bool desc_in_flight = false;  // an async copy may still read h_vols/h_es
   bool step_mode = false;       // one launch per iteration (TV or streamed model)
-  bool use_tv = false;
+  bool use_reg = false;  // a regulariser runs (n_reg > 0, lambda[0] > 0)
+  bool prior = false;    // ... the normal prior
+  bool use_tv = false;   // ... spatial total variation
   bool robust = false;
   bool replaying = false;       // a replay launch: no per-voxel loss outputs
   bool replay_pinned = false;   // ... from the pinned block (its weight modulator)
@@ -144,21 +147,31 @@ struct AskDriver {
 
   // stream_model: the model's data do not fit in registers (step form only)
   int setup(bool stream_model) {
-    use_tv = cfg.n_reg > 0 && cfg.lambda[0] > 0;
+    use_reg = cfg.n_reg > 0 && cfg.lambda[0] > 0;
+    prior = use_reg && cfg.reg_type == GC_REG_PRIOR;
+    use_tv = use_reg && !prior;
     robust = cfg.robust != 0;
-    step_mode = use_tv || stream_model || robust;
+    step_mode = use_reg || stream_model || robust;
+    if (use_reg && cfg.reg_type != GC_REG_TV && cfg.reg_type != GC_REG_PRIOR)
+      return set_err(ctx, GC_ERR_ARG, "unknown regulariser type %d", cfg.reg_type);
     if (robust && (cfg.weight_update_interval < 1 || cfg.outlier_check_window < 1 || cfg.convergence_window < 1))
       return set_err(ctx, GC_ERR_ARG, "robustConvergence needs weightUpdateInterval, outlierCheckWindow and "
                                       "convergenceWindow >= 1");
-    if (use_tv) {
+    if (use_reg) {
       if (cfg.n_reg > GC_MAX_REG) return set_err(ctx, GC_ERR_ARG, "at most %d regularised maps", GC_MAX_REG);
-      if (cfg.tv_mode != GC_TV_2D && cfg.tv_mode != GC_TV_3D) return set_err(ctx, GC_ERR_ARG, "unknown TVmode");
       for (int k = 0; k < cfg.n_reg; ++k)
         if (cfg.regmap[k] < 0 || cfg.regmap[k] >= model_np(md))
           return set_err(ctx, GC_ERR_ARG, "regmap %d names slot %d, not a model parameter", k, cfg.regmap[k]);
+    }
+    if (use_tv) {
+      if (cfg.tv_mode != GC_TV_2D && cfg.tv_mode != GC_TV_3D) return set_err(ctx, GC_ERR_ARG, "unknown TVmode");
       for (int d = 0; d < 3; ++d)
         if (!(cfg.voxel_size[d] > 0)) return set_err(ctx, GC_ERR_ARG, "voxelSize must be positive");
     }
+    if (prior)
+      for (int i = 0; i < n_vol; ++i)
+        if (!vols[i].prior_dev)
+          return set_err(ctx, GC_ERR_ARG, "volume %d: the prior regulariser needs prior_dev (mu and sigma)", i);
     int rcm = upload_model(ctx, &md, &dmd);
     if (rcm) return rcm;
     int64_t nv_total = 0;
@@ -199,6 +212,7 @@ struct AskDriver {
     for (int i = 0; i < n_vol; ++i) {
       VolRun& R = run[i];
       R.nv = vols[i].nv;
+      R.reg_offset = prior ? vols[i].reg_offset : 0.0;
       R.nblk = int((R.nv + kBlock - 1) / kBlock);
       const size_t pbytes = size_t(P) * R.nv * sizeof(float);
       for (int s = 0; s < 3; ++s)
@@ -457,6 +471,7 @@ struct AskDriver {
           d.z_last = (k == n - 1 && zl[i] >= 0) ? R.zlast[zl[i]].as<float>() : nullptr;
           d.partials = R.partials.as<float>();
           d.nbr = use_tv ? R.nbr.as<int32_t>() : nullptr;
+          d.prior = prior ? vols[i].prior_dev : nullptr;
           if (robust) {
             const int e = e0[i] + k;
             // update_outlier_mask returns modulated weights only on a check epoch
@@ -483,8 +498,9 @@ struct AskDriver {
     desc_in_flight = true;
     StepParams sp{};
     sp.n_vol = n_vol;
-    sp.n_reg = use_tv ? cfg.n_reg : 0;
+    sp.n_reg = use_reg ? cfg.n_reg : 0;
     sp.nd = (cfg.tv_mode == GC_TV_3D) ? 3 : 2;
+    sp.reg_type = prior ? GC_REG_PRIOR : GC_REG_TV;
     for (int k = 0; k < GC_MAX_REG; ++k) {
       sp.regmap[k] = k < cfg.n_reg ? cfg.regmap[k] : 0;
       sp.lam[k] = k < cfg.n_reg ? float(cfg.lambda[k]) : 0.f;
@@ -516,7 +532,7 @@ struct AskDriver {
                    double sum_reg, double* trace, const double* loss_c_robust = nullptr) {
     const double count = double(md.n_meas) * double(R.nv);
     R.fid = sum_l / count;
-    R.reg = sum_reg / double(R.nv);  // spatial_total_variation: / Nmasked
+    R.reg = sum_reg / double(R.nv) + R.reg_offset;  // / Nmasked (+ the prior's out-of-mask constant)
     R.loss = R.fid + R.reg;
     if (trace) trace[epoch] = R.loss;
     R.last_epoch = epoch;
@@ -660,7 +676,7 @@ struct AskDriver {
         VolRun& R = run[i];
         const double* s = sums + size_t(i) * K * kSum;
         R.fid = s[0] / (double(md.n_meas) * double(R.nv));
-        R.reg = s[4] / double(R.nv);
+        R.reg = s[4] / double(R.nv) + R.reg_offset;
         R.loss = R.fid + R.reg;
         R.min_loss = R.loss; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = 0; R.last_epoch = 0;
         if (vols[i].loss_trace_host) vols[i].loss_trace_host[0] = R.loss;
@@ -703,7 +719,7 @@ struct AskDriver {
         if (!R.have_init) {
           // starting-point evaluation (:197-219) equals the first iteration's
           R.fid = s[0] / (double(md.n_meas) * double(R.nv));
-          R.reg = s[4] / double(R.nv);
+          R.reg = s[4] / double(R.nv) + R.reg_offset;
           const double l0 = R.fid + R.reg;
           R.loss = l0;
           R.min_loss = l0; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = 0;

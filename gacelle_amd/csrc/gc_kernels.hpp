@@ -422,6 +422,7 @@ struct StepVol {
   float* z_last;       // z (pre-update) of this iteration, or null
   float* partials;     // [nblk][n_epochs][kStepSums]
   const int32_t* nbr;  // [2*nd][nv]: +axis, -axis neighbour G-D index per axis, -1 outside the mask
+  const float* prior;  // GC_REG_PRIOR: [2*n_reg][nv], mu rows then sigma rows, or null
   const float* wmod;   // robustConvergence: per-voxel weight factor of this iteration, or null
   const uint8_t* mconv;  // robustConvergence: convergence voxel set, or null
   float* pvl_out;      // robustConvergence: per-voxel mean loss of this iteration, or null
@@ -438,9 +439,9 @@ struct StepVol {
 struct StepParams {
   const StepVol* vols;
   int32_t n_vol;
-  int32_t n_reg;       // regularised maps (0: no TV)
+  int32_t n_reg;       // regularised maps (0: none)
   int32_t nd;          // TV axes: 2 ('2D') or 3 ('3D')
-  int32_t pad;
+  int32_t reg_type;    // GC_REG_TV | GC_REG_PRIOR
   int32_t regmap[4];   // parameter slot of each regularised map
   float lam[4];
   float inv_vs[3];     // 1/voxelSize
@@ -510,6 +511,10 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
   // (forward differences with periodic wrap, zeros outside the mask), and
   // its subgradient (sign(0) = 0) at i from i's own and its -e_d
   // neighbour's terms
+  //
+  // normal prior (gpuNEXI.prior_distribution_normal): cost
+  // |(theta - mu) / sigma| where finite, subgradient sign(.) / sigma; the
+  // grid voxels outside the mask add a constant the host carries
   float reg = 0.f;
   for (int kr = 0; kr < sp.n_reg; ++kr) {
     const int pr = sp.regmap[kr];
@@ -520,7 +525,16 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
     for (int p = 0; p < NP; ++p)
       if (p == pr) { ti = th[p]; dti = dth[p]; }
     float gt = 0.f;
-    for (int d = 0; d < sp.nd; ++d) {
+    if (sp.reg_type == GC_REG_PRIOR) {
+      const float mu = V.prior[int64_t(kr) * nv + vv];
+      const float sd = V.prior[int64_t(sp.n_reg + kr) * nv + vv];
+      const float zs = (ti - mu) / sd;
+      if (isfinite(zs)) {
+        reg += lam * fabsf(zs);
+        gt = lam * sgnf(zs) / sd;
+      }
+    }
+    for (int d = 0; d < (sp.reg_type == GC_REG_PRIOR ? 0 : sp.nd); ++d) {
       const float ivs = sp.inv_vs[d];
       const int32_t jf = V.nbr[int64_t(2 * d) * nv + vv];
       float tf = 0.f, dd;

@@ -857,8 +857,10 @@ class gpuNEXI(_ModelBase):
     def fit(self, dwi_, mask=None, fitting=None, *args, pars0=None, extraData=None):
         """NEXI/gpuNEXI.m:290-450: fit(dwi, mask, fitting, pars0, extradata);
         the Rician model keeps gpuNEXIrice's order fit(dwi, mask, fitting,
-        extraData, pars0) (NEXI/gpuNEXIrice.m:203-336).  The 'prior'
-        regulariser and surface data are not part of this library."""
+        extraData, pars0) (NEXI/gpuNEXIrice.m:203-336).  regularisationType
+        'TV' or 'prior' (extradata['mu'][name], extradata['sigma'][name] on
+        the image grid for every regmap name); surface data are not part of
+        this library."""
         if args:
             first, rest = args[0], (args[1] if len(args) > 1 else None)
             if self.rician:
@@ -877,8 +879,10 @@ class gpuNEXI(_ModelBase):
         if len(np.atleast_1d(fitting["lb"])) == 0:
             fitting["lb"] = list(self.lb)
         lam = np.atleast_1d(fitting.get("lambda", [0]))
-        if str(fitting.get("regularisationType", "TV")).lower() == "prior" and lam.size and float(lam[0]) > 0:
-            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "the 'prior' regulariser is not part of this library")
+        if str(fitting.get("regularisationType", "TV")).lower() == "prior" and lam.size and float(lam[0]) > 0 \
+                and self.rician:
+            # gpuNEXIrice has no prior path, and its extraData.sigma is the noise map
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "the 'prior' regulariser is a gpuNEXI option (not gpuNEXIrice)")
         if str(fitting.get("dataType", "volumetric")).lower() == "surface":
             raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "surface data (surface TV) are not part of this library")
         w = self.compute_optimisation_weights(mask, fitting["lossFunction"], int(fitting["lmax"]))

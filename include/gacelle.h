@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 3
+#define GC_ABI_VERSION 4
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
@@ -132,6 +132,10 @@ enum { GC_CONV_EMA = 0, GC_CONV_LINEAR = 1 };
  * (effective learning rate) or Kingma & Ba Alg. 1: lr * mhat/(sqrt(vhat)+eps) */
 enum { GC_ADAM_EFFECTIVE_LR = 0, GC_ADAM_KINGMA_BA = 1 };
 enum { GC_TV_2D = 0, GC_TV_3D = 1 };
+/* regulariser of the regmap parameters: spatial total variation
+ * (utils/spatial_total_variation.m:24-66) or the normal prior of gpuNEXI
+ * (prior_distribution_normal, NEXI/gpuNEXI.m:1256-1303) */
+enum { GC_REG_TV = 0, GC_REG_PRIOR = 1 };
 enum {
   GC_STOP_NONE = 0,        /* ran all iterations                      */
   GC_STOP_CONVERGENCE = 1, /* loss convergence below tolerance        */
@@ -156,7 +160,7 @@ typedef struct gc_askadam_cfg {
   int32_t n_reg;              /* number of regularised maps (0 = none)  */
   int32_t regmap[GC_MAX_REG]; /* parameter slot of each regularised map */
   int32_t robust;             /* fitting.robustConvergence              */
-  int32_t reserved;
+  int32_t reg_type;           /* GC_REG_TV | GC_REG_PRIOR (fitting.regularisationType) */
   double lr0;                 /* fitting.initialLearnRate               */
   double decay;               /* fitting.decayRate                      */
   double beta1, beta2, eps;   /* adamupdate decays and epsilon          */
@@ -215,6 +219,14 @@ typedef struct gc_volume {
   double kernel_ms;          /* summed duration of all fused launches    */
   int32_t kernel_launches;   /* number of fused launches                 */
   int32_t iters_per_launch;  /* K, iterations per launch                 */
+  /* GC_REG_PRIOR: loss_reg = sum_k lambda_k / nv * sum over the image grid
+   * of the finite |(theta_k - mu_k) / sigma_k|, theta_k zero outside the
+   * mask.  prior_dev holds mu then sigma of each regularised map at the
+   * masked voxels, [2*n_reg x nv]; reg_offset is the part the grid voxels
+   * outside the mask contribute (lambda_k / nv * sum |mu_k / sigma_k|), a
+   * constant added to loss_reg every iteration. */
+  const float* prior_dev;
+  double reg_offset;
 } gc_volume;
 
 /* ---- MCMC configuration: POD mirror of mcmc.check_set_default_basic

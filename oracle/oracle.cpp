@@ -312,8 +312,8 @@ struct AskOut {
 template <class S, int NP>
 static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv, const float* y,
                        const float* w, const float* extra, const int64_t* vox, const int32_t* dims,
-                       const float* z0, double* zfinal, double* zmin, double* res_final, double* res_min,
-                       double* trace, AskOut& out, int nthreads) {
+                       const float* prior_grid, const float* z0, double* zfinal, double* zmin, double* res_final,
+                       double* res_min, double* trace, AskOut& out, int nthreads) {
   const int P = md.n_par, M = md.n_meas, E = md.n_extra;
   const S inv_count = S(1.0 / (double(M) * double(nv)));
   std::vector<S> z(size_t(P) * nv), g(size_t(P) * nv), m1(size_t(P) * nv, S(0)), m2(size_t(P) * nv, S(0)),
@@ -326,11 +326,14 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
   if (cfg.transform == GC_TRANSFORM_LINEAR)
     for (auto& v : z) v = std::min(std::max(v, S(0)), S(1));
 
-  const bool use_tv = cfg.n_reg > 0 && cfg.lambda[0] > 0;
+  const bool use_reg = cfg.n_reg > 0 && cfg.lambda[0] > 0;
+  const bool use_prior = use_reg && cfg.reg_type == GC_REG_PRIOR;
+  const bool use_tv = use_reg && !use_prior;
+  if (use_prior && (!prior_grid || !vox || !dims)) return GC_ERR_ARG;
   int64_t nvox = 1;
   std::vector<uint8_t> mask;
   std::vector<S> img, gimg;
-  if (use_tv) {
+  if (use_reg) {
     nvox = int64_t(dims[0]) * dims[1] * dims[2];
     mask.assign(nvox, 0);
     for (int64_t v = 0; v < nv; ++v) mask[vox[v]] = 1;
@@ -383,6 +386,36 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
         for (int64_t v = 0; v < nv; ++v)
           g[size_t(p) * nv + v] += gimg[vox[v]] * dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v],
                                                              S(md.lb[p]), S(md.ub[p]));
+      }
+    }
+    // gpuNEXI.prior_distribution_normal (NEXI/gpuNEXI.m:1256-1303): theta as
+    // an image (zeros outside the mask, reshape_GD2ND), zscore =
+    // (theta - mu) ./ sigma over the whole grid, loss_reg +=
+    // sum(abs(zscore(finite))) / Nsample * lambda; the gradient reaches the
+    // masked voxels only (sign(0) = 0, as dlgradient of abs)
+    if (use_prior) {
+      for (int k = 0; k < cfg.n_reg; ++k) {
+        const int p = cfg.regmap[k];
+        const float* mu = prior_grid + size_t(k) * nvox;                 // mu rows, then
+        const float* sd = prior_grid + size_t(cfg.n_reg + k) * nvox;     // sigma rows
+        std::fill(img.begin(), img.end(), S(0));
+        for (int64_t v = 0; v < nv; ++v)
+          img[vox[v]] = unscale<S, S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
+        double sum = 0;
+        for (int64_t i = 0; i < nvox; ++i) {
+          const S zs = (img[i] - S(mu[i])) / S(sd[i]);
+          if (std::isfinite(double(zs))) sum += std::fabs(double(zs));
+        }
+        reg += sum / double(nv) * cfg.lambda[k];
+        const S scale = S(cfg.lambda[k] / double(nv));
+        for (int64_t v = 0; v < nv; ++v) {
+          const int64_t i = vox[v];
+          const S zs = (img[i] - S(mu[i])) / S(sd[i]);
+          if (!std::isfinite(double(zs))) continue;
+          const S sg = zs > S(0) ? S(1) : (zs < S(0) ? S(-1) : S(0));
+          g[size_t(p) * nv + v] += scale * sg / S(sd[i]) *
+                                   dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
+        }
       }
     }
     loss = fid + reg;  // :102
@@ -1255,7 +1288,8 @@ int orc_tv(const double* img, const uint8_t* mask, const int32_t dims[3], int32_
 // askadam; precision 0 = double (oracle), 1 = float (CPU-path baseline).
 // scal_out[10] = {loss, fid, reg, min_loss, min_fid, min_reg, conv, niter, min_iter, stop}
 int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const float* y, const float* w,
-                const float* extra, const int64_t* vox, const int32_t* dims, const float* z0, double* zfinal,
+                const float* extra, const int64_t* vox, const int32_t* dims, const float* prior_grid,
+                const float* z0, double* zfinal,
                 double* zmin, double* res_final, double* res_min, double* trace, double* scal_out, int32_t precision,
                 int32_t nthreads) {
   if (md->n_meas > 1024 || md->n_par > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
@@ -1264,13 +1298,13 @@ int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const
   if (precision == 0) {
     rc = orc::with_np(md->n_par, [&](auto npc) {
       constexpr int NP = decltype(npc)::value;
-      return orc::askadam_run<double, NP>(*md, *cfg, nv, y, w, extra, vox, dims, z0, zfinal, zmin, res_final,
+      return orc::askadam_run<double, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, z0, zfinal, zmin, res_final,
                                           res_min, trace, o, std::max(1, nthreads));
     });
   } else {
     rc = orc::with_np(md->n_par, [&](auto npc) {
       constexpr int NP = decltype(npc)::value;
-      return orc::askadam_run<float, NP>(*md, *cfg, nv, y, w, extra, vox, dims, z0, zfinal, zmin, res_final,
+      return orc::askadam_run<float, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, z0, zfinal, zmin, res_final,
                                          res_min, trace, o, std::max(1, nthreads));
     });
   }

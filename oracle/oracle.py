@@ -55,7 +55,7 @@ def lib():
         L.orc_tv.argtypes = [c_void_p, c_void_p, POINTER(c_int32), c_int32, POINTER(c_double), c_double, c_int64,
                              POINTER(c_double), c_void_p]
         L.orc_askadam.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   POINTER(c_double), c_int32, c_int32]
         L.orc_mcmc.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -173,7 +173,10 @@ def tv(img, mask, mode, vsize, lam, nmasked, grad=True):
     return (loss.value, g) if grad else loss.value
 
 
-def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0, nthreads=1, trace=False):
+def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0, nthreads=1, trace=False,
+            prior_grid=None):
+    """prior_grid: GC_REG_PRIOR maps on the whole grid (column-major),
+    [2*n_reg, ngrid]: the mu rows, then the sigma rows; with vox and dims."""
     y = _f32(y)
     M, nv = y.shape
     z0 = _f32(z0)
@@ -188,7 +191,9 @@ def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0
     rm = np.zeros((M, nv))
     tr = np.zeros(int(cfg.iteration) + 1) if trace else None
     sc = (c_double * 10)()
-    rc = lib().orc_askadam(ctypes.byref(md), ctypes.byref(cfg), nv, _p(y), _p(w), _p(extra), _p(vox), d, _p(z0),
+    pg = None if prior_grid is None else np.ascontiguousarray(prior_grid, np.float32)
+    rc = lib().orc_askadam(ctypes.byref(md), ctypes.byref(cfg), nv, _p(y), _p(w), _p(extra), _p(vox), d, _p(pg),
+                           _p(z0),
                            _p(zf), _p(zm), _p(rf), _p(rm), _p(tr), sc, precision, nthreads)
     assert rc == 0, rc
     out = dict(z_final=zf, z_min=zm, residual=rf, residual_min=rm, loss=sc[0], loss_fidelity=sc[1],

@@ -60,7 +60,8 @@ def dict_search(y, d, w=None, with_sse=True):
 
 
 def askadam(md, cfg, vols, trace=False):
-    """vols: list of dict(y, z0, w=None, extra=None).  Returns per-volume dicts."""
+    """vols: list of dict(y, z0, w=None, extra=None, vox=None, dims=None,
+    prior=None, reg_offset=0).  Returns per-volume dicts."""
     c = ctx()
     n = len(vols)
     arr = (_abi.gc_volume * n)()
@@ -98,6 +99,11 @@ def askadam(md, cfg, vols, trace=False):
             for k in range(3):
                 a.dims[k] = int(v["dims"][k])
             keep.append(vd)
+        if v.get("prior") is not None:   # [2*n_reg, nv] mu rows then sigma rows, and the out-of-mask constant
+            pd = c.to_device(np.asarray(v["prior"], np.float32))
+            a.prior_dev = runtime.ptr(pd)
+            a.reg_offset = float(v.get("reg_offset", 0.0))
+            keep.append(pd)
         keep += [yd, wd, ex, zd, zm, rl, rlm, rs, rsm]
         metas.append((zd, zm, rl, rlm, rs, rsm, tr))
     c.check(c.lib.gc_askadam_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), arr, n))

@@ -16,7 +16,7 @@ import ctypes
 
 import numpy as np
 
-from . import _abi, dwi, runtime, utils
+from . import _abi, dwi, runtime, segment, utils
 from ._abi import GacelleError
 
 
@@ -115,6 +115,24 @@ class _ModelBase:
 
     def estimate_prior(self, y, mask, fitting):
         return utils.initialise_x0(utils.dims3(mask), self.modelParams, self.startPoint)
+
+    def _segment_bytes(self, data, fitting, nmeas=None, n_extra=0, weighted=None):
+        """Device bytes per masked voxel of one fit of this model (the
+        footprint segment.find_optimal_segment_3D splits the volume by)."""
+        data = np.asarray(data)
+        m = int(np.prod(data.shape[3:])) if nmeas is None else nmeas
+        w = bool(fitting.get("isWeighted", False)) if weighted is None else weighted
+        npar = sum(self._param_size(n) for n in self.modelParams)
+        solver = str(fitting.get("solver", "askadam")).lower()
+        ns = 0
+        chains = 1
+        if solver == "mcmc":
+            it, burn = int(fitting.get("iteration", 0)), fitting.get("burnin", 0.1)
+            nb = int(it * burn) if burn < 1 else int(burn)
+            ns = max(0, (it - nb - 1) // max(int(fitting.get("thinning", 20)), 1) + 1)
+            chains = int(fitting.get("repetition", 1)) * (int(fitting.get("Nwalker", 50))
+                                                          if str(fitting.get("algorithm", "MH")).upper() == "GW" else 1)
+        return segment.fit_bytes(m, npar, n_extra, w, solver, ns, chains)
 
     def _solve(self, data, mask, w, pars0, fitting, *fwd_args):
         from .askadam import askadam as _ask
@@ -306,7 +324,8 @@ class gpuR2starMapping(_ModelBase):
         scale = utils.prctile(np.abs(data[:, :, :, 0])[mask], 98)
         data = data / scale
         data, mask = utils.remove_img_naninf(data, mask)
-        out = self.fit(data.astype(np.float32), mask, fitting)
+        out = segment.run_segments(lambda d, m, e: self.fit(np.asarray(d, np.float32), m, fitting), data, mask, None,
+                                   fitting, self._segment_bytes(data, fitting))
         if str(fitting["solver"]).lower() == "mcmc":
             for k in fitting.get("metric", ["mean", "std"]):
                 if k in out and "M0" in out[k]:
@@ -493,7 +512,8 @@ class gpuJointR1R2starMapping(_ModelBase):
         data scale."""
         fitting = self.check_set_default(fitting)
         data, mask, scale = self.prepare_data(data, mask, extraData)
-        out = self.fit(np.asarray(data, np.float32), mask, fitting, extraData)
+        out = segment.run_segments(lambda d, m, e: self.fit(np.asarray(d, np.float32), m, fitting, e), data, mask,
+                                   extraData, fitting, self._segment_bytes(data, fitting, n_extra=1))
         out["mask"] = mask
         if str(fitting["solver"]).lower() == "mcmc":
             for k in fitting.get("metric", ["mean", "std"]):
@@ -903,7 +923,17 @@ class gpuNEXI(_ModelBase):
         else:
             data, mask, extraData = self.prepare_dwi_data(dwi_, mask, extraData, lmax, fitting)
         mask = np.asarray(mask) > 0
-        out = self.fit(np.asarray(data, np.float32), mask, fitting, pars0=pars0, extraData=extraData)
+        if pars0 is not None:
+            # pars0 maps travel with the data slices
+            extraData = dict(extraData or {}, _pars0=pars0)
+
+        def fit_one(d, m, e):
+            e = dict(e or {})
+            p0 = e.pop("_pars0", None)
+            return self.fit(np.asarray(d, np.float32), m, fitting, pars0=p0, extraData=e or None)
+
+        out = segment.run_segments(fit_one, data, mask, extraData, fitting,
+                                   self._segment_bytes(data, fitting, n_extra=1 if self.rician else 0, weighted=True))
         out["mask"] = mask
         return out
 
@@ -1163,7 +1193,9 @@ class gpuMCRMWI(_ModelBase):
         one launch): prepare_data, fit, then S0 back to the data scale."""
         fitting = self.check_set_default(fitting, data)
         data, mask, extraData, scale = self.prepare_data(data, mask, extraData, fitting)
-        out = self.fit(data, mask, fitting, extraData)
+        nm = int(np.prod(np.asarray(data).shape[3:])) * (2 if np.iscomplexobj(data) else 1)
+        out = segment.run_segments(lambda d, m, e: self.fit(d, m, fitting, e), data, mask, extraData, fitting,
+                                   self._segment_bytes(data, fitting, nmeas=nm, n_extra=3 + 2 * self.fa.size))
         out["mask"] = mask
         if str(fitting["solver"]).lower() == "mcmc":
             for k in fitting.get("metric", ["mean", "std"]):

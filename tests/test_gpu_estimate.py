@@ -93,3 +93,35 @@ def test_joint_estimate_mcmc_rescales_m0():
     assert np.nanmedian(post[post > 0]) > 100          # data units, not the unit scale
     rel_r1 = np.abs(out["mean"]["R1"] - r1) / r1
     assert np.median(rel_r1) < 0.1
+
+
+def test_segmented_estimate_matches_one_piece():
+    """gpuR2starMapping.estimate split into slabs (fitting.NSegmentUser, or a
+    memory budget below the footprint) equals the one-piece fit: the voxels
+    are independent and, without stopping criteria, every slab runs the same
+    iterations (Adam normalises the 1/Nv gradient scale), with an L2 loss."""
+    print("segmented estimate", flush=True)
+    from gacelle_amd import segment
+
+    rng = np.random.default_rng(9)
+    dims = (12, 10, 8)
+    te = np.arange(2, 47, 4) * 1e-3
+    s0 = rng.uniform(500, 1500, dims)
+    r2 = rng.uniform(10, 60, dims)
+    data = s0[..., None] * np.exp(-r2[..., None] * te) + rng.normal(0, 5.0, dims + (12,))
+    mask = rng.random(dims) < 0.8
+    # L2: with L1 the sign of near-zero residuals makes trajectories
+    # sensitive to the FP32 rounding of the per-slab gradient scale
+    common = dict(iteration=300, verbose=False, tol=0, convergenceValue=-1, convergenceStepTol=0,
+                  convergenceGradTol=0, initialLearnRate=0.01, lossFunction="L2")
+    one = models.gpuR2starMapping(te).estimate(data, mask, dict(common))
+    two = models.gpuR2starMapping(te).estimate(data, mask, dict(common, NSegmentUser=2))
+    assert isinstance(two["final"]["loss"], list) and len(two["final"]["loss"]) == 2
+    m = mask
+    for name in ("M0", "R2star"):
+        np.testing.assert_allclose(two["final"][name][m], one["final"][name][m], rtol=2e-3)
+    bpv = models.gpuR2starMapping(te)._segment_bytes(data, models.gpuR2starMapping.check_set_default({}))
+    budget = segment.FIXED_BYTES + bpv * int(mask.sum()) // 3
+    three = models.gpuR2starMapping(te).estimate(data, mask, dict(common, memoryBudget=budget))
+    assert len(three["final"]["loss"]) >= 3
+    np.testing.assert_allclose(three["final"]["R2star"][m], one["final"]["R2star"][m], rtol=2e-3)

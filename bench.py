@@ -311,7 +311,7 @@ def bench_c4(ctx, device, iterations, nv=500_000, walkers=16):
     err = {n: float(torch.median(torch.abs(mean[q] - th[q]) / th[q])) for q, n in enumerate(names[:4])}
     return dict(workload=f"NEXI (6 shells x 2 Delta, GL64, Rician mean) Goodman-Weare, {nv} voxels x {walkers} walkers x "
                          f"{iterations} steps (simultaneous update, native stream)",
-                seconds=t, walker_steps_per_s=steps / t, kernel_ms=io.kernel_ms,
+                seconds=t, walker_steps_per_s=steps / t, kernel_ms=io.kernel_ms, block_threads=io.block_threads,
                 transcendentals_per_s=steps * 3072 / t, posterior_mean_median_rel_err=err)
 
 
@@ -391,6 +391,64 @@ def bench_c5(ctx, device, iterations, dims=(128, 128, 64)):
                 avg_launch_ms=v.kernel_ms / max(v.kernel_launches, 1),
                 hbm_gbs_algorithmic=vi * bytes_per_vi / (v.kernel_ms * 1e-3) / 1e9,
                 loss=v.loss, loss_reg=v.loss_reg)
+
+
+def bench_tv_small(ctx, device, iterations, dims=(100, 100, 1)):
+    """Launch-bound step form: the 100x100 mono-exponential slice with 2D TV
+    on R2* (one k_askadam_step per iteration, 10,000 voxels), with the
+    iteration blocks replayed from captured hipGraphs and, beside it, with
+    one launch per iteration."""
+    import torch
+
+    from gacelle_amd import _abi, runtime, utils
+    from gacelle_amd.askadam import askadam
+
+    nx, ny, nz = dims
+    nv = nx * ny * nz
+    g = torch.Generator(device=device)
+    g.manual_seed(11)
+    s0 = (torch.rand(nv, generator=g, device=device) + 0.5)
+    r2 = torch.rand(nv, generator=g, device=device) * 50 + 10
+    te = torch.from_numpy(TE.astype(np.float32)).to(device)
+    y = (s0[None] * torch.exp(-te[:, None] * r2[None]))
+    y = (y + float(s0.mean()) / 100 * torch.randn(y.shape, generator=g, device=device)).contiguous()
+    f = askadam.check_set_default_basic(dict(modelParams=["M0", "R2star"], lb=[0, 0.1], ub=[2, 200],
+                                             iteration=iterations, initialLearnRate=1e-3, lossFunction="L1",
+                                             tol=0, convergenceValue=-1, convergenceStepTol=0, convergenceGradTol=0,
+                                             regmap=["R2star"], TVmode="2D", voxelSize=[1, 1, 1], verbose=False,
+                                             **{"lambda": [1e-3]}))
+    md = r2s_model([0, 0.1], [2, 200], 2)
+    cfg = askadam().make_cfg(f, None)
+    vox = torch.from_numpy(utils.find_mask(np.ones(dims, bool))).to(device)
+    z0 = logit_z0(nv, md, device)
+
+    def run(c):
+        z = z0.clone()
+        arr = (_abi.gc_volume * 1)()
+        arr[0].nv = nv
+        arr[0].y_dev = runtime.ptr(y)
+        arr[0].vox_dev = runtime.ptr(vox)
+        for k in range(3):
+            arr[0].dims[k] = dims[k]
+        arr[0].z_dev = runtime.ptr(z)
+        ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(c), arr, 1))
+        return arr[0], z
+
+    warm = _abi.gc_askadam_cfg.from_buffer_copy(cfg)
+    warm.iteration = 70
+    out = {}
+    for name, opt in (("graphs", 1), ("direct", 0)):
+        ctx.set_option(_abi.GC_OPT_GRAPHS, opt)
+        run(warm)
+        t, (v, z) = timed(lambda: run(cfg))
+        out[name] = dict(seconds=t, us_per_iteration=t / iterations * 1e6, kernel_ms=v.kernel_ms, loss=v.loss,
+                         z=z)
+    ctx.set_option(_abi.GC_OPT_GRAPHS, 1)
+    same = bool(torch.equal(out["graphs"].pop("z"), out["direct"].pop("z")))
+    return dict(workload=f"mono-exponential R2* {nx}x{ny}x{nz} ({nv} voxels), 2D TV on R2* lambda 1e-3, "
+                         f"{iterations} iterations L1/Adam, one k_askadam_step per iteration",
+                graphs=out["graphs"], direct=out["direct"], speedup=out["direct"]["seconds"] / out["graphs"]["seconds"],
+                bit_identical=same, voxels_per_s=nv / out["graphs"]["seconds"])
 
 
 def bench_dict(ctx, device, steps, nv=1_000_000, nd=10_000):
@@ -511,8 +569,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mh", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--configs", default="c3,c4,c5,dict",
-                    help="further workloads to measure (c3, c4, c5, dict or none)")
+    ap.add_argument("--configs", default="c3,c4,c5,tvsmall,dict",
+                    help="further workloads to measure (c3, c4, c5, tvsmall, dict or none)")
     ap.add_argument("--c3-iterations", type=int, default=2000)
     ap.add_argument("--c4-iterations", type=int, default=500)
     ap.add_argument("--c5-iterations", type=int, default=200)
@@ -611,7 +669,8 @@ def main():
         t_mh = time.perf_counter() - t0
         steps_total = float(nv) * args.mh_chains * args.mh_iterations
         mh = dict(samples_per_s=steps_total / t_mh, retained_samples_per_s=float(nv) * args.mh_chains *
-                  io.n_sample / t_mh, seconds=t_mh, kernel_ms=io.kernel_ms, chains=args.mh_chains,
+                  io.n_sample / t_mh, seconds=t_mh, kernel_ms=io.kernel_ms, block_threads=io.block_threads,
+                  chains=args.mh_chains,
                   iterations=args.mh_iterations, retained_per_chain=int(io.n_sample))
         m = mean.cpu().numpy()
         mh["posterior_mean_median_rel_err"] = dict(M0=float(np.median(np.abs(m[0] - s0) / s0)),
@@ -622,7 +681,8 @@ def main():
     extra = {}
     wanted = [c.strip() for c in args.configs.split(",") if c.strip() and c.strip() != "none"]
     for name, fn, it in (("c3", bench_c3, args.c3_iterations), ("c4", bench_c4, args.c4_iterations),
-                         ("c5", bench_c5, args.c5_iterations), ("dict", bench_dict, args.steps)):
+                         ("c5", bench_c5, args.c5_iterations), ("tvsmall", bench_tv_small, args.iterations),
+                         ("dict", bench_dict, args.steps)):
         if name in wanted:
             extra[name] = fn(ctx, device, it)
             log(f"{name}: {json.dumps(extra[name])}")

@@ -10,7 +10,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 4
+GC_ABI_VERSION = 5
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
@@ -50,11 +50,12 @@ GC_TRANSFORM_SIGMOID, GC_TRANSFORM_LINEAR, GC_TRANSFORM_NONE = 0, 1, 2
 GC_CONV_EMA, GC_CONV_LINEAR = 0, 1
 GC_ADAM_EFFECTIVE_LR, GC_ADAM_KINGMA_BA = 0, 1
 GC_TV_2D, GC_TV_3D = 0, 1
-GC_REG_TV, GC_REG_PRIOR = 0, 1
+GC_REG_TV, GC_REG_PRIOR, GC_REG_SURFACE_TV = 0, 1, 2
 GC_STOP_NONE, GC_STOP_CONVERGENCE, GC_STOP_TOL, GC_STOP_STEP, GC_STOP_GRAD = 0, 1, 2, 3, 4
 GC_MCMC_MH, GC_MCMC_GW = 0, 1
 GC_ENSEMBLE_SIMULTANEOUS, GC_ENSEMBLE_REDBLACK = 0, 1
 GC_RNG_NATIVE, GC_RNG_REPLAY = 0, 1
+GC_OPT_GRAPHS, GC_OPT_MCMC_BLOCK = 1, 2
 
 STOP_MESSAGES = {
     GC_STOP_NONE: "",
@@ -155,6 +156,10 @@ class gc_volume(ctypes.Structure):
         ("iters_per_launch", c_int32),
         ("prior_dev", c_void_p),
         ("reg_offset", c_double),
+        ("surf_nbr_dev", c_void_p),
+        ("surf_len_dev", c_void_p),
+        ("surf_max_nb", c_int32),
+        ("reserved1", c_int32),
     ]
 
 
@@ -203,6 +208,7 @@ class gc_mcmc_io(ctypes.Structure):
         ("n_sample", c_int32),
         ("n_stored", c_int32),
         ("kernel_ms", c_double),
+        ("block_threads", c_int32),
     ]
 
 
@@ -213,6 +219,7 @@ SIGNATURES = {
     "gc_ctx_destroy": ([c_void_p], c_int32),
     "gc_ctx_set_stream": ([c_void_p, c_void_p], c_int32),
     "gc_ctx_synchronize": ([c_void_p], c_int32),
+    "gc_ctx_set_option": ([c_void_p, c_int32, c_int64], c_int32),
     "gc_last_error": ([c_void_p], ctypes.c_char_p),
     "gc_nd_to_gd": ([c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p], c_int32),
     "gc_gd_to_nd": ([c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p], c_int32),

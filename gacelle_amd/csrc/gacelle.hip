@@ -109,6 +109,23 @@ int gc_ctx_synchronize(gc_ctx* ctx) {
 
 const char* gc_last_error(const gc_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
+int gc_ctx_set_option(gc_ctx* ctx, int32_t option, int64_t value) {
+  if (!ctx) return GC_ERR_ARG;
+  switch (option) {
+    case GC_OPT_GRAPHS:
+      if (value != 0 && value != 1) return set_err(ctx, GC_ERR_ARG, "GC_OPT_GRAPHS takes 0 or 1");
+      ctx->use_graphs = int(value);
+      return GC_OK;
+    case GC_OPT_MCMC_BLOCK:
+      if (value != 0 && value != 64 && value != 128 && value != 256)
+        return set_err(ctx, GC_ERR_ARG, "GC_OPT_MCMC_BLOCK takes 0 (autotune), 64, 128 or 256");
+      ctx->mcmc_block = int(value);
+      return GC_OK;
+    default:
+      return set_err(ctx, GC_ERR_ARG, "unknown context option %d", option);
+  }
+}
+
 int gc_nd_to_gd(gc_ctx* ctx, const float* nd, int64_t nvox, int32_t M, const int64_t* vox, int64_t nv, float* gd) {
   if (!ctx || !nd || !vox || !gd || nv < 0 || M < 0) return set_err(ctx, GC_ERR_ARG, "gc_nd_to_gd: bad argument");
   if (nv * M == 0) return GC_OK;
@@ -541,6 +558,70 @@ static int mcmc_global_run(gc_ctx* ctx, const ModelOps* ops, const gc_model* md,
   return rc;
 }
 
+// Threads per block of the MH / GW sampler for this model variant and chain
+// layout: each candidate (64, 128, 256 threads, at least one voxel's chains
+// per block) runs a shortened copy of the call, with every output pointer
+// null, on the first voxels, and the fastest is kept in the context.  The
+// trial length doubles from 4 iterations until the 256-thread run takes a
+// millisecond (or the call's own iteration count is reached).  Samples do
+// not depend on the block size: the draws are indexed by (iteration, chain,
+// voxel) and the per-voxel sums run in a fixed order.
+static int tune_mcmc_block(gc_ctx* ctx, const ModelOps* ops, const gc_model* md, const gc_model* dmd, bool gw,
+                           bool hw, const McmcParams& P, hipEvent_t ev0, hipEvent_t ev1, int* bs_out) {
+  const int gsz = P.rep * (gw ? P.n_walker : 1);
+  const std::array<int64_t, 9> key{md->model_id, md->flags, md->n_meas, md->n_par, gw ? 1 : 0, gsz,
+                                   hw ? 1 : 0,    P.rng,     P.ens};
+  auto it = ctx->mcmc_tuned.find(key);
+  if (it != ctx->mcmc_tuned.end()) {
+    *bs_out = it->second;
+    return GC_OK;
+  }
+  std::vector<int> cand;
+  for (int b : {64, 128, 256})
+    if (b >= gsz) cand.push_back(b);
+  if (cand.empty()) return set_err(ctx, GC_ERR_UNSUPPORTED, "Nwalker*repetition must be <= %d", kBlock);
+  int best = cand.back();
+  if (cand.size() > 1 && P.iteration > 0) {
+    McmcParams T = P;
+    T.posterior = T.mean = T.stdv = T.accept = T.xfinal = nullptr;
+    T.trace = nullptr;
+    // about 2^20 chains: enough blocks to fill the GPU at every candidate;
+    // strides by T.nv keep every read inside the call's buffers
+    T.nv = std::min<int64_t>(P.nv, std::max<int64_t>(1, (int64_t(1) << 20) / gsz));
+    auto timed = [&](int b, float* ms) -> int {
+      int e = ops->mcmc(ctx, dmd, gw, hw, T, b, ev0, ev1);
+      if (e) return e;
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      GC_HIP(ctx, hipEventElapsedTime(ms, ev0, ev1));
+      return GC_OK;
+    };
+    float ms = 0.f;
+    T.iteration = std::min(P.iteration, 4);
+    int e = timed(cand.back(), &ms);  // also loads the kernel
+    if (e) return e;
+    while (ms < 1.0f && T.iteration < P.iteration) {
+      T.iteration = std::min(P.iteration, T.iteration * 2);
+      e = timed(cand.back(), &ms);
+      if (e) return e;
+    }
+    float best_ms = ms;
+    for (int b : cand) {
+      if (b == cand.back()) continue;
+      e = timed(b, &ms);  // warm-up of this launch shape
+      if (e) return e;
+      e = timed(b, &ms);
+      if (e) return e;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = b;
+      }
+    }
+  }
+  ctx->mcmc_tuned[key] = best;
+  *bs_out = best;
+  return GC_OK;
+}
+
 int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc_io* io) {
   if (!ctx) return GC_ERR_ARG;
   StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
@@ -587,7 +668,10 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
     if (ng) rc = zero_unfilled(ctx, io->g_posterior_dev, ng, cfg->repetition, ns, io->n_stored, size_t(W));
     if (rc) return rc;
   }
-  if (cfg->global_mask != 0) return mcmc_global_run(ctx, ops, md, cfg, io, ns, io->n_stored);
+  if (cfg->global_mask != 0) {
+    io->block_threads = kBlock;
+    return mcmc_global_run(ctx, ops, md, cfg, io, ns, io->n_stored);
+  }
   McmcParams P{};
   P.y = io->y_dev;
   P.w = io->w_dev;
@@ -649,12 +733,14 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
       P.partners = io->partners_dev;
     }
   }
-  const int gpb = kBlock / gsz;
-  const unsigned grid = unsigned((io->nv + gpb - 1) / gpb);
   hipEvent_t ev0, ev1;
   GC_HIP(ctx, hipEventCreate(&ev0));
   GC_HIP(ctx, hipEventCreate(&ev1));
-  rc = ops->mcmc(ctx, dmd, gw, io->w_dev != nullptr, P, grid, ev0, ev1);
+  int bs = ctx->mcmc_block;
+  if (bs == 0) rc = tune_mcmc_block(ctx, ops, md, dmd, gw, io->w_dev != nullptr, P, ev0, ev1, &bs);
+  if (!rc && bs < gsz) rc = set_err(ctx, GC_ERR_ARG, "GC_OPT_MCMC_BLOCK %d is below Nwalker*repetition = %d", bs, gsz);
+  io->block_threads = bs;
+  if (!rc) rc = ops->mcmc(ctx, dmd, gw, io->w_dev != nullptr, P, bs, ev0, ev1);
   if (rc) {
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);

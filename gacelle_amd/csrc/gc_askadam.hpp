@@ -60,6 +60,9 @@ struct VolRun {
 // regularisation, per-voxel mean loss over the convergence set
 constexpr int kSum = kStepSums;
 
+// step launches per captured graph (a chunk of an iteration block)
+constexpr int kGraphSteps = 64;
+
 // MATLAB median of a vector (mean of the two middle values for even n)
 inline double median_of(std::vector<double>& v) {
   const size_t n = v.size();
@@ -103,9 +106,43 @@ struct AskDriver {
   double kernel_ms = 0.0;
   int kernel_launches = 0;
 
+  // Step form: graphs of n consecutive k_askadam_step launches, captured
+  // once per call and size, then replayed for every chunk of n iterations.
+  // Launch j of a graph reads descriptor row j of d_steps; chunks after the
+  // first copy their rows to the front of d_steps (device to device, in
+  // stream order) before their replay.  A graph holds kernel nodes only.
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  hipStream_t cap_stream = nullptr;  // capture only: nothing executes on it
+
   ~AskDriver() {
     for (auto e : ev_start) (void)hipEventDestroy(e);
     for (auto e : ev_stop) (void)hipEventDestroy(e);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  }
+
+  // the graph of n launches, capturing it on first use; launch_k(j, stream)
+  // issues launch j
+  template <class LaunchK>
+  int step_graph(int n, LaunchK&& launch_k, hipGraphExec_t* out) {
+    for (auto& g : graphs)
+      if (g.first == n) {
+        *out = g.second;
+        return GC_OK;
+      }
+    if (!cap_stream) GC_HIP(ctx, hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+    GC_HIP(ctx, hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < n; ++j) launch_k(j, cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cap_stream, &g);
+    if (ec != hipSuccess || !g) return set_err(ctx, GC_ERR_HIP, "step graph capture: %s", hipGetErrorString(ec));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return set_err(ctx, GC_ERR_HIP, "step graph instantiate: %s", hipGetErrorString(ei));
+    graphs.push_back({n, ex});
+    *out = ex;
+    return GC_OK;
   }
 
   // accumulate the durations of recorded launches (after a stream sync)
@@ -507,6 +544,37 @@ struct AskDriver {
     }
     for (int d = 0; d < 3; ++d) sp.inv_vs[d] = float(1.0 / cfg.voxel_size[d]);
     sp.o = opt_params();
+    if (ctx->use_graphs && nmax >= 2) {
+      StepVol* const d0 = d_steps.as<StepVol>();
+      auto launch_k = [&](int j, hipStream_t s) {
+        StepParams spj = sp;
+        spj.vols = d0 + size_t(j) * n_vol;
+        if (cfg.transform == GC_TRANSFORM_LINEAR)
+          hipLaunchKernelGGL((k_askadam_step<Model, LOSS, HAS_W, GC_TRANSFORM_LINEAR>), dim3(total_blk),
+                             dim3(kBlock), 0, s, dmd, spj);
+        else
+          hipLaunchKernelGGL((k_askadam_step<Model, LOSS, HAS_W, GC_TRANSFORM_SIGMOID>), dim3(total_blk),
+                             dim3(kBlock), 0, s, dmd, spj);
+      };
+      const int G = std::min(nmax, kGraphSteps);
+      hipEvent_t ev_a, ev_b;
+      int rce = next_events(ev_a, ev_b);
+      if (rce) return rce;
+      GC_HIP(ctx, hipEventRecord(ev_a, ctx->stream));
+      for (int c0 = 0; c0 < nmax; c0 += G) {
+        const int n = std::min(G, nmax - c0);
+        hipGraphExec_t ex = nullptr;
+        rce = step_graph(n, launch_k, &ex);
+        if (rce) return rce;
+        if (c0 > 0)
+          GC_HIP(ctx, hipMemcpyAsync(d0, d0 + size_t(c0) * n_vol, sizeof(StepVol) * size_t(n) * n_vol,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+        GC_HIP(ctx, hipGraphLaunch(ex, ctx->stream));
+        kernel_launches += n;
+      }
+      GC_HIP(ctx, hipEventRecord(ev_b, ctx->stream));
+      return reduce_partials(n_ep);
+    }
     for (int k = 0; k < nmax; ++k) {
       sp.vols = d_steps.as<StepVol>() + size_t(k) * n_vol;
       hipEvent_t ev_a, ev_b;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -28,6 +29,11 @@ struct gc_ctx {
   bool own_stream = false;
   std::string err;
   gc_model* d_model = nullptr;  // device copy of the last model description
+  int use_graphs = 1;           // GC_OPT_GRAPHS: step-form blocks replayed from hipGraphs
+  int mcmc_block = 0;           // GC_OPT_MCMC_BLOCK: MCMC threads per block (0 = autotuned)
+  // sampler block sizes picked by the autotuner, per model variant and
+  // chain layout
+  std::map<std::array<int64_t, 9>, int> mcmc_tuned;
 };
 
 
@@ -301,7 +307,8 @@ struct ModelOps {
   // the askadam loop over a batch of volumes
   int (*askadam)(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol);
   // one MH or GW launch (kernel-only timing through ev0/ev1)
-  int (*mcmc)(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
+  // one MH / GW sampler launch with bs threads per block
+  int (*mcmc)(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, int bs,
               hipEvent_t ev0, hipEvent_t ev1);
   // one launch of the GW sampler with global parameters: kind 0 = initial
   // ensemble, 1 = iteration step (S), 2 = final statistics

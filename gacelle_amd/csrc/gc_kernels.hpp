@@ -923,11 +923,11 @@ GC_DEV float model_sos(const gc_model& md, const float* th, const float* ex, con
 // the voxel owning `gsz` consecutive threads of the block.
 template <int Q>
 GC_DEV void group_stats(const McmcParams& P, const double* s1, const double* s2, int gsz, int64_t v, bool active,
-                        const float* xref, double n_total, double* sh) {
-  // sh: [Q][2][kBlock]
+                        const float* xref, double n_total, double* sh, int bs) {
+  // sh: [Q][2][bs]
   for (int q = 0; q < Q; ++q) {
-    sh[(q * 2 + 0) * kBlock + threadIdx.x] = active ? s1[q] : 0.0;
-    sh[(q * 2 + 1) * kBlock + threadIdx.x] = active ? s2[q] : 0.0;
+    sh[(q * 2 + 0) * bs + threadIdx.x] = active ? s1[q] : 0.0;
+    sh[(q * 2 + 1) * bs + threadIdx.x] = active ? s2[q] : 0.0;
   }
   __syncthreads();
   const int g0 = (threadIdx.x / gsz) * gsz;
@@ -935,8 +935,8 @@ GC_DEV void group_stats(const McmcParams& P, const double* s1, const double* s2,
     for (int q = 0; q < Q; ++q) {
       double a = 0.0, b = 0.0;
       for (int i = 0; i < gsz; ++i) {
-        a += sh[(q * 2 + 0) * kBlock + g0 + i];
-        b += sh[(q * 2 + 1) * kBlock + g0 + i];
+        a += sh[(q * 2 + 0) * bs + g0 + i];
+        b += sh[(q * 2 + 1) * bs + g0 + i];
       }
       if (P.mean) P.mean[int64_t(q) * P.nv + v] = float(double(xref[q]) + a / n_total);
       if (P.stdv) {
@@ -947,8 +947,23 @@ GC_DEV void group_stats(const McmcParams& P, const double* s1, const double* s2,
   }
 }
 
+// Dynamic LDS of the samplers for a block of bs threads: the per-lane FP64
+// sums of group_stats [2Q][bs], the acceptance counts [bs] and, for a
+// Goodman-Weare ensemble that spans waves, the walker exchange [Q][bs].
+constexpr size_t mcmc_lds_bytes(int Q, int bs, bool exchange) {
+  return size_t(2) * Q * bs * sizeof(double) + size_t(bs) * sizeof(int) +
+         (exchange ? size_t(Q) * bs * sizeof(float) : 0);
+}
+
+// A voxel's walker group fits in one wavefront (and never straddles two):
+// walker positions are exchanged by cross-lane permutes, with no LDS and no
+// block barrier.
+__host__ __device__ inline bool gw_wave_local(int gsz) { return gsz <= 64 && (64 % gsz) == 0; }
+
 // Metropolis-Hastings, utils/mcmc.m:82-190.  Thread layout: voxel-major,
-// the `rep` chains (repetitions) of a voxel in consecutive lanes.
+// the `rep` chains (repetitions) of a voxel in consecutive lanes.  The
+// block size (64, 128 or 256 threads) is a launch choice; results do not
+// depend on it.
 template <class Model, bool HAS_W>
 __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp, const McmcParams P) {
   const gc_model& md = *mdp;
@@ -956,7 +971,8 @@ __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp,
   constexpr int Q = NP + 1;
   constexpr int MR = Model::MR;
   const int R = P.rep;
-  const int gpb = kBlock / R;  // voxels per block
+  const int bs = blockDim.x;
+  const int gpb = bs / R;  // voxels per block
   const int tl = threadIdx.x;
   const int gi = tl / R, r = tl % R;
   const int64_t v = int64_t(blockIdx.x) * gpb + gi;
@@ -1064,10 +1080,11 @@ __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp,
     s2[q] += d * d * double(P.n_sample - filled);
     if (P.xfinal != nullptr && active) P.xfinal[(int64_t(r) * Q + q) * nv + v] = x[q];
   }
-  __shared__ double sh[2 * Q * kBlock];
-  __shared__ int shacc[kBlock];
+  extern __shared__ double mh_lds[];  // mcmc_lds_bytes(Q, bs, false)
+  double* sh = mh_lds;
+  int* shacc = reinterpret_cast<int*>(mh_lds + 2 * Q * bs);
   shacc[tl] = active ? nacc : 0;
-  group_stats<Q>(P, s1, s2, R, v, active && gi < gpb, x0, double(P.n_sample) * R, sh);
+  group_stats<Q>(P, s1, s2, R, v, active && gi < gpb, x0, double(P.n_sample) * R, sh, bs);
   if (active && r == 0 && P.accept != nullptr) {
     int t = 0;
     for (int i = 0; i < R; ++i) t += shacc[tl + i];
@@ -1086,7 +1103,9 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
   constexpr int MR = Model::MR;
   const int W = P.n_walker, R = P.rep;
   const int gsz = W * R;
-  const int gpb = kBlock / gsz;
+  const int bs = blockDim.x;
+  const int gpb = bs / gsz;
+  const bool wave_local = gw_wave_local(gsz);
   const int tl = threadIdx.x;
   const int gi = tl / gsz, gl = tl % gsz;
   const int r = gl / W, wk = gl % W;
@@ -1148,7 +1167,10 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
   const int M = md.n_meas;
   float lp = logp_gaussian(model_sos<Model, HAS_W>(md, x, ex, yr, wr), x[Q - 1], M);  // :275
 
-  __shared__ float shx[Q][kBlock];
+  extern __shared__ double gw_lds[];  // mcmc_lds_bytes(Q, bs, !wave_local)
+  double* sh = gw_lds;
+  int* shacc = reinterpret_cast<int*>(gw_lds + 2 * Q * bs);
+  float* shx = reinterpret_cast<float*>(shacc + bs);  // [Q][bs], exchange across waves
   double s1[Q], s2[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) { s1[q] = 0.0; s2[q] = 0.0; }
@@ -1161,9 +1183,11 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
     const int32_t* part = P.partners + step * W;
     const int nsub = rb ? 2 : 1;
     for (int sub = 0; sub < nsub; ++sub) {
+      if (!wave_local) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) shx[q][tl] = x[q];
-      __syncthreads();
+        for (int q = 0; q < Q; ++q) shx[q * bs + tl] = x[q];
+        __syncthreads();
+      }
       bool mine = true;
       int j = wk, pw;
       if (rb) {
@@ -1194,14 +1218,16 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
       const float zz = t * t / P.step_size;
       float xp[Q];
       bool oob = false;
+      // partner walker pw of this (voxel, repetition): lane base + pw
+      const int src = wave_local ? ((tl & 63) - gl + r * W + pw) : 0;
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const float xa = shx[q][base + pw];
+        const float xa = wave_local ? __shfl(x[q], src, 64) : shx[q * bs + base + pw];
         float val = xa + (x[q] - xa) * zz;
         oob = oob || (val < lbq[q]) || (val > ubq[q]);
         xp[q] = fminf(fmaxf(val, lbq[q]), ubq[q]);
       }
-      __syncthreads();
+      if (!wave_local) __syncthreads();
       if (mine) {
         const float lpp = logp_gaussian(model_sos<Model, HAS_W>(md, xp, ex, yr, wr), xp[Q - 1], M);
         // min(zz.^(Nvar-1).*exp(logPProposed-logPCurr),1) > rand (:313-315)
@@ -1241,10 +1267,8 @@ __global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp,
     s2[q] += d * d * double(P.n_sample - filled);
     if (P.xfinal != nullptr && active) P.xfinal[((int64_t(r) * W + wk) * Q + q) * nv + v] = x[q];
   }
-  __shared__ double sh[2 * Q * kBlock];
-  __shared__ int shacc[kBlock];
   shacc[tl] = active ? nacc : 0;
-  group_stats<Q>(P, s1, s2, gsz, v, active, xc0, double(P.n_sample) * R * W, sh);
+  group_stats<Q>(P, s1, s2, gsz, v, active, xc0, double(P.n_sample) * R * W, sh, bs);
   if (active && gl == 0 && P.accept != nullptr) {
     int t = 0;
     for (int i = 0; i < gsz; ++i) t += shacc[tl + i];

@@ -62,24 +62,33 @@ struct OpsImpl {
     });
   }
 
-  static int mcmc(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
+  // one sampler launch with bs threads per block (bs / (rep * walkers)
+  // voxels per block)
+  static int mcmc(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, int bs,
                   hipEvent_t ev0, hipEvent_t ev1) {
     if constexpr (Model::kStreamOnly) {
       // the samplers keep a voxel's data in registers
       return set_err(ctx, GC_ERR_UNSUPPORTED, "MCMC is not available for this model");
     } else {
-      return mcmc_launch(ctx, dmd, gw, hw, P, grid, ev0, ev1);
+      return mcmc_launch(ctx, dmd, gw, hw, P, bs, ev0, ev1);
     }
   }
 
-  static int mcmc_launch(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, unsigned grid,
+  static int mcmc_launch(gc_ctx* ctx, const gc_model* dmd, bool gw, bool hw, const McmcParams& P, int bs,
                          hipEvent_t ev0, hipEvent_t ev1) {
+    constexpr int Q = Model::NP + 1;
+    const int gsz = P.rep * (gw ? P.n_walker : 1);
+    if (bs % 64 != 0 || bs > kBlock || gsz > bs)
+      return set_err(ctx, GC_ERR_ARG, "sampler block of %d threads for groups of %d", bs, gsz);
+    const int64_t gpb = bs / gsz;
+    const unsigned grid = unsigned((P.nv + gpb - 1) / gpb);
+    const size_t lds = mcmc_lds_bytes(Q, bs, gw && !gw_wave_local(gsz));
     return with_bool(hw, [&](auto hwc) -> int {
       constexpr bool HW = decltype(hwc)::value;
       if (gw)
-        hipExtLaunchKernelGGL((k_gw<Model, HW>), dim3(grid), dim3(kBlock), 0, ctx->stream, ev0, ev1, 0, dmd, P);
+        hipExtLaunchKernelGGL((k_gw<Model, HW>), dim3(grid), dim3(bs), lds, ctx->stream, ev0, ev1, 0, dmd, P);
       else
-        hipExtLaunchKernelGGL((k_mh<Model, HW>), dim3(grid), dim3(kBlock), 0, ctx->stream, ev0, ev1, 0, dmd, P);
+        hipExtLaunchKernelGGL((k_mh<Model, HW>), dim3(grid), dim3(bs), lds, ctx->stream, ev0, ev1, 0, dmd, P);
       GC_HIP(ctx, hipGetLastError());
       return GC_OK;
     });

@@ -54,6 +54,10 @@ class Context:
     def synchronize(self):
         self.check(self.lib.gc_ctx_synchronize(self.handle))
 
+    def set_option(self, option: int, value: int):
+        """gc_ctx_set_option: GC_OPT_GRAPHS (0/1) or GC_OPT_MCMC_BLOCK (0 = autotune, 64, 128, 256)."""
+        self.check(self.lib.gc_ctx_set_option(self.handle, int(option), int(value)))
+
     # ---- buffers ----------------------------------------------------------
     def to_device(self, a, dtype=np.float32):
         torch = _torch()

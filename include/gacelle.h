@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 4
+#define GC_ABI_VERSION 5
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
@@ -133,9 +133,10 @@ enum { GC_CONV_EMA = 0, GC_CONV_LINEAR = 1 };
 enum { GC_ADAM_EFFECTIVE_LR = 0, GC_ADAM_KINGMA_BA = 1 };
 enum { GC_TV_2D = 0, GC_TV_3D = 1 };
 /* regulariser of the regmap parameters: spatial total variation
- * (utils/spatial_total_variation.m:24-66) or the normal prior of gpuNEXI
- * (prior_distribution_normal, NEXI/gpuNEXI.m:1256-1303) */
-enum { GC_REG_TV = 0, GC_REG_PRIOR = 1 };
+ * (utils/spatial_total_variation.m:24-66), the normal prior of gpuNEXI
+ * (prior_distribution_normal, NEXI/gpuNEXI.m:1256-1303) or the mesh total
+ * variation of surface data (surface_total_variation, NEXI/gpuNEXI.m:1165-1254) */
+enum { GC_REG_TV = 0, GC_REG_PRIOR = 1, GC_REG_SURFACE_TV = 2 };
 enum {
   GC_STOP_NONE = 0,        /* ran all iterations                      */
   GC_STOP_CONVERGENCE = 1, /* loss convergence below tolerance        */
@@ -160,7 +161,8 @@ typedef struct gc_askadam_cfg {
   int32_t n_reg;              /* number of regularised maps (0 = none)  */
   int32_t regmap[GC_MAX_REG]; /* parameter slot of each regularised map */
   int32_t robust;             /* fitting.robustConvergence              */
-  int32_t reg_type;           /* GC_REG_TV | GC_REG_PRIOR (fitting.regularisationType) */
+  int32_t reg_type;           /* GC_REG_* (fitting.regularisationType,
+                                 fitting.dataType 'surface' with 'TV')   */
   double lr0;                 /* fitting.initialLearnRate               */
   double decay;               /* fitting.decayRate                      */
   double beta1, beta2, eps;   /* adamupdate decays and epsilon          */
@@ -227,6 +229,17 @@ typedef struct gc_volume {
    * constant added to loss_reg every iteration. */
   const float* prior_dev;
   double reg_offset;
+  /* GC_REG_SURFACE_TV: the vertex grid is dims = {1, Nvertex, Nhemi} and
+   * vox_dev the grid index of each masked vertex.  surf_nbr_dev is
+   * [surf_max_nb x Nvertex*Nhemi] of global 0-based vertex indices
+   * (the reference's neighbours + hemiOffset - 1, same layout) and
+   * surf_len_dev the matching edge lengths.  loss_reg = sum_k lambda_k / nv
+   * * sum over masked vertices i and slots n of the finite
+   * |(theta_k(i) - theta_k(nbr)) / len|, theta_k zero outside the mask. */
+  const int32_t* surf_nbr_dev;
+  const float* surf_len_dev;
+  int32_t surf_max_nb;
+  int32_t reserved1;
 } gc_volume;
 
 /* ---- MCMC configuration: POD mirror of mcmc.check_set_default_basic
@@ -306,6 +319,8 @@ typedef struct gc_mcmc_io {
   int32_t n_sample;          /* out: Ns = numel(Nburnin+1:thinning:iteration) */
   int32_t n_stored;          /* out: samples actually stored per chain    */
   double kernel_ms;          /* out: sampler kernel duration (HIP events) */
+  int32_t block_threads;     /* out: threads per block of the MH / GW launch
+                              * (GC_OPT_MCMC_BLOCK or the autotuner's pick) */
 } gc_mcmc_io;
 
 /* ---- context ------------------------------------------------------------ */
@@ -323,6 +338,20 @@ int gc_ctx_destroy(gc_ctx* ctx);
 int gc_ctx_set_stream(gc_ctx* ctx, void* hip_stream);
 int gc_ctx_synchronize(gc_ctx* ctx);
 const char* gc_last_error(const gc_ctx* ctx);
+
+/* Context options (no counterpart in the reference: MATLAB's dlaccelerate
+ * trace cache, utils/askadam.m:192, is the nearest analogue of
+ * GC_OPT_GRAPHS).
+ *  GC_OPT_GRAPHS      1 (default): the per-iteration launches of the step
+ *                     form (TV, prior, robustConvergence, streamed models)
+ *                     are captured in hipGraphs of up to 64 launches and
+ *                     replayed per iteration block; 0: launched one by one.
+ *  GC_OPT_MCMC_BLOCK  threads per block of the MCMC samplers: 0 (default)
+ *                     = picked per model and chain layout by a short timed
+ *                     trial on the first call (cached in the context), or
+ *                     64, 128 or 256.  Results do not depend on it. */
+enum { GC_OPT_GRAPHS = 1, GC_OPT_MCMC_BLOCK = 2 };
+int gc_ctx_set_option(gc_ctx* ctx, int32_t option, int64_t value);
 
 /* ---- layout helpers (utils/utils.m:157-163, 194-198, 299-315) ----------- */
 /* G-D gather: gd[m*nv+v] = nd[vox[v] + nvox_grid*m]; nd is column-major

@@ -159,7 +159,8 @@ def mcmc(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partne
     c.check(c.lib.gc_mcmc_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), ctypes.byref(io)))
     return dict(posterior=None if post is None else runtime.to_host(post), mean=runtime.to_host(mean),
                 std=runtime.to_host(std), accept=runtime.to_host(acc), xfinal=runtime.to_host(xf),
-                trace=None if tr is None else runtime.to_host(tr), n_sample=io.n_sample, n_stored=io.n_stored)
+                trace=None if tr is None else runtime.to_host(tr), n_sample=io.n_sample, n_stored=io.n_stored,
+                block_threads=io.block_threads, kernel_ms=io.kernel_ms)
 
 
 def mcmc_global(md, cfg, y, x0, w=None, extra=None, normals=None, uniforms=None, partners=None, g_uniforms=None,

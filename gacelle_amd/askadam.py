@@ -226,6 +226,12 @@ class askadam:
                 pr_d = ctx.to_device(rows)
                 v.prior_dev = runtime.ptr(pr_d)
                 keep.append(pr_d)
+            stv = _surface_tables(fitting, extra_data, mask)
+            if stv is not None:
+                sn_d, sl_d = ctx.to_device(stv[0], np.int32), ctx.to_device(stv[1])
+                v.surf_nbr_dev, v.surf_len_dev = runtime.ptr(sn_d), runtime.ptr(sl_d)
+                v.surf_max_nb = stv[0].shape[0]
+                keep += [sn_d, sl_d]
             keep += [y_d, w_d, ex_d, vox_d, z_d, zmin_d, res_d, resmin_d, rl_d, rlmin_d]
             metas.append(dict(mask=mask, vox=vox, nv=nv, nmeas=nmeas, z=z_d, zmin=zmin_d, res=res_d, resmin=resmin_d,
                               rl=rl_d, rlmin=rlmin_d, trace=trace, theta0=theta0))
@@ -279,7 +285,7 @@ class askadam:
                 c.regmap[k] = slots[name].start if slots is not None else names.index(name)
                 c.lambda_[k] = lam[k]
         c.reg_type = _abi.GC_REG_PRIOR if str(fitting.get("regularisationType", "TV")).lower() == "prior" \
-            else _abi.GC_REG_TV
+            else (_abi.GC_REG_SURFACE_TV if _is_surface_tv(fitting) else _abi.GC_REG_TV)
         c.robust = int(bool(fitting["robustConvergence"]))
         c.lr0 = float(fitting["initialLearnRate"])
         c.decay = float(fitting["decayRate"])
@@ -374,6 +380,31 @@ def _resolve_model(userfuncCell):
         return f.__self__
     raise GacelleError(_abi.GC_ERR_UNSUPPORTED,
                        "the fused GPU path needs one of this package's model classes as the forward model")
+
+
+def _is_surface_tv(fitting) -> bool:
+    lam = [float(x) for x in np.atleast_1d(fitting.get("lambda", [0]))]
+    return (str(fitting.get("dataType", "volumetric")).lower() == "surface"
+            and str(fitting.get("regularisationType", "TV")).lower() == "tv" and bool(lam) and lam[0] > 0)
+
+
+def _surface_tables(fitting, extra_data, mask):
+    """The mesh TV regulariser of surface data (gpuNEXI.surface_total_variation,
+    NEXI/gpuNEXI.m:1165-1254; checks of :416-429): extradata['neighbours']
+    (1-based per-hemisphere indices, [1, Nvertex, Nhemi, maxNb]) and the edge
+    lengths, extradata['dr'] as gpuNEXI.fit passes them (or
+    extradata['edgeLength']), converted to the C ABI's grid-form tables; or
+    None when the fit has no surface TV."""
+    if not _is_surface_tv(fitting):
+        return None
+    ex = extra_data if isinstance(extra_data, dict) else {}
+    el = ex.get("edgeLength", ex.get("dr"))
+    if ex.get("neighbours") is None or el is None:
+        raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:missingSurfaceGeometry: fitting.dataType is 'surface' and "
+                           "fitting.lambda > 0, but extradata.neighbours/extradata.dr were not provided to fit().")
+    from .surface import surface_tv_tables
+
+    return surface_tv_tables(mask, ex["neighbours"], el)
 
 
 def _prior_rows(fitting, extra_data, mask, vox):

@@ -46,6 +46,8 @@ struct VolRun {
   int chunks = 1;       // chunks per partial-sum row (k_reduce_rows)
   DevBuf pp[2][3];      // step form: intermediate states of a launch sequence
   DevBuf nbr;           // step form with TV: [2*nd][nv] grid neighbours
+  DevBuf snb, sie, srv, srie;  // surface TV tables (StepVol)
+  int snf = 0, snr = 0;
   // robustConvergence (utils/askadam.m:448-510, 541-566)
   std::vector<double> pvl_init, pvl, hist;  // per-voxel mean loss; history [W][nv]
   std::vector<uint8_t> main_mask, mask_conv;
@@ -94,6 +96,7 @@ struct AskDriver {
   bool use_reg = false;  // a regulariser runs (n_reg > 0, lambda[0] > 0)
   bool prior = false;    // ... the normal prior
   bool use_tv = false;   // ... spatial total variation
+  bool use_stv = false;  // ... surface (mesh) total variation
   bool robust = false;
   bool replaying = false;       // a replay launch: no per-voxel loss outputs
   bool replay_pinned = false;   // ... from the pinned block (its weight modulator)
@@ -186,10 +189,11 @@ struct AskDriver {
   int setup(bool stream_model) {
     use_reg = cfg.n_reg > 0 && cfg.lambda[0] > 0;
     prior = use_reg && cfg.reg_type == GC_REG_PRIOR;
-    use_tv = use_reg && !prior;
+    use_tv = use_reg && cfg.reg_type == GC_REG_TV;
+    use_stv = use_reg && cfg.reg_type == GC_REG_SURFACE_TV;
     robust = cfg.robust != 0;
     step_mode = use_reg || stream_model || robust;
-    if (use_reg && cfg.reg_type != GC_REG_TV && cfg.reg_type != GC_REG_PRIOR)
+    if (use_reg && !prior && !use_tv && !use_stv)
       return set_err(ctx, GC_ERR_ARG, "unknown regulariser type %d", cfg.reg_type);
     if (robust && (cfg.weight_update_interval < 1 || cfg.outlier_check_window < 1 || cfg.convergence_window < 1))
       return set_err(ctx, GC_ERR_ARG, "robustConvergence needs weightUpdateInterval, outlierCheckWindow and "
@@ -261,6 +265,10 @@ struct AskDriver {
           for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.pp[s][j].alloc(pbytes));
       if (use_tv) {
         const int rc = build_neighbours(i);
+        if (rc) return rc;
+      }
+      if (use_stv) {
+        const int rc = build_surface(i);
         if (rc) return rc;
       }
       if (robust) {
@@ -340,6 +348,83 @@ struct AskDriver {
     GC_HIP(ctx, hipGetLastError());
     GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inv is freed on return
     return GC_OK;
+  }
+
+  // Mesh tables of a surface volume (GC_REG_SURFACE_TV), built on the host
+  // from the caller's grid-form neighbours and edge lengths.  Entries whose
+  // term is zero whatever theta is are left out: a vertex listing itself
+  // (theta_i - theta_i = 0), and edge lengths whose reciprocal is zero or
+  // not finite (0 -> the reference zeroes the non-finite quotient; inf -> 0).
+  int build_surface(int i) {
+    const gc_volume& v = vols[i];
+    VolRun& R = run[i];
+    const int64_t nv = v.nv;
+    const int64_t ngrid = int64_t(v.dims[0]) * v.dims[1] * v.dims[2];
+    const int nb = v.surf_max_nb;
+    if (!v.vox_dev || !v.surf_nbr_dev || !v.surf_len_dev || nb < 1 || v.dims[0] < 1 || v.dims[1] < 1 ||
+        v.dims[2] < 1)
+      return set_err(ctx, GC_ERR_ARG, "volume %d: surface TV needs vox_dev, dims, surf_nbr_dev, surf_len_dev and "
+                                      "surf_max_nb >= 1", i);
+    if (ngrid >= (int64_t(1) << 31) || nv > ngrid)
+      return set_err(ctx, GC_ERR_ARG, "volume %d: vertex grid of %lld unsupported", i, (long long)ngrid);
+    std::vector<int64_t> hv(nv);
+    std::vector<int32_t> hn(size_t(nb) * ngrid);
+    std::vector<float> hl(size_t(nb) * ngrid);
+    GC_HIP(ctx, hipMemcpyAsync(hv.data(), v.vox_dev, sizeof(int64_t) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    GC_HIP(ctx, hipMemcpyAsync(hn.data(), v.surf_nbr_dev, sizeof(int32_t) * hn.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    GC_HIP(ctx, hipMemcpyAsync(hl.data(), v.surf_len_dev, sizeof(float) * hl.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> inv(ngrid, -1);
+    for (int64_t k = 0; k < nv; ++k) {
+      if (hv[k] < 0 || hv[k] >= ngrid || (k > 0 && hv[k] <= hv[k - 1]))
+        return set_err(ctx, GC_ERR_ARG, "volume %d: vox must be increasing grid indices (entry %lld)", i,
+                       (long long)k);
+      inv[hv[k]] = int32_t(k);
+    }
+    std::vector<std::vector<std::pair<int32_t, float>>> fw(nv), rv(nv);
+    for (int64_t k = 0; k < nv; ++k) {
+      const int64_t g = hv[k];
+      for (int n = 0; n < nb; ++n) {
+        const int32_t j = hn[size_t(n) * ngrid + g];
+        if (j < 0 || j >= ngrid)
+          return set_err(ctx, GC_ERR_ARG, "volume %d: surface neighbour %d of vertex %lld is outside the grid", i,
+                         j, (long long)g);
+        if (j == g) continue;
+        const double ie = 1.0 / double(hl[size_t(n) * ngrid + g]);
+        if (!std::isfinite(ie) || ie == 0.0) continue;
+        const int32_t u = inv[j];
+        fw[k].push_back({u, float(ie)});
+        if (u >= 0) rv[u].push_back({int32_t(k), float(ie)});
+      }
+    }
+    R.snf = R.snr = 0;
+    for (int64_t k = 0; k < nv; ++k) {
+      R.snf = std::max<int>(R.snf, int(fw[k].size()));
+      R.snr = std::max<int>(R.snr, int(rv[k].size()));
+    }
+    auto upload = [&](const std::vector<std::vector<std::pair<int32_t, float>>>& L, int width, int32_t pad,
+                      DevBuf& di, DevBuf& de) -> int {
+      if (width == 0) return GC_OK;
+      std::vector<int32_t> idx(size_t(width) * nv, pad);
+      std::vector<float> ie(size_t(width) * nv, 0.f);
+      for (int64_t k = 0; k < nv; ++k)
+        for (size_t n = 0; n < L[k].size(); ++n) {
+          idx[n * nv + k] = L[k][n].first;
+          ie[n * nv + k] = L[k][n].second;
+        }
+      GC_HIP(ctx, di.alloc(sizeof(int32_t) * idx.size()));
+      GC_HIP(ctx, de.alloc(sizeof(float) * ie.size()));
+      GC_HIP(ctx, hipMemcpyAsync(di.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+      GC_HIP(ctx, hipMemcpyAsync(de.p, ie.data(), sizeof(float) * ie.size(), hipMemcpyHostToDevice, ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
+      return GC_OK;
+    };
+    int rc = upload(fw, R.snf, -2, R.snb, R.sie);
+    if (rc) return rc;
+    return upload(rv, R.snr, -1, R.srv, R.srie);
   }
 
   EpochScal epoch_scal(int e) const {
@@ -508,6 +593,14 @@ struct AskDriver {
           d.z_last = (k == n - 1 && zl[i] >= 0) ? R.zlast[zl[i]].as<float>() : nullptr;
           d.partials = R.partials.as<float>();
           d.nbr = use_tv ? R.nbr.as<int32_t>() : nullptr;
+          if (use_stv) {
+            d.snb = R.snb.as<int32_t>();
+            d.sie = R.sie.as<float>();
+            d.srv = R.srv.as<int32_t>();
+            d.srie = R.srie.as<float>();
+            d.snf = R.snf;
+            d.snr = R.snr;
+          }
           d.prior = prior ? vols[i].prior_dev : nullptr;
           if (robust) {
             const int e = e0[i] + k;
@@ -537,7 +630,7 @@ struct AskDriver {
     sp.n_vol = n_vol;
     sp.n_reg = use_reg ? cfg.n_reg : 0;
     sp.nd = (cfg.tv_mode == GC_TV_3D) ? 3 : 2;
-    sp.reg_type = prior ? GC_REG_PRIOR : GC_REG_TV;
+    sp.reg_type = prior ? GC_REG_PRIOR : (use_stv ? GC_REG_SURFACE_TV : GC_REG_TV);
     for (int k = 0; k < GC_MAX_REG; ++k) {
       sp.regmap[k] = k < cfg.n_reg ? cfg.regmap[k] : 0;
       sp.lam[k] = k < cfg.n_reg ? float(cfg.lambda[k]) : 0.f;

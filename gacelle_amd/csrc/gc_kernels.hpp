@@ -426,6 +426,15 @@ struct StepVol {
   const float* wmod;   // robustConvergence: per-voxel weight factor of this iteration, or null
   const uint8_t* mconv;  // robustConvergence: convergence voxel set, or null
   float* pvl_out;      // robustConvergence: per-voxel mean loss of this iteration, or null
+  // GC_REG_SURFACE_TV: per masked vertex, its mesh neighbours (G-D index,
+  // -1 outside the mask, -2 ends the list) with 1/edgeLength, [snf][nv],
+  // and the masked vertices listing it as a neighbour (-1 ends the list)
+  // with their 1/edgeLength, [snr][nv]
+  const int32_t* snb;
+  const float* sie;
+  const int32_t* srv;
+  const float* srie;
+  int32_t snf, snr;
   int64_t nv;
   int32_t blk0, nblk;
   int32_t active;      // the volume runs this iteration
@@ -441,7 +450,7 @@ struct StepParams {
   int32_t n_vol;
   int32_t n_reg;       // regularised maps (0: none)
   int32_t nd;          // TV axes: 2 ('2D') or 3 ('3D')
-  int32_t reg_type;    // GC_REG_TV | GC_REG_PRIOR
+  int32_t reg_type;    // GC_REG_TV | GC_REG_PRIOR | GC_REG_SURFACE_TV
   int32_t regmap[4];   // parameter slot of each regularised map
   float lam[4];
   float inv_vs[3];     // 1/voxelSize
@@ -534,7 +543,31 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
         gt = lam * sgnf(zs) / sd;
       }
     }
-    for (int d = 0; d < (sp.reg_type == GC_REG_PRIOR ? 0 : sp.nd); ++d) {
+    if (sp.reg_type == GC_REG_SURFACE_TV) {
+      // mesh TV (gpuNEXI.surface_total_variation): cost sum over i's
+      // neighbours u of |theta_i - theta_u| / len (theta_u = 0 outside the
+      // mask); i's subgradient from its own terms and from the terms of the
+      // masked vertices that list i
+      for (int n = 0; n < V.snf; ++n) {
+        const int32_t u = V.snb[int64_t(n) * nv + vv];
+        if (u == -2) break;
+        const float ie = V.sie[int64_t(n) * nv + vv];
+        float tu = 0.f, dd;
+        if (u >= 0) transform_fwd_t<TR>(V.z_in[int64_t(pr) * nv + u], lb, ub, tu, dd);
+        const float dv = (ti - tu) * ie;
+        reg += lam * fabsf(dv);
+        gt += lam * sgnf(dv) * ie;
+      }
+      for (int n = 0; n < V.snr; ++n) {
+        const int32_t wv = V.srv[int64_t(n) * nv + vv];
+        if (wv < 0) break;
+        const float ie = V.srie[int64_t(n) * nv + vv];
+        float tw, dd;
+        transform_fwd_t<TR>(V.z_in[int64_t(pr) * nv + wv], lb, ub, tw, dd);
+        gt -= lam * sgnf((tw - ti) * ie) * ie;
+      }
+    }
+    for (int d = 0; d < (sp.reg_type == GC_REG_TV ? sp.nd : 0); ++d) {
       const float ivs = sp.inv_vs[d];
       const int32_t jf = V.nbr[int64_t(2 * d) * nv + vv];
       float tf = 0.f, dd;

@@ -687,6 +687,54 @@ class gpuNEXI(_ModelBase):
         with np.errstate(divide="ignore", invalid="ignore"):
             return (zv @ zt) / np.sqrt((zv * zv).sum(axis=1) * (zt @ zt))
 
+    def get_surface_geometry(self, surf_dir, depth, hemisphere):
+        """NEXI/gpuNEXI.m:254-288: utils.get_surface_neighbours, cached on the
+        object per (surf_dir, depth, hemisphere).  Returns (neighbours,
+        edgeLength)."""
+        key = (str(surf_dir), float(depth), tuple(hemisphere))
+        cache = getattr(self, "_surface_cache", None)
+        if cache is None or cache[0] != key:
+            from .surface import get_surface_neighbours
+
+            nb, _, el = get_surface_neighbours(surf_dir, depth, hemisphere)
+            self._surface_cache = cache = (key, nb, el)
+        return cache[1], cache[2]
+
+    def _prepare_surface(self, data, mask, extradata, fitting):
+        """The surface branch of prepare_dwi_data (NEXI/gpuNEXI.m:482-546):
+        [1, Nvertex, Nhemi] layout, the mesh geometry when the TV regulariser
+        is on (extradata surf_dir, hemisphere, optional depth), and the
+        exact-only segmentation counts."""
+        data, mask = np.asarray(data), np.asarray(mask)
+        if mask.shape[0] != 1 or data.shape[0] != 1:
+            raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:surfaceShapeMismatch: surface mode expects data/mask in "
+                               f"[1, Nvertex, Nhemi] convention (singleton 1st dimension), got mask size "
+                               f"{list(utils.dims3(mask))}")
+        nhemi = utils.dims3(mask)[2]
+        extradata = dict(extradata or {})
+        lam = np.atleast_1d(fitting.get("lambda", [0]))
+        if str(fitting.get("regularisationType", "TV")).lower() == "tv" and lam.size and float(lam[0]) > 0:
+            if not extradata.get("surf_dir"):
+                raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:missingSurfDir: fitting.dataType is 'surface' but "
+                                                    "extradata.surf_dir was not provided.")
+            hemi = extradata.get("hemisphere")
+            if hemi is None or len(hemi) == 0:
+                raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:missingHemisphere: fitting.dataType is 'surface' but "
+                                                    "extradata.hemisphere was not provided.")
+            hemi = [hemi] if isinstance(hemi, str) else list(hemi)
+            if len(hemi) != nhemi:
+                raise GacelleError(_abi.GC_ERR_ARG, f"GACELLE:hemisphereMaskMismatch: extradata.hemisphere has "
+                                                    f"{len(hemi)} entries but mask dim3 (Nhemi) = {nhemi}")
+            depth = extradata.get("depth")
+            depth = 0.5 if depth is None or np.size(depth) == 0 else float(depth)
+            nb, el = self.get_surface_geometry(extradata["surf_dir"], depth, hemi)
+            if nb.shape[2] != nhemi:
+                raise GacelleError(_abi.GC_ERR_ARG, f"GACELLE:surfaceGeometryMismatch: computed surface neighbour "
+                                                    f"geometry has {nb.shape[2]} hemisphere(s) but mask has {nhemi}")
+            extradata["neighbours"], extradata["dr"] = nb, el
+        segment.surface_segments(fitting, nhemi)
+        return extradata
+
     def prepare_dwi_data(self, data, mask, extradata, lmax, fitting):
         """NEXI/gpuNEXI.m:480-636 (volumetric data): the rotationally
         invariant signal when the input holds raw diffusion volumes
@@ -694,8 +742,9 @@ class gpuNEXI(_ModelBase):
         normalised Sl0 exceeds 1 + thres_impossible, the lowest shell is
         below thres_bkg, the Sl0 curve correlates below thres_similarity
         with the median curve of the valid voxels, or a value is NaN/Inf."""
-        if str(fitting.get("dataType", "volumetric")).lower() == "surface":
-            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "surface data (surface TV) are not part of this library")
+        surface = str(fitting.get("dataType", "volumetric")).lower() == "surface"
+        if surface:
+            extradata = self._prepare_surface(data, mask, extradata, fitting)
         data = np.asarray(data, np.float64)
         dims = utils.dims3(data)
         nb = self.b.size
@@ -708,6 +757,9 @@ class gpuNEXI(_ModelBase):
             raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "Input has %d volumes but model expects %d. Check lmax or "
                                "input data." % (data.shape[3], nb * nsl))
         mask = np.asarray(mask).reshape(dims, order="F") > 0
+        if surface:
+            # the signal masks are for volumetric data only (:569)
+            return data, mask, extradata
         sl0 = data[..., :nb]
         with np.errstate(invalid="ignore"):
             valid = ~np.any(np.abs(sl0) > 1 + self.thres_impossible, axis=3)
@@ -879,8 +931,9 @@ class gpuNEXI(_ModelBase):
         the Rician model keeps gpuNEXIrice's order fit(dwi, mask, fitting,
         extraData, pars0) (NEXI/gpuNEXIrice.m:203-336).  regularisationType
         'TV' or 'prior' (extradata['mu'][name], extradata['sigma'][name] on
-        the image grid for every regmap name); surface data are not part of
-        this library."""
+        the image grid for every regmap name); with dataType 'surface' the TV
+        regulariser is the mesh TV of surface_total_variation
+        (extradata['neighbours'], extradata['dr'] as estimate() fills them)."""
         if args:
             first, rest = args[0], (args[1] if len(args) > 1 else None)
             if self.rician:
@@ -903,8 +956,8 @@ class gpuNEXI(_ModelBase):
                 and self.rician:
             # gpuNEXIrice has no prior path, and its extraData.sigma is the noise map
             raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "the 'prior' regulariser is a gpuNEXI option (not gpuNEXIrice)")
-        if str(fitting.get("dataType", "volumetric")).lower() == "surface":
-            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "surface data (surface TV) are not part of this library")
+        if self.rician and str(fitting.get("dataType", "volumetric")).lower() == "surface":
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "surface data are a gpuNEXI option (not gpuNEXIrice)")
         w = self.compute_optimisation_weights(mask, fitting["lossFunction"], int(fitting["lmax"]))
         if pars0 is None or (isinstance(pars0, dict) and not pars0):
             pars0 = self.determine_x0(dwi_, mask, fitting, extraData)
@@ -932,8 +985,12 @@ class gpuNEXI(_ModelBase):
             p0 = e.pop("_pars0", None)
             return self.fit(np.asarray(d, np.float32), m, fitting, pars0=p0, extraData=e or None)
 
-        out = segment.run_segments(fit_one, data, mask, extraData, fitting,
-                                   self._segment_bytes(data, fitting, n_extra=1 if self.rician else 0, weighted=True))
+        if str(fitting.get("dataType", "volumetric")).lower() == "surface":
+            out = segment.run_surface_segments(fit_one, data, mask, extraData, fitting)
+        else:
+            out = segment.run_segments(fit_one, data, mask, extraData, fitting,
+                                       self._segment_bytes(data, fitting, n_extra=1 if self.rician else 0,
+                                                           weighted=True))
         out["mask"] = mask
         return out
 

@@ -214,3 +214,34 @@ def run_segments(fit_one, data, mask, extra, fitting, bytes_per_voxel):
         o = fit_one(ds, ms, es)
         out = restore_segment_structure(out, crop_segment_output(o, sg), sg["owned"], k, nz)
     return out
+
+
+def surface_segments(fitting, nhemi):
+    """Surface data segment exactly per hemisphere or not at all
+    (NEXI/gpuNEXI.m:537-546): fitting['NSegmentUser'] of 1 (or unset) or
+    Nhemi; the mesh graphs of the hemispheres are independent, so the split
+    needs no halo."""
+    n = fitting.get("NSegmentUser")
+    n = 1 if n is None or np.size(n) == 0 else int(round(float(np.asarray(n).ravel()[0])))
+    if n not in (1, nhemi):
+        raise GacelleError(_abi.GC_ERR_ARG, "GACELLE:invalidSurfaceSegmentation: Surface segmentation is exact-only "
+                           f"and limited to fitting.NSegmentUser = 1 (no split) or {nhemi} (exact per-hemisphere "
+                           f"split); got {n}.")
+    if n == 1 or nhemi == 1:
+        return None
+    return [dict(owned=np.array([h]), fit=np.array([h]), local=np.array([0])) for h in range(nhemi)]
+
+
+def run_surface_segments(fit_one, data, mask, extra, fitting):
+    """estimate() on surface data: one fit, or one per hemisphere with the
+    outputs placed back at their hemisphere (dim 3)."""
+    mask = np.asarray(mask)
+    m3 = mask.reshape(mask.shape + (1,) * (3 - mask.ndim)) if mask.ndim < 3 else mask
+    segs = surface_segments(fitting, m3.shape[2])
+    if segs is None:
+        return fit_one(data, mask, extra)
+    out = None
+    for k, sg in enumerate(segs):
+        ds, ms, es = slice_segment(data, m3, sg["fit"], extra)
+        out = restore_segment_structure(out, fit_one(ds, ms, es), sg["owned"], k, m3.shape[2])
+    return out

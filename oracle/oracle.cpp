@@ -293,6 +293,36 @@ static double tv_loss_grad(const S* img, const uint8_t* mask, const int32_t dims
   return acc * scale;
 }
 
+// surface_total_variation, NEXI/gpuNEXI.m:1165-1254.  img is the vertex
+// image [Ngrid] (reshape_GD2ND: zeros outside the mask; Ngrid = Nvertex *
+// Nhemi); nbr is [max_nb][Ngrid] of global 0-based vertex indices
+// (neighbours + hemiOffset, :1232-1233) and elen the matching edge lengths.
+// dthetadr = (theta - theta(nbr)) ./ edgeLength (:1240), non-finite entries
+// zeroed (:1245-1247), summed as |.| over the masked vertices and every
+// slot (reshape_ND2GD, :1248) / Nsample * lambda (:1250).  grad (optional)
+// accumulates the subgradient (sign(0) = 0) with respect to img.
+template <class S>
+static double surface_tv_loss_grad(const S* img, const uint8_t* mask, int64_t ngrid, const int32_t* nbr,
+                                   const float* elen, int max_nb, double lambda, int64_t nmasked, S* grad) {
+  double acc = 0;
+  const double scale = lambda / double(nmasked);
+  for (int n = 0; n < max_nb; ++n)
+    for (int64_t i = 0; i < ngrid; ++i) {
+      if (!mask[i]) continue;
+      const int64_t j = nbr[size_t(n) * ngrid + i];
+      const double e = double(elen[size_t(n) * ngrid + i]);
+      const double d = (double(img[i]) - double(img[j])) / e;
+      if (!std::isfinite(d)) continue;
+      acc += std::fabs(d);
+      if (grad) {
+        const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
+        grad[i] += S(scale * sg / e);
+        grad[j] -= S(scale * sg / e);
+      }
+    }
+  return acc * scale;
+}
+
 // ===========================================================================
 // MATLAB median of a vector (mean of the two middle values for even n)
 static double median_of(std::vector<double> v) {
@@ -312,7 +342,8 @@ struct AskOut {
 template <class S, int NP>
 static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv, const float* y,
                        const float* w, const float* extra, const int64_t* vox, const int32_t* dims,
-                       const float* prior_grid, const float* z0, double* zfinal, double* zmin, double* res_final,
+                       const float* prior_grid, const int32_t* surf_nbr, const float* surf_len, int surf_max_nb,
+                       const float* z0, double* zfinal, double* zmin, double* res_final,
                        double* res_min, double* trace, AskOut& out, int nthreads) {
   const int P = md.n_par, M = md.n_meas, E = md.n_extra;
   const S inv_count = S(1.0 / (double(M) * double(nv)));
@@ -328,8 +359,11 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
 
   const bool use_reg = cfg.n_reg > 0 && cfg.lambda[0] > 0;
   const bool use_prior = use_reg && cfg.reg_type == GC_REG_PRIOR;
-  const bool use_tv = use_reg && !use_prior;
+  const bool use_stv = use_reg && cfg.reg_type == GC_REG_SURFACE_TV;
+  const bool use_tv = use_reg && cfg.reg_type == GC_REG_TV;
+  if (use_reg && !use_prior && !use_stv && !use_tv) return GC_ERR_ARG;
   if (use_prior && (!prior_grid || !vox || !dims)) return GC_ERR_ARG;
+  if (use_stv && (!surf_nbr || !surf_len || surf_max_nb < 1 || !vox || !dims)) return GC_ERR_ARG;
   int64_t nvox = 1;
   std::vector<uint8_t> mask;
   std::vector<S> img, gimg;
@@ -383,6 +417,20 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
           img[vox[v]] = unscale<S, S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
         reg += tv_loss_grad<S>(img.data(), mask.data(), dims, cfg.tv_mode, cfg.voxel_size, cfg.lambda[k], nv,
                                gimg.data());
+        for (int64_t v = 0; v < nv; ++v)
+          g[size_t(p) * nv + v] += gimg[vox[v]] * dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v],
+                                                             S(md.lb[p]), S(md.ub[p]));
+      }
+    }
+    if (use_stv) {
+      for (int k = 0; k < cfg.n_reg; ++k) {
+        const int p = cfg.regmap[k];
+        std::fill(img.begin(), img.end(), S(0));
+        std::fill(gimg.begin(), gimg.end(), S(0));
+        for (int64_t v = 0; v < nv; ++v)
+          img[vox[v]] = unscale<S, S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
+        reg += surface_tv_loss_grad<S>(img.data(), mask.data(), nvox, surf_nbr, surf_len, surf_max_nb,
+                                       cfg.lambda[k], nv, gimg.data());
         for (int64_t v = 0; v < nv; ++v)
           g[size_t(p) * nv + v] += gimg[vox[v]] * dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v],
                                                              S(md.lb[p]), S(md.ub[p]));
@@ -1285,11 +1333,19 @@ int orc_tv(const double* img, const uint8_t* mask, const int32_t dims[3], int32_
   return 0;
 }
 
+int orc_surface_tv(const double* img, const uint8_t* mask, int64_t ngrid, const int32_t* nbr, const float* elen,
+                   int32_t max_nb, double lambda, int64_t nmasked, double* loss, double* grad) {
+  for (int64_t i = 0; i < int64_t(max_nb) * ngrid; ++i)
+    if (nbr[i] < 0 || nbr[i] >= ngrid) return GC_ERR_ARG;
+  *loss = orc::surface_tv_loss_grad<double>(img, mask, ngrid, nbr, elen, max_nb, lambda, nmasked, grad);
+  return 0;
+}
+
 // askadam; precision 0 = double (oracle), 1 = float (CPU-path baseline).
 // scal_out[10] = {loss, fid, reg, min_loss, min_fid, min_reg, conv, niter, min_iter, stop}
 int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const float* y, const float* w,
                 const float* extra, const int64_t* vox, const int32_t* dims, const float* prior_grid,
-                const float* z0, double* zfinal,
+                const int32_t* surf_nbr, const float* surf_len, int32_t surf_max_nb, const float* z0, double* zfinal,
                 double* zmin, double* res_final, double* res_min, double* trace, double* scal_out, int32_t precision,
                 int32_t nthreads) {
   if (md->n_meas > 1024 || md->n_par > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
@@ -1298,13 +1354,15 @@ int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const
   if (precision == 0) {
     rc = orc::with_np(md->n_par, [&](auto npc) {
       constexpr int NP = decltype(npc)::value;
-      return orc::askadam_run<double, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, z0, zfinal, zmin, res_final,
+      return orc::askadam_run<double, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, surf_nbr, surf_len,
+                                          surf_max_nb, z0, zfinal, zmin, res_final,
                                           res_min, trace, o, std::max(1, nthreads));
     });
   } else {
     rc = orc::with_np(md->n_par, [&](auto npc) {
       constexpr int NP = decltype(npc)::value;
-      return orc::askadam_run<float, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, z0, zfinal, zmin, res_final,
+      return orc::askadam_run<float, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, surf_nbr, surf_len,
+                                         surf_max_nb, z0, zfinal, zmin, res_final,
                                          res_min, trace, o, std::max(1, nthreads));
     });
   }

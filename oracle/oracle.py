@@ -55,8 +55,10 @@ def lib():
         L.orc_tv.argtypes = [c_void_p, c_void_p, POINTER(c_int32), c_int32, POINTER(c_double), c_double, c_int64,
                              POINTER(c_double), c_void_p]
         L.orc_askadam.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  POINTER(c_double), c_int32, c_int32]
+                                  POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_double), c_int32, c_int32]
+        L.orc_surface_tv.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int32, c_double, c_int64,
+                                     POINTER(c_double), c_void_p]
         L.orc_mcmc.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_int32, c_int32]
@@ -173,10 +175,27 @@ def tv(img, mask, mode, vsize, lam, nmasked, grad=True):
     return (loss.value, g) if grad else loss.value
 
 
+def surface_tv(img, mask, nbr, elen, lam, nmasked, grad=True):
+    """surface_total_variation on a vertex image [ngrid] with global 0-based
+    neighbours nbr [max_nb, ngrid] and edge lengths elen [max_nb, ngrid]."""
+    img = np.ascontiguousarray(img, dtype=np.float64).ravel()
+    mask = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    elen = np.ascontiguousarray(elen, dtype=np.float32)
+    loss = c_double()
+    g = np.zeros(img.size) if grad else None
+    rc = lib().orc_surface_tv(_p(img), _p(mask), img.size, _p(nbr), _p(elen), nbr.shape[0], lam, nmasked,
+                              ctypes.byref(loss), _p(g))
+    assert rc == 0, rc
+    return (loss.value, g) if grad else loss.value
+
+
 def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0, nthreads=1, trace=False,
-            prior_grid=None):
+            prior_grid=None, surf_nbr=None, surf_len=None):
     """prior_grid: GC_REG_PRIOR maps on the whole grid (column-major),
-    [2*n_reg, ngrid]: the mu rows, then the sigma rows; with vox and dims."""
+    [2*n_reg, ngrid]: the mu rows, then the sigma rows; with vox and dims.
+    surf_nbr / surf_len: GC_REG_SURFACE_TV neighbours (global 0-based vertex
+    indices) and edge lengths, [max_nb, ngrid], with vox and dims."""
     y = _f32(y)
     M, nv = y.shape
     z0 = _f32(z0)
@@ -192,8 +211,10 @@ def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0
     tr = np.zeros(int(cfg.iteration) + 1) if trace else None
     sc = (c_double * 10)()
     pg = None if prior_grid is None else np.ascontiguousarray(prior_grid, np.float32)
+    sn = None if surf_nbr is None else np.ascontiguousarray(surf_nbr, np.int32)
+    sl = None if surf_len is None else np.ascontiguousarray(surf_len, np.float32)
     rc = lib().orc_askadam(ctypes.byref(md), ctypes.byref(cfg), nv, _p(y), _p(w), _p(extra), _p(vox), d, _p(pg),
-                           _p(z0),
+                           _p(sn), _p(sl), 0 if sn is None else sn.shape[0], _p(z0),
                            _p(zf), _p(zm), _p(rf), _p(rm), _p(tr), sc, precision, nthreads)
     assert rc == 0, rc
     out = dict(z_final=zf, z_min=zm, residual=rf, residual_min=rm, loss=sc[0], loss_fidelity=sc[1],

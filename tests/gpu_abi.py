@@ -61,7 +61,8 @@ def dict_search(y, d, w=None, with_sse=True):
 
 def askadam(md, cfg, vols, trace=False):
     """vols: list of dict(y, z0, w=None, extra=None, vox=None, dims=None,
-    prior=None, reg_offset=0).  Returns per-volume dicts."""
+    prior=None, reg_offset=0, surf_nbr=None, surf_len=None).  Returns
+    per-volume dicts."""
     c = ctx()
     n = len(vols)
     arr = (_abi.gc_volume * n)()
@@ -104,6 +105,12 @@ def askadam(md, cfg, vols, trace=False):
             a.prior_dev = runtime.ptr(pd)
             a.reg_offset = float(v.get("reg_offset", 0.0))
             keep.append(pd)
+        if v.get("surf_nbr") is not None:   # [max_nb, ngrid] global vertex indices and edge lengths
+            sn = c.to_device(np.asarray(v["surf_nbr"], np.int32), np.int32)
+            sl = c.to_device(np.asarray(v["surf_len"], np.float32))
+            a.surf_nbr_dev, a.surf_len_dev = runtime.ptr(sn), runtime.ptr(sl)
+            a.surf_max_nb = int(np.shape(v["surf_nbr"])[0])
+            keep += [sn, sl]
         keep += [yd, wd, ex, zd, zm, rl, rlm, rs, rsm]
         metas.append((zd, zm, rl, rlm, rs, rsm, tr))
     c.check(c.lib.gc_askadam_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), arr, n))

@@ -10,11 +10,13 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 5
+GC_ABI_VERSION = 6
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
 GC_MAX_REG = 4
+GC_MLP_MAX_LAYERS = 6
+GC_MLP_MAX_WIDTH = 64
 
 # status codes
 GC_OK = 0
@@ -40,6 +42,7 @@ GC_FLAG_MCRMWI_COMPLEX = 1 << 3
 GC_FLAG_MCRMWI_EXCHANGE = 1 << 4
 GC_FLAG_UNIFORM_TE = 1 << 5
 GC_FLAG_MCRMWI_PINI_FA = 1 << 6
+GC_FLAG_MCRMWI_EPG = 1 << 7
 
 GC_QUAD_TRAPZ = 0
 GC_QUAD_TABLE = 1
@@ -66,6 +69,16 @@ STOP_MESSAGES = {
 }
 
 
+class gc_mlp(ctypes.Structure):
+    _fields_ = [
+        ("n_layer", c_int32),
+        ("width", c_int32 * (GC_MLP_MAX_LAYERS + 1)),
+        ("alpha", c_float),
+        ("reserved", c_int32),
+        ("params", c_void_p),
+    ]
+
+
 class gc_model(ctypes.Structure):
     _fields_ = [
         ("model_id", c_int32),
@@ -78,6 +91,7 @@ class gc_model(ctypes.Structure):
         ("table", c_float * GC_MAX_TABLE),
         ("lb", c_float * GC_MAX_PAR),
         ("ub", c_float * GC_MAX_PAR),
+        ("mlp", gc_mlp * 2),
     ]
 
 

@@ -164,9 +164,12 @@ class askadam:
                 if np.all(w == 1):
                     w = None
             mdi = model.gc_model(fitting, "askadam", nmeas)
+            # the acquisition part (network parameter pointers aside) is shared
+            acq = bytes(mdi)[: _abi.gc_model.mlp.offset]
             if md is None:
-                md = mdi
-            elif bytes(md) != bytes(mdi):
+                md_acq = acq
+                md = model.bind_device(mdi, ctx, keep) if hasattr(model, "bind_device") else mdi
+            elif md_acq != acq:
                 raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "all volumes of a batch must share the acquisition")
             extra = model.extra_rows(extra_data, mask, vox, nv)
             P = md.n_par

@@ -46,6 +46,11 @@ int select_ops(gc_ctx* ctx, const gc_model& md, const ModelOps** out) {
       if (md.i[1] != 4)
         return set_err(ctx, GC_ERR_UNSUPPORTED, "MCRMWI: 4 flip angles supported (got %d)", md.i[1]);
       const bool ex = (md.flags & GC_FLAG_MCRMWI_EXCHANGE) != 0, cx = (md.flags & GC_FLAG_MCRMWI_COMPLEX) != 0;
+      if (md.flags & GC_FLAG_MCRMWI_EPG) {
+        if (!ex) return set_err(ctx, GC_ERR_ARG, "MCRMWI: the EPG-X path needs the exchange model (isFitExchange)");
+        *out = cx ? ops_mcrmwi4_epg_cplx() : ops_mcrmwi4_epg_mag();
+        return GC_OK;
+      }
       *out = ex ? (cx ? ops_mcrmwi4_exch_cplx() : ops_mcrmwi4_exch_mag())
                 : (cx ? ops_mcrmwi4_cplx() : ops_mcrmwi4_mag());
       return GC_OK;
@@ -168,6 +173,16 @@ int gc_unscale_parameters(gc_ctx* ctx, const gc_model* md, int32_t transform, co
   return GC_OK;
 }
 
+// The EPG-X rows of MCR-MWI at parameters x (nothing for other models)
+static int epgx_aux(gc_ctx* ctx, const gc_model& md, const gc_model* dmd, const float* x, const float* extra,
+                    int64_t nv, int transform, EpgxDev& e, DevBuf& aux) {
+  if (md.model_id != GC_MODEL_MCRMWI || !(md.flags & GC_FLAG_MCRMWI_EPG)) return GC_OK;
+  int rc = epgx_prepare(ctx, md, e);
+  if (rc) return rc;
+  GC_HIP(ctx, aux.alloc(sizeof(float) * size_t(kEpgxRows) * md.i[1] * nv));
+  return epgx_eval(ctx, e, dmd, x, extra, nv, transform, aux.as<float>(), ctx->stream);
+}
+
 int gc_fwd(gc_ctx* ctx, const gc_model* md, const float* theta, const float* extra, float* s, int64_t nv) {
   if (!ctx) return GC_ERR_ARG;
   int rc = check_model(ctx, md);
@@ -180,7 +195,15 @@ int gc_fwd(gc_ctx* ctx, const gc_model* md, const float* theta, const float* ext
   const gc_model* dmd = nullptr;
   rc = upload_model(ctx, md, &dmd);
   if (rc) return rc;
-  return ops->fwd(ctx, dmd, theta, extra, s, nv);
+  StreamScope scope_(ctx->stream);  // buffers of this call drain on its stream
+  EpgxDev epgx;
+  DevBuf aux;
+  rc = epgx_aux(ctx, *md, dmd, theta, extra, nv, GC_TRANSFORM_NONE, epgx, aux);
+  if (rc) return rc;
+  rc = ops->fwd(ctx, dmd, theta, extra, s, nv, aux.as<float>());
+  if (rc) return rc;
+  if (aux.p) GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // scratch freed on return
+  return GC_OK;
 }
 
 int gc_fwd_adj(gc_ctx* ctx, const gc_model* md, int32_t loss, int32_t transform, const float* x, const float* y,
@@ -200,8 +223,12 @@ int gc_fwd_adj(gc_ctx* ctx, const gc_model* md, int32_t loss, int32_t transform,
   const gc_model* dmd = nullptr;
   rc = upload_model(ctx, md, &dmd);
   if (rc) return rc;
+  EpgxDev epgx;
+  DevBuf aux;
+  rc = epgx_aux(ctx, *md, dmd, x, extra, nv, transform, epgx, aux);
+  if (rc) return rc;
   rc = ops->fwd_adj(ctx, dmd, loss, transform, x, y, w, extra, nv, scale, grad,
-                    loss_sum ? part.as<double>() : nullptr, residual);
+                    loss_sum ? part.as<double>() : nullptr, residual, aux.as<float>());
   if (rc) return rc;
   if (loss_sum) {
     hipLaunchKernelGGL(k_reduce_rows_d, dim3(1), dim3(kBlock), 0, ctx->stream, part.as<double>(), int64_t(nb),

@@ -47,6 +47,7 @@ struct VolRun {
   DevBuf pp[2][3];      // step form: intermediate states of a launch sequence
   DevBuf nbr;           // step form with TV: [2*nd][nv] grid neighbours
   DevBuf snb, sie, srv, srie;  // surface TV tables (StepVol)
+  DevBuf aux;                  // EPG-X rows [15 n_fa][nv] of the current evaluation
   int snf = 0, snr = 0;
   // robustConvergence (utils/askadam.m:448-510, 541-566)
   std::vector<double> pvl_init, pvl, hist;  // per-voxel mean loss; history [W][nv]
@@ -97,6 +98,8 @@ struct AskDriver {
   bool prior = false;    // ... the normal prior
   bool use_tv = false;   // ... spatial total variation
   bool use_stv = false;  // ... surface (mesh) total variation
+  bool use_epgx = false; // MCR-MWI EPG-X: k_epgx before every evaluation
+  EpgxDev epgx;
   bool robust = false;
   bool replaying = false;       // a replay launch: no per-voxel loss outputs
   bool replay_pinned = false;   // ... from the pinned block (its weight modulator)
@@ -186,7 +189,9 @@ struct AskDriver {
   }
 
   // stream_model: the model's data do not fit in registers (step form only)
-  int setup(bool stream_model) {
+  // epgx_model: the MCR-MWI EPG-X path (aux rows from the k_epgx pre-pass
+  // before every model evaluation)
+  int setup(bool stream_model, bool epgx_model = false) {
     use_reg = cfg.n_reg > 0 && cfg.lambda[0] > 0;
     prior = use_reg && cfg.reg_type == GC_REG_PRIOR;
     use_tv = use_reg && cfg.reg_type == GC_REG_TV;
@@ -215,6 +220,11 @@ struct AskDriver {
           return set_err(ctx, GC_ERR_ARG, "volume %d: the prior regulariser needs prior_dev (mu and sigma)", i);
     int rcm = upload_model(ctx, &md, &dmd);
     if (rcm) return rcm;
+    use_epgx = epgx_model;
+    if (use_epgx) {
+      rcm = epgx_prepare(ctx, md, epgx);
+      if (rcm) return rcm;
+    }
     int64_t nv_total = 0;
     for (int i = 0; i < n_vol; ++i) {
       if (vols[i].nv <= 0) return set_err(ctx, GC_ERR_ARG, "volume %d has no voxels", i);
@@ -271,6 +281,7 @@ struct AskDriver {
         const int rc = build_surface(i);
         if (rc) return rc;
       }
+      if (use_epgx) GC_HIP(ctx, R.aux.alloc(sizeof(float) * size_t(kEpgxRows) * md.i[1] * R.nv));
       if (robust) {
         GC_HIP(ctx, R.wmod.alloc(sizeof(float) * R.nv));
         GC_HIP(ctx, R.wmod_pin.alloc(sizeof(float) * R.nv));
@@ -593,6 +604,7 @@ struct AskDriver {
           d.z_last = (k == n - 1 && zl[i] >= 0) ? R.zlast[zl[i]].as<float>() : nullptr;
           d.partials = R.partials.as<float>();
           d.nbr = use_tv ? R.nbr.as<int32_t>() : nullptr;
+          d.aux = use_epgx ? R.aux.as<float>() : nullptr;
           if (use_stv) {
             d.snb = R.snb.as<int32_t>();
             d.sie = R.sie.as<float>();
@@ -642,6 +654,7 @@ struct AskDriver {
       auto launch_k = [&](int j, hipStream_t s) {
         StepParams spj = sp;
         spj.vols = d0 + size_t(j) * n_vol;
+        if (use_epgx) (void)epgx_step(ctx, epgx, dmd, spj, unsigned(total_blk), cfg.transform, s);
         if (cfg.transform == GC_TRANSFORM_LINEAR)
           hipLaunchKernelGGL((k_askadam_step<Model, LOSS, HAS_W, GC_TRANSFORM_LINEAR>), dim3(total_blk),
                              dim3(kBlock), 0, s, dmd, spj);
@@ -670,6 +683,10 @@ struct AskDriver {
     }
     for (int k = 0; k < nmax; ++k) {
       sp.vols = d_steps.as<StepVol>() + size_t(k) * n_vol;
+      if (use_epgx) {
+        const int rcx = epgx_step(ctx, epgx, dmd, sp, unsigned(total_blk), cfg.transform, ctx->stream);
+        if (rcx) return rcx;
+      }
       hipEvent_t ev_a, ev_b;
       int rce = next_events(ev_a, ev_b);
       if (rce) return rce;
@@ -1011,6 +1028,12 @@ struct AskDriver {
         const float* zsrc = which == 0 ? vols[i].z_dev : vols[i].zmin_dev;
         if ((!rl && !rs) || !zsrc) continue;
         EvalParams ep{};
+        if (use_epgx) {
+          const int rcx = epgx_eval(ctx, epgx, dmd, zsrc, vols[i].extra_dev, R.nv, cfg.transform, R.aux.as<float>(),
+                                    ctx->stream);
+          if (rcx) return rcx;
+          ep.aux = R.aux.as<float>();
+        }
         ep.x = zsrc;
         ep.y = vols[i].y_dev;
         ep.w = HAS_W ? (vols[i].w_dev ? vols[i].w_dev : R.ones.as<float>()) : nullptr;

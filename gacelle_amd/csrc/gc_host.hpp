@@ -22,6 +22,7 @@
 
 #include "../../include/gacelle.h"
 #include "gc_gwglobal.hpp"
+#include "gc_epgx.hpp"
 
 struct gc_ctx {
   int device = 0;
@@ -298,12 +299,14 @@ inline int check_model(gc_ctx* ctx, const gc_model* md) {
 // Operations that depend on the register-tiled model type.
 struct ModelOps {
   // forward model: s = FWD(theta) (model_forward of each model class)
-  int (*fwd)(gc_ctx* ctx, const gc_model* dmd, const float* theta, const float* extra, float* s, int64_t nv);
+  // (aux: the per-voxel rows of models that take them, else null)
+  int (*fwd)(gc_ctx* ctx, const gc_model* dmd, const float* theta, const float* extra, float* s, int64_t nv,
+             const float* aux);
   // loss, adjoint and residuals at x (askadam.model_gradient); part gets
   // one FP64 loss partial per block of kBlock voxels when not null
   int (*fwd_adj)(gc_ctx* ctx, const gc_model* dmd, int loss, int transform, const float* x, const float* y,
                  const float* w, const float* extra, int64_t nv, float scale, float* grad, double* part,
-                 float* residual);
+                 float* residual, const float* aux);
   // the askadam loop over a batch of volumes
   int (*askadam)(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol);
   // one MH or GW launch (kernel-only timing through ev0/ev1)
@@ -315,6 +318,22 @@ struct ModelOps {
   int (*gwg)(gc_ctx* ctx, const gc_model* dmd, int kind, bool hw, const GwgParams& P, const GwgStep& S,
              unsigned grid);
 };
+
+// ---- EPG-X networks (MCR-MWI isEPG) ------------------------------------------
+// The two networks of a gc_model in the padded layout the k_epgx kernels
+// read (gc_epgx.hpp), built once per call; wm = 16, 32 or 64 units.
+struct EpgxDev {
+  DevBuf wbuf;
+  EpgxNets nets{};
+  int wm = 0;
+};
+int epgx_prepare(gc_ctx* ctx, const gc_model& md, EpgxDev& out);
+// aux rows from z_in of every active volume of a step launch (epgx.hip)
+int epgx_step(gc_ctx* ctx, const EpgxDev& e, const gc_model* dmd, const StepParams& sp, unsigned grid, int transform,
+              hipStream_t stream);
+// aux rows from parameters x [P][nv] in the given transform space
+int epgx_eval(gc_ctx* ctx, const EpgxDev& e, const gc_model* dmd, const float* x, const float* ex, int64_t nv,
+              int transform, float* aux, hipStream_t stream);
 
 // one getter per model variant, each defined in its own ops_*.hip
 const ModelOps* ops_r2s12_uniform();
@@ -330,5 +349,7 @@ const ModelOps* ops_mcrmwi4_exch_cplx();
 const ModelOps* ops_mcrmwi4_exch_mag();
 const ModelOps* ops_mcrmwi4_cplx();
 const ModelOps* ops_mcrmwi4_mag();
+const ModelOps* ops_mcrmwi4_epg_cplx();
+const ModelOps* ops_mcrmwi4_epg_mag();
 
 }  // namespace gc

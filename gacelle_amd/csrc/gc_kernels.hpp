@@ -110,9 +110,21 @@ GC_DEV void accum_grad(float dl, float w, float res, const float* J, float* gth)
 // returns d loss / d s (0 for a measurement that carries no gradient).
 // Models with a fused adjoint (kFusedAdj) accumulate the gradient
 // themselves; the others hand a Jacobian per measurement.
+template <class Model>
+struct has_aux {
+  template <class M>
+  static constexpr bool f(decltype(M::kAux)*) { return M::kAux; }
+  template <class M>
+  static constexpr bool f(...) { return false; }
+  static constexpr bool value = f<Model>(nullptr);
+};
+
 template <class Model, class LF>
-GC_DEV void model_adjoint(const gc_model& md, const float* th, const float* ex, LF&& lossfn, float* gth) {
-  if constexpr (Model::kFusedAdj) {
+GC_DEV void model_adjoint(const gc_model& md, const float* th, const float* ex, LF&& lossfn, float* gth,
+                          AuxRef aux = {}) {
+  if constexpr (has_aux<Model>::value) {
+    Model::eval_adj(md, th, ex, lossfn, gth, aux);
+  } else if constexpr (Model::kFusedAdj) {
     Model::eval_adj(md, th, ex, lossfn, gth);
   } else {
     Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
@@ -426,6 +438,7 @@ struct StepVol {
   const float* wmod;   // robustConvergence: per-voxel weight factor of this iteration, or null
   const uint8_t* mconv;  // robustConvergence: convergence voxel set, or null
   float* pvl_out;      // robustConvergence: per-voxel mean loss of this iteration, or null
+  float* aux;          // models with per-iteration aux rows (MCR-MWI EPG-X): [rows][nv]
   // GC_REG_SURFACE_TV: per masked vertex, its mesh neighbours (G-D index,
   // -1 outside the mask, -2 ends the list) with 1/edgeLength, [snf][nv],
   // and the masked vertices listing it as a neighbour (-1 ends the list)
@@ -508,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restr
         lsum += l;
         return fin ? dl : 0.f;
       },
-      gth);
+      gth, AuxRef{V.aux ? V.aux + vv : nullptr, nv});
   float g[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * V.inv_count;
@@ -733,6 +746,7 @@ struct EvalParams {
   float* residual;      // [M][nv] elementwise loss
   float* resloss;       // [nv] mean elementwise loss per voxel
   double* partial;      // [nblk] block loss sums
+  const float* aux;     // [rows][nv] aux rows of models that take them, or null
   int64_t nv;
   int32_t transform;
   int32_t mode;         // 0 forward, 1 loss (+ optional outputs)
@@ -756,10 +770,13 @@ __global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ md
   for (int p = 0; p < NP; ++p)
     transform_fwd(ep.transform, ep.x[int64_t(p) * nv + vv], md.lb[p], md.ub[p], th[p], dth[p]);
   double lsum = 0.0;
+  const AuxRef aux{ep.aux ? ep.aux + vv : nullptr, nv};
   if (ep.mode == 0) {
-    Model::template eval<false>(md, th, ex, [&](int r, float s, const float*) {
+    auto put = [&](int r, float s, const float*) {
       if (valid) ep.s_out[int64_t(Model::meas(md, r)) * nv + v] = isfinite(s) ? s : 0.f;
-    });
+    };
+    if constexpr (has_aux<Model>::value) Model::template eval<false>(md, th, ex, put, aux);
+    else Model::template eval<false>(md, th, ex, put);
   } else {
     float gth[NP];
 #pragma unroll
@@ -778,7 +795,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(const gc_model* __restrict__ md
           if (ep.residual != nullptr && valid) ep.residual[m * nv + v] = l;
           return fin ? dl : 0.f;
         },
-        gth);
+        gth, aux);
     lsum = valid ? double(lvox) : 0.0;
     if (valid && ep.resloss != nullptr) ep.resloss[v] = lvox / float(md.n_meas);
     if (valid && ep.grad != nullptr) {

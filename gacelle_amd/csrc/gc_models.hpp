@@ -404,8 +404,24 @@ GC_DEV DualF<N> dmax(const DualF<N>& a, float c) {
 // is accumulated from the loss derivatives the caller returns per
 // measurement.  Data are streamed (kStreamOnly): M = 2 x 32 x MF is too
 // large to keep in registers.
-template <int MF, bool EXCH, bool CPLX>
+// Per-voxel rows computed by a pre-pass (stride = voxels of the volume)
+struct AuxRef {
+  const float* p = nullptr;
+  int64_t stride = 0;
+  GC_DEV float operator[](int k) const { return p[int64_t(k) * stride]; }
+};
+
+// EPG-X path (EPG, needs EXCH; MCRMWI/gpuMCRMWI.m:664-681): the MLP
+// surrogate's outputs at each flip angle come from the k_epgx pre-pass as
+// 15 aux rows per flip angle, [k * n_fa + fi]: the IEW phase, the IEW and MW
+// steady-state corrections, each followed by its derivatives with respect
+// to (MVF, R1IEW, kIEWM, R2sIW).
+constexpr int kEpgxRows = 15;
+
+template <int MF, bool EXCH, bool CPLX, bool EPG = false>
 struct McrmwiModel {
+  static_assert(!EPG || EXCH, "the EPG-X path needs the exchange model");
+  static constexpr bool kAux = EPG;
   static constexpr int NA = EXCH ? 5 : 4;
   static constexpr int NP = NA + 5 + (CPLX ? MF + 1 : 0);
   static constexpr int MT = 32;
@@ -418,9 +434,11 @@ struct McrmwiModel {
   GC_DEV static bool valid(const gc_model& md, int r) { return r < md.n_meas; }
   GC_DEV static int meas(const gc_model&, int r) { return r; }
 
-  // compartment amplitudes (MW, IW, EW) at flip angle index fi
+  // compartment amplitudes (MW, IW, EW) at flip angle index fi; with EPG
+  // also the IEW phase ph and the R2sIW derivatives (r2[0..2] of the
+  // amplitudes, r2[3] of ph), which the NA duals do not carry
   GC_DEV static void amplitudes(const gc_model& md, const float* th, const float* ex, int fi, D& aMW, D& aIW,
-                                D& aEW) {
+                                D& aEW, AuxRef aux = {}, D* ph = nullptr, float* r2 = nullptr) {
     const int nte = md.i[0];
     const float TR = md.f[0], rho = md.f[3], t1mw = md.f[4];
     const D S0 = D::var(th[0], 0), mwf = D::var(th[1], 1), iwf = D::var(th[2], 2), r1iew = D::var(th[3], 3);
@@ -457,8 +475,35 @@ struct McrmwiModel {
       const D dp = ca * (e21 * e12) + (1.f - ca * e11) * (e22 - D::cst(1.f));
       const D siew = ((ap * d - bp * c) * r1iew * (1.f - f) + R1r * (bp * a - ap * b) * f) * C;
       const D sm = ((cp * d - c * dp) * r1iew * (1.f - f) + R1r * (a * dp - cp * b) * f) * C;
-      aMW = rho * (sm * tv);
-      aIEW = siew * tv;
+      if constexpr (EPG) {
+        // MLP corrections, chained to the NA directions through MVF (f),
+        // R1IEW (slot 3) and kIEWM (slot 4); clamped at askadam.epsilon
+        auto row = [&](int k) { return aux[k * md.i[1] + fi]; };
+        auto lift = [&](int k) {
+          D o = D::cst(row(k));
+#pragma unroll
+          for (int q = 0; q < NA; ++q) o.d[q] = row(k + 1) * f.d[q];
+          o.d[3] += row(k + 2);
+          o.d[4] += row(k + 3);
+          return o;
+        };
+        *ph = lift(0);
+        const D dIEW = lift(5), dMW = lift(10);
+        const D mw = rho * ((sm + dMW) * tv);
+        const D iew = (siew + dIEW) * tv;
+        const bool mw_on = mw.v >= 1e-8f, iew_on = iew.v >= 1e-8f;
+        aMW = dmax(mw, 1e-8f);
+        aIEW = dmax(iew, 1e-8f);
+        const float tvr = tv.v;
+        r2[0] = mw_on ? rho * tvr * row(14) : 0.f;
+        const float riew = iew_on ? tvr * row(9) : 0.f;
+        r2[1] = riew * th[2];
+        r2[2] = riew * (1.f - th[2]);
+        r2[3] = row(4);
+      } else {
+        aMW = rho * (sm * tv);
+        aIEW = siew * tv;
+      }
     } else {
       const float e1mw = __expf(-TR / t1mw);
       aMW = (sa * (1.f - e1mw) / (1.f - ca * e1mw)) * S0MW;
@@ -474,7 +519,7 @@ struct McrmwiModel {
   // equally spaced echoes both are advanced by recurrence (one multiply
   // and one complex rotation per pool and echo).
   template <class G>
-  GC_DEV static void per_echo(const gc_model& md, const float* th, const float* ex, int fi, G&& g) {
+  GC_DEV static void per_echo(const gc_model& md, const float* th, const float* ex, int fi, G&& g, float ph = 0.f) {
     const int nte = md.i[0];
     const float B0g = md.f[1] * md.f[2];
     const float freqEW = md.f[5];
@@ -495,7 +540,7 @@ struct McrmwiModel {
       w[p] = k2pi * (freq[p] + fbkg) * B0g;
       if (uni) {
         e[p] = __expf(-te0 * r2s[p]);
-        sincosf(te0 * w[p] + pini, &sn[p], &cs[p]);
+        sincosf(te0 * w[p] + pini + (p > 0 ? ph : 0.f), &sn[p], &cs[p]);
         qe[p] = __expf(-dte * r2s[p]);
         sincosf(dte * w[p], &qs[p], &qc[p]);
       }
@@ -506,7 +551,7 @@ struct McrmwiModel {
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
           e[p] = __expf(-te * r2s[p]);
-          sincosf(te * w[p] + pini, &sn[p], &cs[p]);
+          sincosf(te * w[p] + pini + (p > 0 ? ph : 0.f), &sn[p], &cs[p]);
         }
       }
       g(t, te, e, cs, sn);
@@ -524,13 +569,14 @@ struct McrmwiModel {
   }
 
   template <bool JAC, class F>
-  GC_DEV static void eval(const gc_model& md, const float* th, const float* ex, F&& f) {
+  GC_DEV static void eval(const gc_model& md, const float* th, const float* ex, F&& f, AuxRef aux = {}) {
     static_assert(!JAC, "McrmwiModel provides its derivatives through eval_adj");
     const int nte = md.i[0], nfa = md.i[1];
     const float ff = ex[2];
     for (int fi = 0; fi < nfa; ++fi) {
-      D aMW, aIW, aEW;
-      amplitudes(md, th, ex, fi, aMW, aIW, aEW);
+      D aMW, aIW, aEW, ph = D::cst(0.f);
+      float r2[4];
+      amplitudes(md, th, ex, fi, aMW, aIW, aEW, aux, &ph, r2);
       const float amp[3] = {aMW.v, aIW.v, aEW.v};
       per_echo(md, th, ex, fi, [&](int t, float, const float* e, const float* c, const float* s) {
         const float d[3] = {amp[0] * e[0], amp[1] * e[1], amp[2] * e[2]};
@@ -542,24 +588,27 @@ struct McrmwiModel {
         } else {
           f(t + nte * fi, sqrtf(re * re + im * im), nullptr);
         }
-      });
+      }, ph.v);
     }
   }
 
   // dloss(r, s) returns d loss / d s_r (0 when the measurement carries no
   // gradient); g[NP] accumulates d loss / d theta.
   template <class F>
-  GC_DEV static void eval_adj(const gc_model& md, const float* th, const float* ex, F&& dloss, float* g) {
+  GC_DEV static void eval_adj(const gc_model& md, const float* th, const float* ex, F&& dloss, float* g,
+                              AuxRef aux = {}) {
     const int nte = md.i[0], nfa = md.i[1];
     const float ff = ex[2];
     const float B0g = md.f[1] * md.f[2];
     constexpr float k2pi = 6.283185307179586f;
     for (int fi = 0; fi < nfa; ++fi) {
-      D aMW, aIW, aEW;
-      amplitudes(md, th, ex, fi, aMW, aIW, aEW);
+      D aMW, aIW, aEW, ph = D::cst(0.f);
+      float r2[4] = {0.f, 0.f, 0.f, 0.f};
+      amplitudes(md, th, ex, fi, aMW, aIW, aEW, aux, &ph, r2);
       const float amp[3] = {aMW.v, aIW.v, aEW.v};
       float gamp[3] = {0.f, 0.f, 0.f};  // d loss / d a_p at this flip angle
       float gbkg = 0.f;                  // d loss / d (phase slope common to all pools) / w0
+      float gph = 0.f;                   // d loss / d (IEW phase)
       per_echo(md, th, ex, fi, [&](int t, float te, const float* e, const float* c, const float* s) {
         const float d[3] = {amp[0] * e[0], amp[1] * e[1], amp[2] * e[2]};
         const float re = (d[0] * c[0] + d[1] * c[1] + d[2] * c[2]) * ff;
@@ -585,15 +634,18 @@ struct McrmwiModel {
           gamp[p] += u * e[p];
           g[NA + p] += u * d[p] * (-te);                    // R2s of pool p
           if (p < 2) g[NA + 3 + p] += v * w0;              // freqMW, freqIW
+          if (EPG && p > 0) gph += v;
           vsum += v;
         }
         if constexpr (CPLX) {
           gbkg += vsum * w0;
           g[NA + 5 + MF] += vsum;                           // dpini
         }
-      });
+      }, ph.v);
 #pragma unroll
-      for (int q = 0; q < NA; ++q) g[q] += gamp[0] * aMW.d[q] + gamp[1] * aIW.d[q] + gamp[2] * aEW.d[q];
+      for (int q = 0; q < NA; ++q)
+        g[q] += gamp[0] * aMW.d[q] + gamp[1] * aIW.d[q] + gamp[2] * aEW.d[q] + (EPG ? gph * ph.d[q] : 0.f);
+      if constexpr (EPG) g[NA + 1] += gamp[0] * r2[0] + gamp[1] * r2[1] + gamp[2] * r2[2] + gph * r2[3];  // R2sIW
       if constexpr (CPLX) g[NA + 5 + fi] += gbkg;
     }
   }

@@ -8,8 +8,10 @@ namespace gc {
 
 template <class Model>
 struct OpsImpl {
-  static int fwd(gc_ctx* ctx, const gc_model* dmd, const float* theta, const float* extra, float* s, int64_t nv) {
+  static int fwd(gc_ctx* ctx, const gc_model* dmd, const float* theta, const float* extra, float* s, int64_t nv,
+                 const float* aux) {
     EvalParams ep{};
+    ep.aux = aux;
     ep.x = theta;
     ep.ex = extra;
     ep.s_out = s;
@@ -23,7 +25,7 @@ struct OpsImpl {
 
   static int fwd_adj(gc_ctx* ctx, const gc_model* dmd, int loss, int transform, const float* x, const float* y,
                      const float* w, const float* extra, int64_t nv, float scale, float* grad, double* part,
-                     float* residual) {
+                     float* residual, const float* aux) {
     return with_loss(ctx, loss, [&](auto lc) -> int {
       constexpr int LOSS = decltype(lc)::value;
       return with_bool(w != nullptr, [&](auto hw) -> int {
@@ -36,6 +38,7 @@ struct OpsImpl {
         ep.grad = grad;
         ep.residual = residual;
         ep.partial = part;
+        ep.aux = aux;
         ep.nv = nv;
         ep.transform = transform;
         ep.mode = 1;
@@ -49,7 +52,7 @@ struct OpsImpl {
 
   static int askadam(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol) {
     AskDriver drv(ctx, *md, *cfg, vols, n_vol);
-    int rc = drv.setup(Model::kStreamOnly);
+    int rc = drv.setup(Model::kStreamOnly, has_aux<Model>::value);
     if (rc) return rc;
     return with_loss(ctx, cfg->loss, [&](auto lc) -> int {
       constexpr int LOSS = decltype(lc)::value;

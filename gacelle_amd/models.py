@@ -998,16 +998,59 @@ class gpuNEXI(_ModelBase):
 gpuNEXIrice = gpuNEXI  # the Rician variant is gpuNEXI(..., rician=True)
 
 
+def mlp_pack(net, n_out):
+    """A network of mlp_model_leakyRelu (MCRMWI/EPGXgen_net/mlp_model_leakyRelu.m:23-47)
+    in the C ABI's packed form: (widths [11, ..., n_out], alpha, float32
+    array of each layer's Weights in MATLAB's column-major [out, in] order
+    followed by its Bias).  net: the dlnet as a dict ({'parameters': {'fc1':
+    {'Weights': W, 'Bias': b}, ...}, 'alpha': a} or the parameters dict
+    itself) or the path of an .npz with fc<l>_Weights / fc<l>_Bias (and an
+    optional alpha).  alpha defaults to 0.01, the value gpuMCRMWI.fit sets
+    (MCRMWI/gpuMCRMWI.m:386-387)."""
+    if isinstance(net, str):
+        with np.load(net, allow_pickle=False) as d:
+            params = {}
+            l = 1
+            while f"fc{l}_Weights" in d:
+                params[f"fc{l}"] = {"Weights": np.array(d[f"fc{l}_Weights"]), "Bias": np.array(d[f"fc{l}_Bias"])}
+                l += 1
+            alpha = float(d["alpha"]) if "alpha" in d else 0.01
+    else:
+        params = net.get("parameters", net)
+        alpha = float(net.get("alpha", 0.01)) if "parameters" in net else 0.01
+    widths = [11]
+    parts = []
+    l = 1
+    while f"fc{l}" in params:
+        W = np.asarray(params[f"fc{l}"]["Weights"], np.float32)
+        b = np.asarray(params[f"fc{l}"]["Bias"], np.float32).ravel()
+        if W.ndim != 2 or W.shape[1] != widths[-1] or b.size != W.shape[0]:
+            raise GacelleError(_abi.GC_ERR_ARG, f"EPG-X network layer fc{l}: Weights {list(W.shape)} / Bias "
+                               f"{b.size} do not follow {widths[-1]} inputs")
+        widths.append(W.shape[0])
+        parts += [W.ravel(order="F"), b]
+        l += 1
+    if len(widths) < 2 or widths[-1] != n_out:
+        raise GacelleError(_abi.GC_ERR_ARG, f"EPG-X network: {n_out} output(s) expected, layers {widths}")
+    if len(widths) - 1 > _abi.GC_MLP_MAX_LAYERS or max(widths) > _abi.GC_MLP_MAX_WIDTH:
+        raise GacelleError(_abi.GC_ERR_UNSUPPORTED, f"EPG-X network {widths}: at most {_abi.GC_MLP_MAX_LAYERS} layers "
+                           f"of at most {_abi.GC_MLP_MAX_WIDTH} units")
+    return widths, alpha, np.ascontiguousarray(np.concatenate(parts), np.float32)
+
+
 # ===========================================================================
 class gpuMCRMWI(_ModelBase):
     """Multi-compartment relaxometry myelin water imaging (3-pool complex
     GRE with VFA T1 weighting), MCRMWI/gpuMCRMWI.m.
 
     The device model covers the DIMWI-fitted path (isFitFreqMW,
-    isFitFreqIW, isFitR2sEW and isFitIWF true) with isEPG = false: the
-    T1 steady state is the Bloch-McConnell analytical solution when
-    isFitExchange, else the non-exchanging Bloch model.  Complex data fit
-    dfreqBKG (one map per flip angle) and dpini.  Four flip angles.
+    isFitFreqIW, isFitR2sEW and isFitIWF true).  The T1 steady state is the
+    Bloch-McConnell analytical solution when isFitExchange, else the
+    non-exchanging Bloch model; with isEPG (and isFitExchange) the EPG-X
+    MLPs add their phase of the long-T2 pools and their corrections of the
+    BM steady state (networks from fitting['epgx_phase_ann'] and
+    fitting['epgx_mag_ann'], see mlp_pack).  Complex data fit dfreqBKG (one
+    map per flip angle) and dpini.  Four flip angles.
     """
 
     modelParams = ["S0", "MWF", "IWF", "R1IEW", "kIEWM", "R2sMW", "R2sIW", "R2sEW", "freqMW", "freqIW", "dfreqBKG",
@@ -1053,8 +1096,10 @@ class gpuMCRMWI(_ModelBase):
             dimwi.setdefault(k, True)
         f2["DIMWI"] = dimwi
         f2.setdefault("isFitExchange", True)
-        # the EPG-X MLP path (reference default) is not part of this library
-        f2.setdefault("isEPG", False)
+        # The reference defaults to the EPG-X path with its bundled networks
+        # (MATLAB dlnet objects this library does not read): here isEPG is on
+        # when the caller passes the networks, otherwise the BM path runs.
+        f2.setdefault("isEPG", fitting.get("epgx_phase_ann") is not None and fitting.get("epgx_mag_ann") is not None)
         f2.setdefault("isComplex", True)
         if data is not None and not np.iscomplexobj(data):
             f2["isComplex"] = False
@@ -1217,8 +1262,10 @@ class gpuMCRMWI(_ModelBase):
         return w.astype(np.float32)
 
     def fit(self, data, mask=None, fitting=None, extraData=None):
-        """MCRMWI/gpuMCRMWI.m:320-438 (isEPG = false: no pre-trained network
-        is loaded).  data: complex [x,y,z,nte,nfa] (or magnitude)."""
+        """MCRMWI/gpuMCRMWI.m:320-438.  With isEPG the networks come from
+        fitting['epgx_phase_ann'] / fitting['epgx_mag_ann'] (dicts or .npz
+        paths, mlp_pack) instead of the reference's .mat files.  data: complex
+        [x,y,z,nte,nfa] (or magnitude)."""
         data = np.asarray(data)
         dims = utils.dims3(data)
         if mask is None:
@@ -1264,6 +1311,16 @@ class gpuMCRMWI(_ModelBase):
             out["min"]["S0"] = out["min"]["S0"] * scale
         return out
 
+    def bind_device(self, md, ctx, keep):
+        """Point md's network parameters at device copies (kept alive in keep)."""
+        if not (md.flags & _abi.GC_FLAG_MCRMWI_EPG):
+            return md
+        for n, packed in enumerate(self._mlp_host):
+            t = ctx.to_device(packed)
+            keep.append(t)
+            md.mlp[n].params = runtime.ptr(t)
+        return md
+
     def _flags(self, fitting):
         f = 0
         if fitting.get("isComplex", True):
@@ -1275,13 +1332,36 @@ class gpuMCRMWI(_ModelBase):
         return f
 
     def gc_model(self, fitting, solver, nmeas):
+        """The C-ABI description.  With isEPG the two networks
+        (fitting['epgx_phase_ann'], fitting['epgx_mag_ann']) are packed here and
+        md.mlp[].params point at HOST copies (the oracle's form); bind_device()
+        swaps in device copies before a library call."""
         dimwi = fitting.get("DIMWI", {})
-        if fitting.get("isEPG", False) or not all(dimwi.get(k, True) for k in
-                                                  ("isFitFreqMW", "isFitFreqIW", "isFitR2sEW", "isFitIWF")):
+        if not all(dimwi.get(k, True) for k in ("isFitFreqMW", "isFitFreqIW", "isFitR2sEW", "isFitIWF")):
             raise GacelleError(_abi.GC_ERR_UNSUPPORTED,
-                               "gpuMCRMWI: the EPG-X and HCFM paths are not available (isEPG=false, DIMWI fitted)")
+                               "gpuMCRMWI: the HCFM paths are not available (DIMWI parameters fitted)")
+        epg = bool(fitting.get("isEPG", False))
+        if epg and not fitting.get("isFitExchange", True):
+            raise GacelleError(_abi.GC_ERR_ARG, "gpuMCRMWI: isEPG needs isFitExchange")
         md = self._base_model(fitting)
         md.flags = self._flags(fitting)
+        if epg:
+            md.flags |= _abi.GC_FLAG_MCRMWI_EPG
+            self._mlp_host = []
+            for n, key in enumerate(("epgx_phase_ann", "epgx_mag_ann")):
+                if fitting.get(key) is None:
+                    raise GacelleError(_abi.GC_ERR_ARG, f"gpuMCRMWI: isEPG needs fitting['{key}']: the network as "
+                                       "{'parameters': {'fc1': {'Weights', 'Bias'}, ...}} or an .npz of fc<l>_Weights "
+                                       "/ fc<l>_Bias (the reference's .mat files hold MATLAB dlnet objects, which "
+                                       "this library does not read)")
+                widths, alpha, packed = mlp_pack(fitting[key], n_out=1 if n == 0 else 2)
+                m = md.mlp[n]
+                m.n_layer = len(widths) - 1
+                for l, wl in enumerate(widths):
+                    m.width[l] = wl
+                m.alpha = alpha
+                m.params = packed.ctypes.data
+                self._mlp_host.append(packed)
         nte, nfa = self.te.size, self.fa.size
         md.i[0], md.i[1] = nte, nfa
         md.n_meas = nte * nfa * (2 if md.flags & _abi.GC_FLAG_MCRMWI_COMPLEX else 1)

@@ -34,11 +34,13 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 5
+#define GC_ABI_VERSION 6
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
 #define GC_MAX_REG 4    /* regularised parameter maps             */
+#define GC_MLP_MAX_LAYERS 6  /* fully connected layers per network     */
+#define GC_MLP_MAX_WIDTH 64  /* units per layer (inputs included)      */
 
 /* ---- status codes (mirroring the MATLAB error identifiers) ------------- */
 enum {
@@ -57,7 +59,7 @@ enum {
   GC_MODEL_R2STAR = 1,         /* R2star/gpuR2starMapping.m                 */
   GC_MODEL_JOINT_R1R2STAR = 2, /* R1R2s/gpuJointR1R2starMapping.m           */
   GC_MODEL_NEXI = 3,           /* NEXI/gpuNEXI.m, NEXI/gpuNEXIrice.m        */
-  GC_MODEL_MCRMWI = 4          /* MCRMWI/gpuMCRMWI.m (BM / Bloch T1 paths)  */
+  GC_MODEL_MCRMWI = 4          /* MCRMWI/gpuMCRMWI.m (BM / Bloch / EPG-X T1) */
 };
 
 /* model option flags (gc_model.flags) */
@@ -69,9 +71,29 @@ enum {
   GC_FLAG_MCRMWI_EXCHANGE = 1 << 4, /* BM two-pool exchange steady state (kIEWM fitted) */
   GC_FLAG_MCRMWI_PINI_FA = 1 << 6,  /* one initial phase per flip angle: extra
                                      * rows 3+n_fa..3+2n_fa replace row 1    */
-  GC_FLAG_UNIFORM_TE = 1 << 5       /* echo times equally spaced: f[8] = TE spacing;
+  GC_FLAG_UNIFORM_TE = 1 << 5,      /* echo times equally spaced: f[8] = TE spacing;
                                        exp(-TE_k R) is formed by recurrence */
+  GC_FLAG_MCRMWI_EPG = 1 << 7       /* EPG-X MLP T1 path (isEPG, needs EXCHANGE):
+                                     * gc_model.mlp[0] = phase network,
+                                     * mlp[1] = magnitude network           */
 };
+
+/* A multilayer perceptron of the MCR-MWI EPG-X path
+ * (MCRMWI/EPGXgen_net/mlp_model_leakyRelu.m:23-47): fc1, then leaky ReLU
+ * and fc_l for l = 2..n_layer.  params points at, per layer l = 1..n_layer,
+ * Weights [width[l] x width[l-1]] row-major (MATLAB dlnet.parameters.fc<l>
+ * .Weights, [out, in]) followed by Bias [width[l]], FP32.  width[0] = 11
+ * (the features of feature_preprocess_MCRMWI_MLP_EPGX_leakyrelu.m:24-95);
+ * width[n_layer] = 1 (phase network) or 2 (magnitude network: the IEW, then
+ * the MW steady-state difference).  The library reads params as device
+ * memory; the oracle reads it as host memory. */
+typedef struct gc_mlp {
+  int32_t n_layer;
+  int32_t width[GC_MLP_MAX_LAYERS + 1];
+  float alpha;        /* leaky ReLU scale (dlnet.alpha = 0.01, gpuMCRMWI.m:386-387) */
+  int32_t reserved;
+  const float* params;
+} gc_mlp;
 
 /* NEXI quadrature kinds (gc_model.i[3]) */
 enum { GC_QUAD_TRAPZ = 0, GC_QUAD_TABLE = 1 };
@@ -120,6 +142,7 @@ typedef struct gc_model {
   float table[GC_MAX_TABLE];
   float lb[GC_MAX_PAR]; /* fitting.lb, one entry per slot          */
   float ub[GC_MAX_PAR]; /* fitting.ub                              */
+  gc_mlp mlp[2];        /* GC_FLAG_MCRMWI_EPG: phase and magnitude networks */
 } gc_model;
 
 /* ---- askadam configuration: POD mirror of askadam.check_set_default_basic

@@ -65,6 +65,9 @@ template <class S, int N> inline Dual<S, N> vmin(const Dual<S, N>& a, S b) { ret
 template <class S, int N> inline S value(const Dual<S, N>& a) { return a.v; }
 template <class S, int N> inline bool finite(const Dual<S, N>& a) { return std::isfinite(a.v); }
 
+template <class S, int N> inline S vval(const Dual<S, N>& a) { return a.v; }
+inline double vval(double a) { return a; }
+inline float vval(float a) { return a; }
 inline double vmax(double a, double b) { return a >= b ? a : b; }
 inline double vmin(double a, double b) { return a <= b ? a : b; }
 inline float vmax(float a, float b) { return a >= b ? a : b; }

@@ -181,6 +181,53 @@ void bm_2t1(S TR, S famp, const V& f, const V& R1f, S R1r, const V& kfr, V& S0M,
   S0M = ((cp * d - c * dp) * R1f * (S(1) - f) + (-cp * b + a * dp) * R1r * f) * C;
 }
 
+// feature_preprocess_MCRMWI_MLP_EPGX_leakyrelu.m:24-95: the 11 features of a
+// (voxel, flip angle) sample
+template <class V, class S>
+void epgx_features(const V& fm, const V& t1iew, const V& kiewmw, S fa, S tr, const V& t2iew, S t1mw, V* x) {
+  const V kmwiew = (vval(fm) == S(0)) ? V(S(0)) : kiewmw * (S(1) - fm) / fm;   // :59-60
+  const S sa = S(std::sin(fa)), ca = S(std::cos(fa));
+  x[0] = fm;                                                      // :65
+  x[1] = exp(-(tr / t1iew));                                      // :66 E1
+  x[2] = exp(-(tr * kiewmw));                                     // :67
+  x[3] = V(sa);                                                   // :68
+  x[4] = S(1) - x[1] * ca;                                        // :71
+  x[5] = exp(-(tr * kmwiew));                                     // :72
+  x[6] = exp(-(tr * (S(1) / t1iew + kiewmw)));                    // :73
+  x[7] = exp(-(tr * (S(1) / t1mw + kmwiew)));                     // :74
+  // model_GRE_steadystate_Bloch (:97-100): m0 (1-E1) sin(a) / (1 - E1 cos(a))
+  const V e1 = exp(-(tr / t1iew));
+  x[8] = ((S(1) - fm) * (S(1) - e1) * sa) / (S(1) - e1 * ca);    // :77
+  const S e1m = S(std::exp(-tr / t1mw));
+  x[9] = (fm * (S(1) - e1m) * sa) / (S(1) - e1m * ca);           // :78
+  x[10] = exp(-(tr * (S(1) / t2iew)));                            // :81
+}
+
+// mlp_model_leakyRelu.m:23-47: fc1, then leaky ReLU and fc_l for the other
+// layers.  Weights are column-major [out, in] (MATLAB's own storage),
+// followed by the bias, layer after layer.
+template <class V, class S>
+void mlp_eval(const gc_mlp& net, const V* x, V* out) {
+  V a[GC_MLP_MAX_WIDTH], z[GC_MLP_MAX_WIDTH];
+  for (int i = 0; i < net.width[0]; ++i) a[i] = x[i];
+  const float* p = net.params;
+  for (int l = 1; l <= net.n_layer; ++l) {
+    const int ni = net.width[l - 1], no = net.width[l];
+    if (l > 1)
+      for (int i = 0; i < ni; ++i) a[i] = (vval(a[i]) > S(0)) ? a[i] : S(net.alpha) * a[i];  // leakyrelu
+    const float* W = p;
+    const float* b = p + size_t(no) * ni;
+    for (int j = 0; j < no; ++j) {
+      V t = V(S(b[j]));
+      for (int i = 0; i < ni; ++i) t = t + S(W[j + size_t(no) * i]) * a[i];
+      z[j] = t;
+    }
+    for (int j = 0; j < no; ++j) a[j] = z[j];
+    p = b + no;
+  }
+  for (int j = 0; j < net.width[net.n_layer]; ++j) out[j] = a[j];
+}
+
 // Number of model parameters of the MCRMWI slot layout for the given flags.
 inline int mcrmwi_npar(const gc_model& md) {
   int p = 10;
@@ -190,7 +237,7 @@ inline int mcrmwi_npar(const gc_model& md) {
 }
 
 // MCR-MWI forward model, MCRMWI/gpuMCRMWI.m:566-714 with the DIMWI terms
-// fitted (no HCFM), isEPG = false, and compute_gremwi_signal
+// fitted (no HCFM), the EPG-X MLP, BM or Bloch T1 path, and compute_gremwi_signal
 // (private/compute_gremwi_signal.m:1-15).  The flip-angle axis is kept in
 // dimension 5 for both T1 paths (the Bloch path's shiftdim at :691 is taken
 // to mean the same axis).
@@ -226,8 +273,18 @@ void fwd_mcrmwi(const gc_model& md, const V* th, const S* extra, V* s) {
   const S ff = extra[2];
   for (int fi = 0; fi < nfa; ++fi) {
     const S famp = S(md.table[nte + fi]) * b1;
-    V aMW, aIEW;
-    if (exch) {
+    V aMW, aIEW, ph = V(S(0));
+    if (exch && (md.flags & GC_FLAG_MCRMWI_EPG)) {
+      // EPG-X (MCRMWI/gpuMCRMWI.m:664-681): the MLPs' phase of the long-T2
+      // pools and their corrections of the BM steady state
+      V sm, siew, x[11], sd[2];
+      bm_2t1<V, S>(TR, famp, mvf, r1iew, S(1) / t1mw, kiewm, sm, siew);
+      epgx_features<V, S>(mvf, S(1) / r1iew, kiewm, famp, TR, S(1) / r2sIW, t1mw, x);
+      mlp_eval<V, S>(md.mlp[0], x, &ph);
+      mlp_eval<V, S>(md.mlp[1], x, sd);
+      aMW = vmax((sm + sd[1]) * tv * rho, S(1e-8));   // :677
+      aIEW = vmax((siew + sd[0]) * tv, S(1e-8));      // :678
+    } else if (exch) {
       V sm, siew;
       bm_2t1<V, S>(TR, famp, mvf, r1iew, S(1) / t1mw, kiewm, sm, siew);
       aMW = sm * tv * rho;
@@ -252,8 +309,9 @@ void fwd_mcrmwi(const gc_model& md, const V* th, const S* extra, V* s) {
       const S te = S(md.table[t]);
       const S w0 = te * S(2) * S(kPi);
       const V pMW = w0 * (freqMW + fbkg) * B0 * gyro + pini;
-      const V pIW = w0 * (freqIW + fbkg) * B0 * gyro + pini;
-      const V pEW = w0 * (freqEW + fbkg) * B0 * gyro + pini;
+      // compute_gremwi_signal: the IEW phase offsets IW and EW (:8-9)
+      const V pIW = w0 * (freqIW + fbkg) * B0 * gyro + pini + ph;
+      const V pEW = w0 * (freqEW + fbkg) * B0 * gyro + pini + ph;
       const V dMW = aMW * exp(-(te * r2sMW));
       const V dIW = aIW * exp(-(te * r2sIW));
       const V dEW = aEW * exp(-(te * r2sEW));

@@ -1333,6 +1333,25 @@ int orc_tv(const double* img, const uint8_t* mask, const int32_t dims[3], int32_
   return 0;
 }
 
+// The EPG-X pre-pass of one (voxel, flip angle) sample: out[15] = phase,
+// IEW and MW corrections, each followed by its derivatives with respect to
+// (MVF, R1IEW, kIEWM, R2sIW), from the features and networks of
+// fwd_mcrmwi's EPG path, differentiated with duals.
+int orc_epgx_sample(const gc_model* md, double mvf, double r1iew, double kfr, double r2siw, double fa, double* out) {
+  using D = orc::Dual<double, 4>;
+  const D fm = D::var(mvf, 0), r1 = D::var(r1iew, 1), kf = D::var(kfr, 2), r2 = D::var(r2siw, 3);
+  D x[11], ph, sd[2];
+  orc::epgx_features<D, double>(fm, 1.0 / r1, kf, fa, double(md->f[0]), 1.0 / r2, double(md->f[4]), x);
+  orc::mlp_eval<D, double>(md->mlp[0], x, &ph);
+  orc::mlp_eval<D, double>(md->mlp[1], x, sd);
+  const D* o[3] = {&ph, &sd[0], &sd[1]};
+  for (int k = 0; k < 3; ++k) {
+    out[5 * k] = o[k]->v;
+    for (int q = 0; q < 4; ++q) out[5 * k + 1 + q] = o[k]->d[q];
+  }
+  return 0;
+}
+
 int orc_surface_tv(const double* img, const uint8_t* mask, int64_t ngrid, const int32_t* nbr, const float* elen,
                    int32_t max_nb, double lambda, int64_t nmasked, double* loss, double* grad) {
   for (int64_t i = 0; i < int64_t(max_nb) * ngrid; ++i)

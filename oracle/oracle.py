@@ -57,6 +57,8 @@ def lib():
         L.orc_askadam.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                   POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_double), c_int32, c_int32]
+        L.orc_epgx_sample.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_double,
+                                      POINTER(c_double)]
         L.orc_surface_tv.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int32, c_double, c_int64,
                                      POINTER(c_double), c_void_p]
         L.orc_mcmc.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -173,6 +175,13 @@ def tv(img, mask, mode, vsize, lam, nmasked, grad=True):
     lib().orc_tv(img.ctypes.data_as(c_void_p), mask.ctypes.data_as(c_void_p), dims, mode, vs, lam, nmasked,
                  ctypes.byref(loss), None if g is None else g.ctypes.data_as(c_void_p))
     return (loss.value, g) if grad else loss.value
+
+
+def epgx_sample(md, mvf, r1iew, kfr, r2siw, fa):
+    """The EPG-X outputs of one sample and their derivatives (15 values)."""
+    out = (c_double * 15)()
+    lib().orc_epgx_sample(ctypes.byref(md), mvf, r1iew, kfr, r2siw, fa, out)
+    return np.array(out[:])
 
 
 def surface_tv(img, mask, nbr, elen, lam, nmasked, grad=True):

@@ -315,10 +315,25 @@ def bench_c4(ctx, device, iterations, nv=500_000, walkers=16):
                 transcendentals_per_s=steps * 3072 / t, posterior_mean_median_rel_err=err)
 
 
-def bench_c5(ctx, device, iterations, dims=(128, 128, 64)):
+def epgx_networks(seed=17, hidden=(64, 64, 64)):
+    """Random EPG-X networks (dlnet.parameters layout) of the given hidden
+    widths: the reference's trained networks are MATLAB objects this build
+    does not read, so the EPG-X workload times networks of that shape."""
+    rng = np.random.default_rng(seed)
+
+    def net(n_out, scale):
+        w = [11, *hidden, n_out]
+        return {"parameters": {f"fc{l}": {"Weights": rng.normal(0, scale / np.sqrt(w[l - 1]), (w[l], w[l - 1])),
+                                          "Bias": rng.normal(0, 0.05, w[l])} for l in range(1, len(w))}}
+
+    return net(1, 0.3), net(2, 0.05)
+
+
+def bench_c5(ctx, device, iterations, dims=(128, 128, 64), epg=False):
     """MCR-MWI complex 3-pool GRE (BM exchange), 24 echoes x 4 flip angles,
     complex Gaussian SNR 150, askadam with 3D TV on MWF (one launch per
-    iteration through the step kernel)."""
+    iteration through the step kernel).  epg: the EPG-X MLP T1 path on top
+    (the k_epgx pre-pass before every step kernel)."""
     import torch
 
     from gacelle_amd import _abi, models, runtime, utils
@@ -326,13 +341,20 @@ def bench_c5(ctx, device, iterations, dims=(128, 128, 64)):
 
     te = np.arange(2, 49, 2) * 1e-3
     m = models.gpuMCRMWI(te, 38e-3, np.array([5.0, 10.0, 20.0, 50.0]))
-    f = m.check_set_default(dict(isComplex=True, isFitExchange=True))
+    opts = dict(isComplex=True, isFitExchange=True)
+    if epg:
+        nets = epgx_networks()
+        opts.update(epgx_phase_ann=nets[0], epgx_mag_ann=nets[1])
+    f = m.check_set_default(opts)
     m.updateProperty(f)
     f.update(modelParams=list(m.modelParams), lb=list(m.lb), ub=list(m.ub), iteration=iterations, tol=0,
              convergenceValue=-1, convergenceStepTol=0, convergenceGradTol=0, TVmode="3D", voxelSize=[1, 1, 1],
              verbose=False, **{"lambda": [1e-3]})
     f = askadam.check_set_default_basic(f)
     md = m.gc_model(f, "askadam", None)
+    keep = []
+    if epg:
+        md = m.bind_device(md, ctx, keep)
     cfg = askadam().make_cfg(f, m)
     nx, ny, nz = dims
     nv = nx * ny * nz
@@ -385,12 +407,20 @@ def bench_c5(ctx, device, iterations, dims=(128, 128, 64)):
     t, v = timed(lambda: run(cfg))
     vi = float(nv) * iterations
     bytes_per_vi = 4 * md.n_meas + 2 * 3 * md.n_par * 4 + 6 * 4  # y, (z, m, v) r/w, TV neighbour reads
-    return dict(workload=f"MCR-MWI complex BM 24 TE x 4 FA, {dims[0]}x{dims[1]}x{dims[2]} ({nv} voxels), 3D TV on "
-                         f"MWF lambda 1e-3, {iterations} iterations L1/Adam, one launch per iteration",
+    path = "EPG-X MLP (networks 11-64-64-64-1 / -2, random weights) on BM" if epg else "BM"
+    extra_out = {}
+    if epg:
+        # per (voxel, flip angle): forward passes of both networks and one
+        # reverse pass per output (3), each sum_l w_l w_(l-1) FMAs
+        fma = lambda w: sum(w[l] * w[l - 1] for l in range(1, len(w)))
+        per_sample = 2 * fma([11, 64, 64, 64, 1]) + 3 * fma([11, 64, 64, 64, 2])
+        extra_out = dict(epgx_flop_per_voxel_iteration=2 * 4 * per_sample)
+    return dict(workload=f"MCR-MWI complex {path} 24 TE x 4 FA, {dims[0]}x{dims[1]}x{dims[2]} ({nv} voxels), 3D TV "
+                         f"on MWF lambda 1e-3, {iterations} iterations L1/Adam, one step launch per iteration",
                 seconds=t, voxel_iterations_per_s=vi / t, kernel_ms=v.kernel_ms, launches=v.kernel_launches,
                 avg_launch_ms=v.kernel_ms / max(v.kernel_launches, 1),
                 hbm_gbs_algorithmic=vi * bytes_per_vi / (v.kernel_ms * 1e-3) / 1e9,
-                loss=v.loss, loss_reg=v.loss_reg)
+                loss=v.loss, loss_reg=v.loss_reg, **extra_out)
 
 
 def bench_tv_small(ctx, device, iterations, dims=(100, 100, 1)):
@@ -569,8 +599,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mh", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--configs", default="c3,c4,c5,tvsmall,dict",
-                    help="further workloads to measure (c3, c4, c5, tvsmall, dict or none)")
+    ap.add_argument("--configs", default="c3,c4,c5,c5epg,tvsmall,dict",
+                    help="further workloads to measure (c3, c4, c5, c5epg, tvsmall, dict or none)")
     ap.add_argument("--c3-iterations", type=int, default=2000)
     ap.add_argument("--c4-iterations", type=int, default=500)
     ap.add_argument("--c5-iterations", type=int, default=200)
@@ -681,7 +711,9 @@ def main():
     extra = {}
     wanted = [c.strip() for c in args.configs.split(",") if c.strip() and c.strip() != "none"]
     for name, fn, it in (("c3", bench_c3, args.c3_iterations), ("c4", bench_c4, args.c4_iterations),
-                         ("c5", bench_c5, args.c5_iterations), ("tvsmall", bench_tv_small, args.iterations),
+                         ("c5", bench_c5, args.c5_iterations),
+                         ("c5epg", lambda c, d, it: bench_c5(c, d, it, epg=True), args.c5_iterations // 4),
+                         ("tvsmall", bench_tv_small, args.iterations),
                          ("dict", bench_dict, args.steps)):
         if name in wanted:
             extra[name] = fn(ctx, device, it)

@@ -32,7 +32,11 @@ constexpr size_t epgx_layer_floats() { return size_t(WM) * WM + WM; }
 
 // Forward pass of one network on a[] (inputs, zero beyond width[0]); on
 // return a[] holds the outputs.  The sign mask of hidden layer l goes to
-// msk[l * kBlock] (this thread's column of an LDS table).
+// msk[l * kBlock] (this thread's column of an LDS table).  Zero rows beyond
+// a layer's width and zero chunks of 16 inputs are skipped by uniform
+// branches.  (Straight-line row groups, which let the scalar loads run
+// further ahead, raise the 64-unit kernel above 256 registers: one wave per
+// SIMD, measured no faster.)
 template <int WM>
 GC_DEV void mlp_fwd(const float* __restrict__ W, int nl, const int32_t* width, float alpha, float (&a)[WM],
                     uint64_t* msk) {
@@ -106,50 +110,60 @@ GC_DEV void mlp_rev(const float* __restrict__ W, int nl, const int32_t* width, f
   }
 }
 
-// One (voxel, flip angle) sample: out[15] = {ph, dph/d(MVF, R1IEW, kIEWM,
-// R2sIW), dIEW, d.../d(...), dMW, d.../d(...)}.  fa = nominal flip angle x B1
-// [rad]; msk points at this thread's column of the mask table.
+// One (voxel, flip angle) sample: rows k * nfa + fi of aux (stride nv) get
+// {ph, dph/d(MVF, R1IEW, kIEWM, R2sIW), dIEW, d.../d(...), dMW, d.../d(...)}.
+// fa = nominal flip angle x B1 [rad]; msk and xs point at this thread's
+// columns of the LDS tables (sign masks; the 11 features and their 4
+// derivatives, kept out of registers while the networks run).
 template <int WM>
 GC_DEV void epgx_sample(const EpgxNets& N, float mvf, float r1iew, float kfr, float r2siw, float fa, float tr,
-                        float t1mw, uint64_t* msk, float* out) {
-  using D4 = DualF<4>;
-  const D4 fm = D4::var(mvf, 0), r1 = D4::var(r1iew, 1), kf = D4::var(kfr, 2), r2 = D4::var(r2siw, 3);
-  float sa, ca;
-  sincosf(fa, &sa, &ca);
-  const D4 kmw = (kf * (1.f - fm)) / fm;
-  D4 x[11];
-  x[0] = fm;
-  x[1] = dexp(-tr * r1);                       // exp(-TR / T1iew)
-  x[2] = dexp(-tr * kf);                       // exp(-TR kiewmw)
-  x[3] = D4::cst(sa);
-  x[4] = 1.f - ca * x[1];                      // 1 - E1 cos(alpha)
-  x[5] = dexp(-tr * kmw);                      // exp(-TR kmwiew)
-  x[6] = dexp(-tr * (r1 + kf));
-  x[7] = dexp(-tr * ((1.f / t1mw) + kmw));
-  x[8] = (sa * ((1.f - fm) * (1.f - x[1]))) / (1.f - ca * x[1]);   // Bloch steady state of IEW
-  const float e1m = __expf(-tr / t1mw);
-  x[9] = (sa * (1.f - e1m) / (1.f - ca * e1m)) * fm;                // ... of M
-  x[10] = dexp(-tr * r2);                      // exp(-TR / T2iew)
+                        float t1mw, uint64_t* msk, float* xs, float* __restrict__ aux, int64_t rstride) {
+  {
+    using D4 = DualF<4>;
+    const D4 fm = D4::var(mvf, 0), r1 = D4::var(r1iew, 1), kf = D4::var(kfr, 2), r2 = D4::var(r2siw, 3);
+    float sa, ca;
+    sincosf(fa, &sa, &ca);
+    const D4 kmw = (kf * (1.f - fm)) / fm;
+    D4 x[11];
+    x[0] = fm;
+    x[1] = dexp(-tr * r1);                       // exp(-TR / T1iew)
+    x[2] = dexp(-tr * kf);                       // exp(-TR kiewmw)
+    x[3] = D4::cst(sa);
+    x[4] = 1.f - ca * x[1];                      // 1 - E1 cos(alpha)
+    x[5] = dexp(-tr * kmw);                      // exp(-TR kmwiew)
+    x[6] = dexp(-tr * (r1 + kf));
+    x[7] = dexp(-tr * ((1.f / t1mw) + kmw));
+    x[8] = (sa * ((1.f - fm) * (1.f - x[1]))) / (1.f - ca * x[1]);   // Bloch steady state of IEW
+    const float e1m = __expf(-tr / t1mw);
+    x[9] = (sa * (1.f - e1m) / (1.f - ca * e1m)) * fm;                // ... of M
+    x[10] = dexp(-tr * r2);                      // exp(-TR / T2iew)
+#pragma unroll
+    for (int f = 0; f < 11; ++f) {
+      xs[(f * 5) * kBlock] = x[f].v;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xs[(f * 5 + 1 + q) * kBlock] = x[f].d[q];
+    }
+  }
   int k = 0;
 #pragma unroll 1
   for (int net = 0; net < 2; ++net) {
     float a[WM];
 #pragma unroll
-    for (int i = 0; i < WM; ++i) a[i] = (i < 11) ? x[i < 11 ? i : 0].v : 0.f;
+    for (int i = 0; i < WM; ++i) a[i] = (i < 11) ? xs[((i < 11 ? i : 0) * 5) * kBlock] : 0.f;
     mlp_fwd<WM>(N.w[net], N.nl[net], N.width[net], N.alpha[net], a, msk);
     const int nout = N.width[net][N.nl[net]];
-    float val[2] = {a[0], a[1]};
+    const float val[2] = {a[0], a[1]};
 #pragma unroll 1
     for (int o = 0; o < nout; ++o) {
       float g[WM];
       mlp_rev<WM>(N.w[net], N.nl[net], N.width[net], N.alpha[net], o, msk, g);
-      out[k] = val[o];
+      aux[int64_t(k) * rstride] = val[o];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         float s = 0.f;
 #pragma unroll
-        for (int f = 0; f < 11; ++f) s = fmaf(g[f], x[f].d[q], s);
-        out[k + 1 + q] = s;
+        for (int f = 0; f < 11; ++f) s = fmaf(g[f], xs[(f * 5 + 1 + q) * kBlock], s);
+        aux[int64_t(k + 1 + q) * rstride] = s;
       }
       k += 5;
     }
@@ -166,12 +180,30 @@ GC_DEV float epgx_mvf(const float* th, float rho) {
   return fmaxf(((1.f / rho) * S0MW) / tv, 1e-8f);
 }
 
+// One voxel's flip angles.  The pointers are __restrict__ so that the
+// weight loads are known not to be clobbered by the aux stores: the weights
+// then come through scalar loads (a plain pointer to aux lets every store
+// alias them, and the loads turn into per-lane vector loads).
+template <int WM, class TF>
+GC_DEV void epgx_voxel(const gc_model& md, const EpgxNets& N, const float* __restrict__ z,
+                       const float* __restrict__ ex, float* __restrict__ aux, int64_t nv, int64_t v, TF&& transform,
+                       uint64_t* msk, float* xs) {
+  float th[7];
+#pragma unroll
+  for (int p = 0; p < 7; ++p) th[p] = transform(z[int64_t(p) * nv + v], md.lb[p], md.ub[p]);
+  const float mvf = epgx_mvf(th, md.f[3]);
+  const int nte = md.i[0], nfa = md.i[1];
+  const float b1 = ex[v];
+  for (int fi = 0; fi < nfa; ++fi)
+    epgx_sample<WM>(N, mvf, th[3], th[4], th[6], md.table[nte + fi] * b1, md.f[0], md.f[4], msk, xs,
+                    aux + int64_t(fi) * nv + v, int64_t(nfa) * nv);
+}
+
 // the step form: aux rows of every volume's voxels from z_in (same block
 // mapping as k_askadam_step)
 template <int WM, int TR>
 __global__ __launch_bounds__(kBlock) void k_epgx_step(const gc_model* __restrict__ mdp, const StepParams sp,
                                                       const EpgxNets N) {
-  const gc_model& md = *mdp;
   int vi = 0;
   for (int i = 1; i < sp.n_vol; ++i)
     if (int(blockIdx.x) >= sp.vols[i].blk0) vi = i;
@@ -181,22 +213,14 @@ __global__ __launch_bounds__(kBlock) void k_epgx_step(const gc_model* __restrict
   const int64_t v = int64_t(blockIdx.x - V.blk0) * kBlock + threadIdx.x;
   if (v >= nv) return;
   __shared__ uint64_t msk[GC_MLP_MAX_LAYERS][kBlock];
-  uint64_t* m = &msk[0][threadIdx.x];
-  float th[7];
-#pragma unroll
-  for (int p = 0; p < 7; ++p) {
-    float d;
-    transform_fwd_t<TR>(V.z_in[int64_t(p) * nv + v], md.lb[p], md.ub[p], th[p], d);
-  }
-  const float mvf = epgx_mvf(th, md.f[3]);
-  const int nte = md.i[0], nfa = md.i[1];
-  const float b1 = V.ex[v];
-  for (int fi = 0; fi < nfa; ++fi) {
-    float out[kEpgxRows];
-    epgx_sample<WM>(N, mvf, th[3], th[4], th[6], md.table[nte + fi] * b1, md.f[0], md.f[4], m, out);
-#pragma unroll
-    for (int k = 0; k < kEpgxRows; ++k) V.aux[(int64_t(k) * nfa + fi) * nv + v] = out[k];
-  }
+  __shared__ float xs[55][kBlock];
+  epgx_voxel<WM>(*mdp, N, V.z_in, V.ex, V.aux, nv, v,
+                 [](float z, float lb, float ub) {
+                   float t, d;
+                   transform_fwd_t<TR>(z, lb, ub, t, d);
+                   return t;
+                 },
+                 &msk[0][threadIdx.x], &xs[0][threadIdx.x]);
 }
 
 // the one-shot form (gc_fwd, gc_fwd_adj, the askadam outputs): aux rows
@@ -205,26 +229,17 @@ template <int WM>
 __global__ __launch_bounds__(kBlock) void k_epgx_eval(const gc_model* __restrict__ mdp, const float* __restrict__ x,
                                                       const float* __restrict__ ex, int64_t nv, int32_t transform,
                                                       float* __restrict__ aux, const EpgxNets N) {
-  const gc_model& md = *mdp;
   const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (v >= nv) return;
   __shared__ uint64_t msk[GC_MLP_MAX_LAYERS][kBlock];
-  uint64_t* m = &msk[0][threadIdx.x];
-  float th[7];
-#pragma unroll
-  for (int p = 0; p < 7; ++p) {
-    float d;
-    transform_fwd(transform, x[int64_t(p) * nv + v], md.lb[p], md.ub[p], th[p], d);
-  }
-  const float mvf = epgx_mvf(th, md.f[3]);
-  const int nte = md.i[0], nfa = md.i[1];
-  const float b1 = ex[v];
-  for (int fi = 0; fi < nfa; ++fi) {
-    float out[kEpgxRows];
-    epgx_sample<WM>(N, mvf, th[3], th[4], th[6], md.table[nte + fi] * b1, md.f[0], md.f[4], m, out);
-#pragma unroll
-    for (int k = 0; k < kEpgxRows; ++k) aux[(int64_t(k) * nfa + fi) * nv + v] = out[k];
-  }
+  __shared__ float xs[55][kBlock];
+  epgx_voxel<WM>(*mdp, N, x, ex, aux, nv, v,
+                 [transform](float z, float lb, float ub) {
+                   float t, d;
+                   transform_fwd(transform, z, lb, ub, t, d);
+                   return t;
+                 },
+                 &msk[0][threadIdx.x], &xs[0][threadIdx.x]);
 }
 
 }  // namespace gc

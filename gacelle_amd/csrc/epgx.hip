@@ -30,7 +30,9 @@ int epgx_prepare(gc_ctx* ctx, const gc_model& md, EpgxDev& out) {
   for (int n = 0; n < 2; ++n) {
     const gc_mlp& m = md.mlp[n];
     std::vector<float> h(count[n]);
-    GC_HIP(ctx, hipMemcpy(h.data(), m.params, sizeof(float) * count[n], hipMemcpyDeviceToHost));
+    // on the context's stream: the caller may have just written the parameters there
+    GC_HIP(ctx, hipMemcpyAsync(h.data(), m.params, sizeof(float) * count[n], hipMemcpyDeviceToHost, ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     size_t src = 0;
     for (int l = 1; l <= m.n_layer; ++l) {
       const int ni = m.width[l - 1], no = m.width[l];
@@ -47,7 +49,9 @@ int epgx_prepare(gc_ctx* ctx, const gc_model& md, EpgxDev& out) {
     base += per_layer * m.n_layer;
   }
   GC_HIP(ctx, out.wbuf.alloc(sizeof(float) * pad.size()));
-  GC_HIP(ctx, hipMemcpy(out.wbuf.p, pad.data(), sizeof(float) * pad.size(), hipMemcpyHostToDevice));
+  GC_HIP(ctx, hipMemcpyAsync(out.wbuf.p, pad.data(), sizeof(float) * pad.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+  GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // pad is freed on return
   out.wm = wm;
   const float* w0 = out.wbuf.as<float>();
   for (int n = 0; n < 2; ++n) {

@@ -402,11 +402,15 @@ static int partner_table(gc_ctx* ctx, const gc_mcmc_cfg* cfg, int W, int steps, 
       for (int k = 1; k <= steps; ++k)
         host_partners(cfg->seed, r, k, W, cfg->ensemble_update, &tab[(size_t(r) * steps + (k - 1)) * W], tag);
     GC_HIP(ctx, buf.alloc(n * sizeof(int32_t)));
-    GC_HIP(ctx, hipMemcpy(buf.p, tab.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    GC_HIP(ctx, hipMemcpyAsync(buf.p, tab.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // tab is freed on return
     *out = buf.as<int32_t>();
     return GC_OK;
   }
-  if (steps > 0) GC_HIP(ctx, hipMemcpy(tab.data(), replay_dev, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (steps > 0) {
+    GC_HIP(ctx, hipMemcpyAsync(tab.data(), replay_dev, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
   const int h = W / 2;
   for (size_t i = 0; i < n && steps > 0; ++i) {
     const int w = int(i % W);
@@ -739,13 +743,16 @@ int gc_mcmc_run(gc_ctx* ctx, const gc_model* md, const gc_mcmc_cfg* cfg, gc_mcmc
         for (int k = 1; k <= cfg->iteration; ++k)
           host_partners(cfg->seed, r, k, W, cfg->ensemble_update, &tab[(size_t(r) * cfg->iteration + (k - 1)) * W]);
       GC_HIP(ctx, dpart.alloc(n * sizeof(int32_t)));
-      GC_HIP(ctx, hipMemcpy(dpart.p, tab.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+      GC_HIP(ctx, hipMemcpyAsync(dpart.p, tab.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // tab is freed on return
       P.partners = dpart.as<int32_t>();
     } else {
       // replayed partner indices are checked before they index walker slots
       const size_t n = size_t(cfg->repetition) * std::max(cfg->iteration, 1) * W;
       std::vector<int32_t> tab(n);
-      GC_HIP(ctx, hipMemcpy(tab.data(), io->partners_dev, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+      GC_HIP(ctx, hipMemcpyAsync(tab.data(), io->partners_dev, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+      GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
       const int h = W / 2;
       for (size_t i = 0; i < n; ++i) {
         const int w = int(i % W);

@@ -470,8 +470,20 @@ struct StepParams {
   OptParams o;
 };
 
+// minimum waves per SIMD the step kernel is compiled for (a register
+// budget): models may ask for more than the default
+template <class Model>
+struct step_waves {
+  template <class M>
+  static constexpr int f(decltype(M::kStepWaves)*) { return M::kStepWaves; }
+  template <class M>
+  static constexpr int f(...) { return 1; }
+  static constexpr int value = f<Model>(nullptr);
+};
+
 template <class Model, int LOSS, bool HAS_W, int TR>
-__global__ __launch_bounds__(kBlock) void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(step_waves<Model>::value)))
+void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
   const gc_model& md = *mdp;
   constexpr int NP = Model::NP;
   int vi = 0;

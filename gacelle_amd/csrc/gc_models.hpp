@@ -422,6 +422,7 @@ template <int MF, bool EXCH, bool CPLX, bool EPG = false>
 struct McrmwiModel {
   static_assert(!EPG || EXCH, "the EPG-X path needs the exchange model");
   static constexpr bool kAux = EPG;
+  static constexpr int kStepWaves = 3;  // step kernel within 168 VGPRs
   static constexpr int NA = EXCH ? 5 : 4;
   static constexpr int NP = NA + 5 + (CPLX ? MF + 1 : 0);
   static constexpr int MT = 32;

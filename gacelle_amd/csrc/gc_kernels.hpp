@@ -1157,8 +1157,19 @@ __global__ __launch_bounds__(kBlock) void k_mh(const gc_model* __restrict__ mdp,
 // Goodman-Weare affine-invariant ensemble, utils/mcmc.m:192-383.  A group of
 // rep*W consecutive threads owns one voxel: walker w of repetition r at
 // group lane r*W + w.  Walker positions are exchanged through LDS.
+// minimum waves per SIMD the GW sampler is compiled for (a register budget)
+template <class Model>
+struct gw_waves {
+  template <class M>
+  static constexpr int f(decltype(M::kGwWaves)*) { return M::kGwWaves; }
+  template <class M>
+  static constexpr int f(...) { return 1; }
+  static constexpr int value = f<Model>(nullptr);
+};
+
 template <class Model, bool HAS_W>
-__global__ __launch_bounds__(kBlock) void k_gw(const gc_model* __restrict__ mdp, const McmcParams P) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(gw_waves<Model>::value)))
+void k_gw(const gc_model* __restrict__ mdp, const McmcParams P) {
   const gc_model& md = *mdp;
   constexpr int NP = Model::NP;
   constexpr int Q = NP + 1;

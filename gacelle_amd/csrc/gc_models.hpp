@@ -128,6 +128,7 @@ struct JointModel {
 template <int MS, bool L2>
 struct NexiModel {
   static constexpr int NP = L2 ? 5 : 4, MR = L2 ? 2 * MS : MS, NE = 1;
+  static constexpr int kGwWaves = 4;  // GW sampler within 128 VGPRs
   static constexpr bool kRelJ = false;
   static constexpr bool kFusedAdj = false, kStreamOnly = false;
   GC_DEV static void post_grad(const float*, float*) {}

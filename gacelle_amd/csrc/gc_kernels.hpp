@@ -206,8 +206,20 @@ GC_DEV int find_volume(const VolDev* vols, int n_vol, int blk) {
   return vi;
 }
 
+// minimum waves per SIMD the blocked askadam kernel is compiled for (a
+// register budget)
+template <class Model>
+struct ask_waves {
+  template <class M>
+  static constexpr int f(decltype(M::kAskWaves)*) { return M::kAskWaves; }
+  template <class M>
+  static constexpr int f(...) { return 1; }
+  static constexpr int value = f<Model>(nullptr);
+};
+
 template <class Model, int LOSS, bool HAS_W>
-__global__ __launch_bounds__(kBlock) void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ask_waves<Model>::value)))
+void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   const gc_model& md = *mdp;
   constexpr int NP = Model::NP;
   constexpr int MR = Model::MR;

@@ -74,6 +74,7 @@ struct R2sModel {
 // theta = {M0, R1, R2star}; register slot t + MT*f holds measurement t + nte*f.
 template <int MT, int MF>
 struct JointModel {
+  static constexpr int kAskWaves = 4;  // blocked askadam kernel within 128 VGPRs
   static constexpr int NP = 3, MR = MT * MF, NE = 1;
   static constexpr bool kRelJ = false;
   static constexpr bool kFusedAdj = false, kStreamOnly = false;

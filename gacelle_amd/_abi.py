@@ -10,7 +10,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 6
+GC_ABI_VERSION = 7
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
@@ -174,6 +174,8 @@ class gc_volume(ctypes.Structure):
         ("surf_len_dev", c_void_p),
         ("surf_max_nb", c_int32),
         ("reserved1", c_int32),
+        ("n_mean", c_int64),
+        ("sign_trace_dev", c_void_p),
     ]
 
 

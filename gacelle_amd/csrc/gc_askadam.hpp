@@ -18,6 +18,7 @@ namespace gc {
 // ---- askadam driver --------------------------------------------------------
 struct VolRun {
   int64_t nv = 0;
+  double nmean = 0;     // N of the loss mean (gc_volume.n_mean, else nv)
   int nblk = 0;
   DevBuf set[3][3];  // [state set][z, m, v]
   DevBuf zlast[2];
@@ -263,6 +264,10 @@ struct AskDriver {
     for (int i = 0; i < n_vol; ++i) {
       VolRun& R = run[i];
       R.nv = vols[i].nv;
+      if (vols[i].n_mean < 0) return set_err(ctx, GC_ERR_ARG, "volume %d: n_mean must be >= 0", i);
+      R.nmean = vols[i].n_mean > 0 ? double(vols[i].n_mean) : double(R.nv);
+      if (vols[i].sign_trace_dev && (step_mode || cfg.loss != GC_LOSS_L1))
+        return set_err(ctx, GC_ERR_UNSUPPORTED, "volume %d: the sign trace needs the L1 loss and the blocked form", i);
       R.reg_offset = prior ? vols[i].reg_offset : 0.0;
       R.nblk = int((R.nv + kBlock - 1) / kBlock);
       const size_t pbytes = size_t(P) * R.nv * sizeof(float);
@@ -519,6 +524,7 @@ struct AskDriver {
     if (desc_in_flight) GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     VolDev* hv = h_vols.as<VolDev>();
     EpochScal* hes = h_es.as<EpochScal>();
+    bool want_signs = false;
     int blk0 = 0;
     for (int i = 0; i < n_vol; ++i) {
       VolRun& R = run[i];
@@ -542,7 +548,9 @@ struct AskDriver {
       d.epoch0 = e0[i];
       d.n_epochs = n_ep[i];
       d.iteration = iteration_cap;
-      d.inv_count = float(1.0 / (double(md.n_meas) * double(R.nv)));
+      d.inv_count = float(1.0 / (double(md.n_meas) * R.nmean));
+      d.signs = vols[i].sign_trace_dev;
+      if (d.signs) want_signs = true;
       hv[i] = d;
       for (int k = 0; k < n_ep[i]; ++k) hes[size_t(i) * K + k] = epoch_scal(e0[i] + k);
       blk0 += R.nblk;
@@ -561,8 +569,17 @@ struct AskDriver {
     int rce = next_events(ev_a, ev_b);
     if (rce) return rce;
     // events attached to the dispatch itself time the kernel alone
-    hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
-                          ev_b, 0, dmd, ap);
+    if constexpr (LOSS == GC_LOSS_L1) {
+      if (want_signs)
+        hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W, true>), dim3(total_blk), dim3(kBlock), 0, ctx->stream,
+                              ev_a, ev_b, 0, dmd, ap);
+      else
+        hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
+                              ev_b, 0, dmd, ap);
+    } else {
+      hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
+                            ev_b, 0, dmd, ap);
+    }
     GC_HIP(ctx, hipGetLastError());
     ++kernel_launches;
     return reduce_partials(n_ep);
@@ -625,8 +642,8 @@ struct AskDriver {
           d.k = k;
           d.n_epochs = n;
           d.update = (e0[i] + k < iteration_cap) ? 1 : 0;
-          d.inv_count = float(1.0 / (double(md.n_meas) * double(R.nv)));
-          d.inv_nv = float(1.0 / double(R.nv));
+          d.inv_count = float(1.0 / (double(md.n_meas) * R.nmean));
+          d.inv_nv = float(1.0 / R.nmean);
           d.es = epoch_scal(e0[i] + k);
         }
         hs[size_t(k) * n_vol + i] = d;
@@ -708,9 +725,9 @@ struct AskDriver {
   // robustConvergence the mean per-voxel loss over the convergence set)
   bool epoch_logic(VolRun& R, int epoch, double sum_l, double sum_step_prev, double sum_den, double sum_g,
                    double sum_reg, double* trace, const double* loss_c_robust = nullptr) {
-    const double count = double(md.n_meas) * double(R.nv);
+    const double count = double(md.n_meas) * R.nmean;
     R.fid = sum_l / count;
-    R.reg = sum_reg / double(R.nv) + R.reg_offset;  // / Nmasked (+ the prior's out-of-mask constant)
+    R.reg = sum_reg / R.nmean + R.reg_offset;  // / Nmasked (+ the prior's out-of-mask constant)
     R.loss = R.fid + R.reg;
     if (trace) trace[epoch] = R.loss;
     R.last_epoch = epoch;
@@ -853,8 +870,8 @@ struct AskDriver {
       for (int i = 0; i < n_vol; ++i) {
         VolRun& R = run[i];
         const double* s = sums + size_t(i) * K * kSum;
-        R.fid = s[0] / (double(md.n_meas) * double(R.nv));
-        R.reg = s[4] / double(R.nv) + R.reg_offset;
+        R.fid = s[0] / (double(md.n_meas) * R.nmean);
+        R.reg = s[4] / R.nmean + R.reg_offset;
         R.loss = R.fid + R.reg;
         R.min_loss = R.loss; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = 0; R.last_epoch = 0;
         if (vols[i].loss_trace_host) vols[i].loss_trace_host[0] = R.loss;
@@ -896,8 +913,8 @@ struct AskDriver {
         }
         if (!R.have_init) {
           // starting-point evaluation (:197-219) equals the first iteration's
-          R.fid = s[0] / (double(md.n_meas) * double(R.nv));
-          R.reg = s[4] / double(R.nv) + R.reg_offset;
+          R.fid = s[0] / (double(md.n_meas) * R.nmean);
+          R.reg = s[4] / R.nmean + R.reg_offset;
           const double l0 = R.fid + R.reg;
           R.loss = l0;
           R.min_loss = l0; R.min_fid = R.fid; R.min_reg = R.reg; R.min_iter = 0;

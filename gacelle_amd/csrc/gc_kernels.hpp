@@ -161,6 +161,9 @@ struct VolDev {
   int32_t epoch0, n_epochs, iteration, pad;
   float inv_count;   // 1/(M*nv)
   float pad2[3];
+  // test hook (k_askadam<..., SIGNS = true>): residual signs, two bits per
+  // measurement, [iteration][ceil(M/16)][nv] words, row e-1 for epoch e
+  uint32_t* signs;
 };
 
 // optimiser of askadam.update_parameters (utils/askadam.m:673-691)
@@ -217,7 +220,11 @@ struct ask_waves {
   static constexpr int value = f<Model>(nullptr);
 };
 
-template <class Model, int LOSS, bool HAS_W>
+// SIGNS (L1 only): also write the sign code of every differentiated residual
+// to V.signs (1 = positive, 2 = negative, 0 = zero or a non-finite model
+// signal), so that an FP64 replay can follow the same subgradient path.  The
+// arithmetic is that of the SIGNS = false kernel; only the stores differ.
+template <class Model, int LOSS, bool HAS_W, bool SIGNS = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ask_waves<Model>::value)))
 void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   const gc_model& md = *mdp;
@@ -298,12 +305,18 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
       // pass takes the L1 derivative as copysign(w, r) and keeps min |r|: an
       // exact zero residual (sign 0) sends the voxel to the checked pass too.
       float amin = 1.f;
+      // sign words of this voxel and epoch (SIGNS): word j at sw[j * nv]
+      uint32_t* const sw = SIGNS ? V.signs + int64_t(e - 1) * ((md.n_meas + 15) >> 4) * nv + v : nullptr;
       auto accumulate = [&](auto checked) {
         constexpr bool CHECK = decltype(checked)::value;
         constexpr bool L1FAST = !CHECK && LOSS == GC_LOSS_L1;
         lsum = 0.f;
 #pragma unroll
         for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+        if constexpr (SIGNS) {
+          if (valid)
+            for (int j = 0; j < ((md.n_meas + 15) >> 4); ++j) sw[int64_t(j) * nv] = 0u;
+        }
         Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
           bool fin = true;
           float sv = s;
@@ -322,6 +335,14 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
           }
           lsum += l;
           if (!CHECK || fin) accum_grad<Model, L1FAST, HAS_W>(L1FAST ? copysignf(w, res) : dl, w, res, J, gth);
+          if constexpr (SIGNS) {
+            if (valid) {
+              const int m = Model::meas(md, r);
+              const uint32_t code = (!CHECK || fin) ? (res > 0.f ? 1u : (res < 0.f ? 2u : 0u)) : 0u;
+              uint32_t* const p = sw + int64_t(m >> 4) * nv;
+              *p = *p | (code << ((m & 15) * 2));
+            }
+          }
         });
       };
       accumulate(std::false_type{});

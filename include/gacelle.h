@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 6
+#define GC_ABI_VERSION 7
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
@@ -263,6 +263,20 @@ typedef struct gc_volume {
   const float* surf_len_dev;
   int32_t surf_max_nb;
   int32_t reserved1;
+  /* The voxel count N of the mean in askadam.model_gradient (loss =
+   * mean over N*M residuals, utils/askadam.m:96) and of the regulariser's
+   * 1/Nmasked; 0 = nv.  Set it to the whole problem's count when this
+   * volume is one piece (a shard) of a larger fit, so that its gradient
+   * scale, and with stopping off its trajectory, are those of the whole
+   * fit.  Scalar losses are reported with the same N. */
+  int64_t n_mean;
+  /* Test hook (L1 loss, blocked form only): when not NULL, receives the
+   * sign of every residual the fused kernel differentiates, two bits per
+   * measurement m in word m/16 at bit 2*(m%16): 1 = positive, 2 =
+   * negative, 0 = zero or non-finite model signal (no gradient).  Layout
+   * [iteration][ceil(M/16)][nv] of uint32, row e-1 for epoch e.  Lets an
+   * FP64 replay follow the same L1 subgradient path. */
+  uint32_t* sign_trace_dev;
 } gc_volume;
 
 /* ---- MCMC configuration: POD mirror of mcmc.check_set_default_basic

@@ -68,6 +68,17 @@ static inline void draw(uint64_t seed, uint32_t it, uint64_t chain, uint32_t pur
   philox4x32_10(ctr, key, out);
 }
 
+// Goodman-Weare native draws of walker `chain` at iteration `it` and
+// (red-black) half `sub`: one Philox call gives the stretch uniform (o[0])
+// and the acceptance uniform (o[1]).
+static inline void native_gw_uniforms(uint64_t seed, uint32_t it, uint64_t chain, int sub, double* u_zz,
+                                      double* u_acc) {
+  uint32_t o[4];
+  draw(seed, it, chain, PURPOSE_ZZ + uint32_t(sub), o);
+  *u_zz = u01(o[0]);
+  *u_acc = u01(o[1]);
+}
+
 // Box-Muller normal number q of a draw group (4 normals per Philox call).
 static inline double native_normal(uint64_t seed, uint32_t it, uint64_t chain, uint32_t base, int q) {
   uint32_t o[4];
@@ -219,9 +230,16 @@ static inline V unscale(int transform, const V& z, S lb, S ub) {
 
 // One voxel of askadam.model_gradient without the regulariser: the loss
 // elements and d(sum of loss elements)/dz.
+// codes (optional, L1 only): the sign of each residual to differentiate
+// with (1 = +, 2 = -, 0 = no gradient), e.g. the signs the device kernel
+// took at this iteration; the loss value stays w|r| of this evaluation.
+// The L1 subgradient w*sign(r) is discontinuous at r = 0, where an FP32 and
+// an FP64 evaluation can disagree on the sign: following one sign sequence
+// isolates the rest of the arithmetic.
 template <class S, int NP>
 static void eval_voxel(const gc_model& md, int loss_kind, int transform, const S* z, const float* y,
-                       const float* w, const S* extra, double& lsum, S* grad, S* elem) {
+                       const float* w, const S* extra, double& lsum, S* grad, S* elem,
+                       const uint8_t* codes = nullptr) {
   using D = Dual<S, NP>;
   const int P = md.n_par;
   D th[GC_MAX_PAR];
@@ -240,7 +258,12 @@ static void eval_voxel(const gc_model& md, int loss_kind, int transform, const S
     D sm = s[m];
     if (!finite(sm)) sm = D(S(0));  // utils.set_nan_inf_zero, utils/utils.m:347-350
     const D r = sm - S(y[m]);
-    const D l = loss_elem<D, S>(loss_kind, r, w ? S(w[m]) : S(1));
+    D l = loss_elem<D, S>(loss_kind, r, w ? S(w[m]) : S(1));
+    if (codes && loss_kind == GC_LOSS_L1) {
+      const S sg = codes[m] == 1 ? S(1) : (codes[m] == 2 ? S(-1) : S(0));
+      const S ws = w ? S(w[m]) : S(1);
+      for (int p = 0; p < NP; ++p) l.d[p] = r.d[p] * (sg * ws);
+    }
     lsum += double(l.v);
     if (elem) elem[m] = l.v;
     for (int p = 0; p < P && p < NP; ++p) grad[p] += l.d[p];
@@ -344,9 +367,14 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
                        const float* w, const float* extra, const int64_t* vox, const int32_t* dims,
                        const float* prior_grid, const int32_t* surf_nbr, const float* surf_len, int surf_max_nb,
                        const float* z0, double* zfinal, double* zmin, double* res_final,
-                       double* res_min, double* trace, AskOut& out, int nthreads) {
+                       double* res_min, double* trace, AskOut& out, int nthreads,
+                       const uint32_t* sign_trace = nullptr, int64_t n_mean = 0) {
   const int P = md.n_par, M = md.n_meas, E = md.n_extra;
-  const S inv_count = S(1.0 / (double(M) * double(nv)));
+  // N of the mean (:96): the caller's count when this is a piece of a larger fit
+  const double nmean = n_mean > 0 ? double(n_mean) : double(nv);
+  const S inv_count = S(1.0 / (double(M) * nmean));
+  const int nw = (M + 15) / 16;
+  int sign_row = 0;  // row of sign_trace the next evaluation follows (epoch - 1)
   std::vector<S> z(size_t(P) * nv), g(size_t(P) * nv), m1(size_t(P) * nv, S(0)), m2(size_t(P) * nv, S(0)),
       zprev, zmn;
   for (size_t i = 0; i < z.size(); ++i) z[i] = S(z0[i]);
@@ -401,12 +429,18 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
         wv[m] = w ? w[size_t(m) * nv + v] : 1.0f;
       }
       double ls;
-      eval_voxel<S, NP>(md, cfg.loss, cfg.transform, zv, yv, w ? wv : nullptr, exv, ls, gv, el);
+      uint8_t codes[1024];
+      if (sign_trace) {
+        const uint32_t* row = sign_trace + size_t(sign_row) * nw * nv;
+        for (int m = 0; m < M; ++m) codes[m] = uint8_t((row[size_t(m >> 4) * nv + v] >> ((m & 15) * 2)) & 3u);
+      }
+      eval_voxel<S, NP>(md, cfg.loss, cfg.transform, zv, yv, w ? wv : nullptr, exv, ls, gv, el,
+                        sign_trace ? codes : nullptr);
       lsum_total += ls;
       for (int p = 0; p < P; ++p) g[size_t(p) * nv + v] = gv[p] * inv_count;
       for (int m = 0; m < M; ++m) elem_cur[size_t(m) * nv + v] = el[m];
     }
-    fid = lsum_total / (double(M) * double(nv));  // mean(residuals), :96
+    fid = lsum_total / (double(M) * nmean);  // mean(residuals), :96
     reg = 0;
     if (use_tv) {
       for (int k = 0; k < cfg.n_reg; ++k) {
@@ -415,7 +449,7 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
         std::fill(gimg.begin(), gimg.end(), S(0));
         for (int64_t v = 0; v < nv; ++v)
           img[vox[v]] = unscale<S, S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
-        reg += tv_loss_grad<S>(img.data(), mask.data(), dims, cfg.tv_mode, cfg.voxel_size, cfg.lambda[k], nv,
+        reg += tv_loss_grad<S>(img.data(), mask.data(), dims, cfg.tv_mode, cfg.voxel_size, cfg.lambda[k], int64_t(nmean),
                                gimg.data());
         for (int64_t v = 0; v < nv; ++v)
           g[size_t(p) * nv + v] += gimg[vox[v]] * dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v],
@@ -430,7 +464,7 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
         for (int64_t v = 0; v < nv; ++v)
           img[vox[v]] = unscale<S, S>(cfg.transform, z[size_t(p) * nv + v], S(md.lb[p]), S(md.ub[p]));
         reg += surface_tv_loss_grad<S>(img.data(), mask.data(), nvox, surf_nbr, surf_len, surf_max_nb,
-                                       cfg.lambda[k], nv, gimg.data());
+                                       cfg.lambda[k], int64_t(nmean), gimg.data());
         for (int64_t v = 0; v < nv; ++v)
           g[size_t(p) * nv + v] += gimg[vox[v]] * dtheta_dz<S>(cfg.transform, z[size_t(p) * nv + v],
                                                              S(md.lb[p]), S(md.ub[p]));
@@ -454,8 +488,8 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
           const S zs = (img[i] - S(mu[i])) / S(sd[i]);
           if (std::isfinite(double(zs))) sum += std::fabs(double(zs));
         }
-        reg += sum / double(nv) * cfg.lambda[k];
-        const S scale = S(cfg.lambda[k] / double(nv));
+        reg += sum / nmean * cfg.lambda[k];
+        const S scale = S(cfg.lambda[k] / nmean);
         for (int64_t v = 0; v < nv; ++v) {
           const int64_t i = vox[v];
           const S zs = (img[i] - S(mu[i])) / S(sd[i]);
@@ -509,6 +543,7 @@ static int askadam_run(const gc_model& md, const gc_askadam_cfg& cfg, int64_t nv
   int stop = GC_STOP_NONE;
 
   for (epoch = 1; epoch <= cfg.iteration; ++epoch) {
+    sign_row = epoch - 1;
     if (epoch > 1) evaluate(loss, fid, reg);  // epoch 1 evaluates the unchanged start
     if (trace) trace[epoch] = loss;
     // 3.2 update_min_loss (:433-446): strict improvement, pre-update params
@@ -828,7 +863,7 @@ static void gw_run(const McmcArgs& a, int nthreads) {
             const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v);
             // native: one draw gives the stretch (o[0]) and acceptance (o[1]) uniforms
             if (c.rng_mode == GC_RNG_REPLAY) u1 = a.uniforms[(ubase + j) * nv + v];
-            else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o); u1 = u01(o[0]); }
+            else { double ua; native_gw_uniforms(c.seed, uint32_t(k), chain, sub, &u1, &ua); }
             // zz = ((a-1)*rand + 1).^2 / a (:299, :332)
             const S t = (aa - S(1)) * S(u1) + S(1);
             zz[wk] = t * t / aa;
@@ -849,7 +884,7 @@ static void gw_run(const McmcArgs& a, int nthreads) {
             const uint64_t chain = (uint64_t(r) * W + wk) * uint64_t(nv) + uint64_t(v);
             double u2;
             if (c.rng_mode == GC_RNG_REPLAY) u2 = a.uniforms[(ubase + ustride + j) * nv + v];
-            else { uint32_t o[4]; draw(c.seed, uint32_t(k), chain, PURPOSE_ZZ + sub, o); u2 = u01(o[1]); }
+            else { double uz; native_gw_uniforms(c.seed, uint32_t(k), chain, sub, &uz, &u2); }
             // min(zz.^(Nvar-1).*exp(logPProposed-logPCurr),1) > rand (:313-315)
             const S ratio = min1<S>(S(std::pow(zz[wk], S(Q - 1)) * std::exp(lpp[wk] - lp[wk])));
             const bool acc = ratio > S(u2) && !oob[wk];
@@ -1285,6 +1320,12 @@ int orc_dict_search(const float* y, int64_t nv, int32_t m, const float* dict, in
 
 double orc_norm_inv(double u) { return orc::norm_inv(u); }
 
+// Goodman-Weare native draws (stretch uniform, acceptance uniform)
+int orc_native_gw_uniforms(uint64_t seed, uint32_t it, uint64_t chain, int32_t sub, double* u_zz, double* u_acc) {
+  orc::native_gw_uniforms(seed, it, chain, sub, u_zz, u_acc);
+  return 0;
+}
+
 // MH native draws: n[Q] normals and the acceptance uniform
 int orc_native_mh_draws(uint64_t seed, uint32_t it, uint64_t chain, int32_t Q, double* n, double* u_acc) {
   if (Q < 1 || Q > GC_MAX_PAR) return GC_ERR_ARG;
@@ -1361,12 +1402,15 @@ int orc_surface_tv(const double* img, const uint8_t* mask, int64_t ngrid, const 
 }
 
 // askadam; precision 0 = double (oracle), 1 = float (CPU-path baseline).
+// sign_trace (optional, L1): [iteration][ceil(M/16)][nv] residual sign codes
+// (gc_volume.sign_trace_dev layout) that replace the oracle's own L1 signs;
+// n_mean: N of the loss mean (0 = nv), as gc_volume.n_mean.
 // scal_out[10] = {loss, fid, reg, min_loss, min_fid, min_reg, conv, niter, min_iter, stop}
 int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const float* y, const float* w,
                 const float* extra, const int64_t* vox, const int32_t* dims, const float* prior_grid,
                 const int32_t* surf_nbr, const float* surf_len, int32_t surf_max_nb, const float* z0, double* zfinal,
                 double* zmin, double* res_final, double* res_min, double* trace, double* scal_out, int32_t precision,
-                int32_t nthreads) {
+                int32_t nthreads, const uint32_t* sign_trace, int64_t n_mean) {
   if (md->n_meas > 1024 || md->n_par > GC_MAX_PAR || md->n_extra > 64) return GC_ERR_ARG;
   orc::AskOut o{};
   int rc;
@@ -1375,14 +1419,14 @@ int orc_askadam(const gc_model* md, const gc_askadam_cfg* cfg, int64_t nv, const
       constexpr int NP = decltype(npc)::value;
       return orc::askadam_run<double, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, surf_nbr, surf_len,
                                           surf_max_nb, z0, zfinal, zmin, res_final,
-                                          res_min, trace, o, std::max(1, nthreads));
+                                          res_min, trace, o, std::max(1, nthreads), sign_trace, n_mean);
     });
   } else {
     rc = orc::with_np(md->n_par, [&](auto npc) {
       constexpr int NP = decltype(npc)::value;
       return orc::askadam_run<float, NP>(*md, *cfg, nv, y, w, extra, vox, dims, prior_grid, surf_nbr, surf_len,
                                          surf_max_nb, z0, zfinal, zmin, res_final,
-                                         res_min, trace, o, std::max(1, nthreads));
+                                         res_min, trace, o, std::max(1, nthreads), sign_trace, n_mean);
     });
   }
   scal_out[0] = o.loss; scal_out[1] = o.fid; scal_out[2] = o.reg;

@@ -56,7 +56,8 @@ def lib():
                              POINTER(c_double), c_void_p]
         L.orc_askadam.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                   POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
-                                  c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_double), c_int32, c_int32]
+                                  c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_double), c_int32, c_int32,
+                                  c_void_p, c_int64]
         L.orc_epgx_sample.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_double,
                                       POINTER(c_double)]
         L.orc_surface_tv.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int32, c_double, c_int64,
@@ -68,6 +69,8 @@ def lib():
         L.orc_norm_inv.argtypes = [c_double]
         L.orc_norm_inv.restype = c_double
         L.orc_native_mh_draws.argtypes = [c_uint64, c_uint32, c_uint64, c_int32, POINTER(c_double), POINTER(c_double)]
+        L.orc_native_gw_uniforms.argtypes = [c_uint64, c_uint32, c_uint64, c_int32, POINTER(c_double),
+                                             POINTER(c_double)]
         L.orc_dict_search.argtypes = [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                       c_int32]
         L.orc_mcmc_global.argtypes = [c_void_p, c_void_p, c_int64] + [c_void_p] * 20 + [c_int32, c_int32]
@@ -135,6 +138,14 @@ def native_mh_draws(seed, it, chain, Q):
     return np.array(n), u.value
 
 
+def native_gw_uniforms(seed, it, chain, sub=0):
+    """The Goodman-Weare native stream at (iteration, walker chain, half):
+    (stretch uniform, acceptance uniform)."""
+    uz, ua = c_double(), c_double()
+    assert lib().orc_native_gw_uniforms(seed, it, chain, sub, ctypes.byref(uz), ctypes.byref(ua)) == 0
+    return uz.value, ua.value
+
+
 def native_normal(seed, it, chain, base, q):
     return lib().orc_native_normal(seed, it, chain, base, q)
 
@@ -200,11 +211,15 @@ def surface_tv(img, mask, nbr, elen, lam, nmasked, grad=True):
 
 
 def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0, nthreads=1, trace=False,
-            prior_grid=None, surf_nbr=None, surf_len=None):
+            prior_grid=None, surf_nbr=None, surf_len=None, sign_trace=None, n_mean=0):
     """prior_grid: GC_REG_PRIOR maps on the whole grid (column-major),
     [2*n_reg, ngrid]: the mu rows, then the sigma rows; with vox and dims.
     surf_nbr / surf_len: GC_REG_SURFACE_TV neighbours (global 0-based vertex
-    indices) and edge lengths, [max_nb, ngrid], with vox and dims."""
+    indices) and edge lengths, [max_nb, ngrid], with vox and dims.
+    sign_trace: uint32 [iteration, ceil(M/16), nv] residual sign codes (two
+    bits per measurement: 1 = +, 2 = -, 0 = no gradient) that the L1
+    derivative follows instead of the oracle's own signs.
+    n_mean: N of the loss mean (0 = nv)."""
     y = _f32(y)
     M, nv = y.shape
     z0 = _f32(z0)
@@ -222,9 +237,14 @@ def askadam(md, cfg, y, z0, w=None, extra=None, vox=None, dims=None, precision=0
     pg = None if prior_grid is None else np.ascontiguousarray(prior_grid, np.float32)
     sn = None if surf_nbr is None else np.ascontiguousarray(surf_nbr, np.int32)
     sl = None if surf_len is None else np.ascontiguousarray(surf_len, np.float32)
+    st = None
+    if sign_trace is not None:
+        st = np.ascontiguousarray(sign_trace, np.uint32)
+        assert st.shape == (int(cfg.iteration), (M + 15) // 16, nv), st.shape
     rc = lib().orc_askadam(ctypes.byref(md), ctypes.byref(cfg), nv, _p(y), _p(w), _p(extra), _p(vox), d, _p(pg),
                            _p(sn), _p(sl), 0 if sn is None else sn.shape[0], _p(z0),
-                           _p(zf), _p(zm), _p(rf), _p(rm), _p(tr), sc, precision, nthreads)
+                           _p(zf), _p(zm), _p(rf), _p(rm), _p(tr), sc, precision, nthreads,
+                           _p(st), int(n_mean))
     assert rc == 0, rc
     out = dict(z_final=zf, z_min=zm, residual=rf, residual_min=rm, loss=sc[0], loss_fidelity=sc[1],
                loss_reg=sc[2], min_loss=sc[3], min_loss_fidelity=sc[4], min_loss_reg=sc[5], convergence=sc[6],

@@ -59,10 +59,11 @@ def dict_search(y, d, w=None, with_sse=True):
     return runtime.to_host(best)[:nv], (runtime.to_host(sse)[:nv] if with_sse else None)
 
 
-def askadam(md, cfg, vols, trace=False):
+def askadam(md, cfg, vols, trace=False, signs=False):
     """vols: list of dict(y, z0, w=None, extra=None, vox=None, dims=None,
-    prior=None, reg_offset=0, surf_nbr=None, surf_len=None).  Returns
-    per-volume dicts."""
+    prior=None, reg_offset=0, surf_nbr=None, surf_len=None, n_mean=0).
+    Returns per-volume dicts; signs=True adds each volume's residual sign
+    trace ("signs", uint32 [iteration, ceil(M/16), nv])."""
     c = ctx()
     n = len(vols)
     arr = (_abi.gc_volume * n)()
@@ -82,7 +83,10 @@ def askadam(md, cfg, vols, trace=False):
         rs = c.empty((M, nv))
         rsm = c.empty((M, nv))
         tr = (ctypes.c_double * (int(cfg.iteration) + 1))() if trace else None
+        sg = c.zeros((int(cfg.iteration), (M + 15) // 16, nv), np.int32) if signs else None
         a = arr[i]
+        a.n_mean = int(v.get("n_mean", 0))
+        a.sign_trace_dev = runtime.ptr(sg)
         a.nv = nv
         a.y_dev = runtime.ptr(yd)
         a.w_dev = runtime.ptr(wd)
@@ -111,11 +115,11 @@ def askadam(md, cfg, vols, trace=False):
             a.surf_nbr_dev, a.surf_len_dev = runtime.ptr(sn), runtime.ptr(sl)
             a.surf_max_nb = int(np.shape(v["surf_nbr"])[0])
             keep += [sn, sl]
-        keep += [yd, wd, ex, zd, zm, rl, rlm, rs, rsm]
-        metas.append((zd, zm, rl, rlm, rs, rsm, tr))
+        keep += [yd, wd, ex, zd, zm, rl, rlm, rs, rsm, sg]
+        metas.append((zd, zm, rl, rlm, rs, rsm, tr, sg))
     c.check(c.lib.gc_askadam_run(c.handle, ctypes.byref(md), ctypes.byref(cfg), arr, n))
     outs = []
-    for i, (zd, zm, rl, rlm, rs, rsm, tr) in enumerate(metas):
+    for i, (zd, zm, rl, rlm, rs, rsm, tr, sg) in enumerate(metas):
         a = arr[i]
         o = dict(z_final=runtime.to_host(zd), z_min=runtime.to_host(zm), resloss=runtime.to_host(rl),
                  resloss_min=runtime.to_host(rlm), residual=runtime.to_host(rs), residual_min=runtime.to_host(rsm),
@@ -124,6 +128,8 @@ def askadam(md, cfg, vols, trace=False):
                  convergence=a.convergence, n_launch=a.n_launch)
         if tr is not None:
             o["trace"] = np.array(tr[: a.n_iteration + 1])
+        if sg is not None:
+            o["signs"] = runtime.to_host(sg).view(np.uint32)
         outs.append(o)
     return outs
 

@@ -4,7 +4,7 @@ plausible restatement error (a dropped term, a wrong sign, a transposed
 operand) fails one of these."""
 import numpy as np
 import pytest
-from scipy import integrate, linalg, special
+from scipy import integrate, linalg, special, stats
 
 from gacelle_amd import _abi, models
 from gacelle_amd.askadam import askadam
@@ -365,13 +365,139 @@ def test_askadam_stopping_rules_on_known_sequences():
     assert (out3["n_iteration"], out3["stop_reason"]) == (6, _abi.GC_STOP_STEP)
 
 
-def test_askadam_linear_convergence_signal_is_ols_slope():
-    """update_convergence (utils/askadam.m:1185-1190) is minus the OLS slope:
-    a linear loss decrease of 1e-3 per iteration gives exactly 1e-3."""
-    x = np.arange(1, 21)
-    buf = 5.0 - 1e-3 * x
-    slope = np.polyfit(x, buf, 1)[0]
-    assert -slope == pytest.approx(1e-3)
+def _l1_sgdm(iteration, lr=0.1, **kw):
+    f = askadam.check_set_default_basic(dict(modelParams=["M0", "R2star"], lb=[0, 0.1], ub=[2, 200],
+                                             iteration=iteration, lossFunction="L1", optimiser="sgdm",
+                                             initialLearnRate=lr, tol=0, convergenceValue=-1,
+                                             convergenceStepTol=0, convergenceGradTol=0, **kw))
+    return askadam().make_cfg(f, None)
+
+
+def _pack_codes(codes):
+    """[iteration, M, nv] codes (0/1/2) -> [iteration, ceil(M/16), nv] words."""
+    it, M, nv = codes.shape
+    w = np.zeros((it, (M + 15) // 16, nv), np.uint32)
+    for m in range(M):
+        w[:, m // 16, :] |= codes[:, m, :].astype(np.uint32) << np.uint32(2 * (m % 16))
+    return w
+
+
+def test_askadam_matched_sign_mode():
+    """The L1 derivative follows the supplied residual signs (row e-1 for
+    epoch e, the start evaluation with row 0): the oracle's own signs give
+    the unmatched run exactly; all-zero codes give no gradient; flipped
+    signs reverse the first SGDM step, z1 = z0 + lr*g instead of z0 - lr*g,
+    with g = sum_m w*sign(r_m)*ds_m/dz / (M*N) from the closed form."""
+    nv = 37
+    md = _r2s_md()
+    rng = np.random.default_rng(4)
+    th = np.stack([rng.uniform(0.5, 1.5, nv), rng.uniform(10, 60, nv)])
+    y = (orc.fwd_value(md, th) + 0.01 * rng.standard_normal((12, nv))).astype(np.float32)
+    z0 = np.stack([np.full(nv, 0.3), np.full(nv, -1.2)]).astype(np.float32)
+    th0 = np.stack([md.lb[p] + (md.ub[p] - md.lb[p]) / (1 + np.exp(-np.float64(z0[p]))) for p in range(2)])
+    te = np.array([md.table[t] for t in range(12)], np.float64)
+    s0 = th0[0] * np.exp(-np.outer(te, th0[1]))
+    sg = np.sign(s0 - y.astype(np.float64))
+    own = np.where(sg > 0, 1, np.where(sg < 0, 2, 0))
+    cfg = _l1_sgdm(2)
+    plain = orc.askadam(md, cfg, y, z0)
+    codes = np.stack([own, np.ones_like(own)])          # row 1 (the last epoch) has no update
+    matched = orc.askadam(md, cfg, y, z0, sign_trace=_pack_codes(codes))
+    np.testing.assert_array_equal(matched["z_final"], plain["z_final"])
+    # the loss stays the oracle's own w|r|, whatever the signs
+    flipped = orc.askadam(md, cfg, y, z0, sign_trace=_pack_codes(np.stack([3 - own, own])), trace=True)
+    ptr = orc.askadam(md, cfg, y, z0, trace=True)["trace"]
+    assert flipped["trace"][0] == ptr[0]
+    np.testing.assert_allclose(ptr[0], np.mean(np.abs(s0 - y.astype(np.float64))), rtol=1e-12)
+    # closed-form first step (sgdm: m1 = -lr*g, z1 = z0 + m1)
+    sig = [1 / (1 + np.exp(-np.float64(z0[p]))) for p in range(2)]
+    dth = [(md.ub[p] - md.lb[p]) * sig[p] * (1 - sig[p]) for p in range(2)]
+    e = np.exp(-np.outer(te, th0[1]))
+    J = [e * dth[0], -te[:, None] * th0[0] * e * dth[1]]
+    g = np.stack([np.sum(sg * J[p], axis=0) / (12 * nv) for p in range(2)])
+    np.testing.assert_allclose(plain["z_final"], z0 - 0.1 * g, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(flipped["z_final"], z0 + 0.1 * g, rtol=1e-6, atol=1e-9)
+    zero = orc.askadam(md, _l1_sgdm(6), y, z0, sign_trace=np.zeros((6, 1, nv), np.uint32))
+    np.testing.assert_array_equal(zero["z_final"], z0.astype(np.float64))
+    # a sign on measurement 5 only: the step is that measurement's term
+    one = np.zeros((2, 12, nv), np.int64)
+    one[0, 5] = 1
+    o5 = orc.askadam(md, cfg, y, z0, sign_trace=_pack_codes(one))
+    np.testing.assert_allclose(o5["z_final"], z0 - 0.1 * np.stack([J[p][5] for p in range(2)]) / (12 * nv),
+                               rtol=1e-6, atol=1e-12)
+
+
+def test_askadam_n_mean_scales_the_mean():
+    """n_mean = N replaces nv in the mean over residuals (utils/askadam.m:96):
+    twice the count halves the loss and the first SGDM step."""
+    nv = 25
+    md = _r2s_md()
+    rng = np.random.default_rng(8)
+    th = np.stack([rng.uniform(0.5, 1.5, nv), rng.uniform(10, 60, nv)])
+    y = (orc.fwd_value(md, th) + 0.01 * rng.standard_normal((12, nv))).astype(np.float32)
+    z0 = np.zeros((2, nv), np.float32)
+    a = orc.askadam(md, _l1_sgdm(2), y, z0, trace=True)
+    b = orc.askadam(md, _l1_sgdm(2), y, z0, trace=True, n_mean=2 * nv)
+    # the start and the first epoch (same parameters) halve; later losses
+    # follow the smaller step
+    np.testing.assert_allclose(b["trace"][:2], a["trace"][:2] / 2, rtol=1e-14)
+    np.testing.assert_allclose(b["z_final"] - z0, (a["z_final"] - z0) / 2, rtol=1e-12, atol=1e-15)
+
+
+def test_askadam_linear_convergence_stop():
+    """convergenceModel 'linear' (update_convergence, utils/askadam.m:1185-1190,
+    and the patience rules of :573-597) driven through the oracle's loop on a
+    constructed loss sequence.  TE ~ 0 makes s = M0; with the linear
+    transform, L1 loss, data below lb and SGDM without momentum every step
+    lowers z by the same amount, so the loss falls linearly (slope 0.015 per
+    iteration, checked in closed form) until the clamp at z = 0 flattens it.
+    The stop must land where minus the least-squares slope over the
+    20-iteration buffer (primed with ones) stays <= convergenceValue for more
+    than patienceConvergence iterations after the window has filled."""
+    m = models.gpuR2starMapping(np.full(12, 1e-9))
+    nv = 8
+    rng = np.random.default_rng(2)
+    lb, ub = [0.5, 0.1], [2.0, 200.0]
+    c = 1e-3
+    f = askadam.check_set_default_basic(dict(
+        modelParams=["M0", "R2star"], lb=lb, ub=ub, iteration=400, lossFunction="L1", optimiser="sgdm",
+        sgdmupdateMomentum=0.0, initialLearnRate=nv * 0.01 / 1.5, parameterTransform="linear", tol=0,
+        convergenceModel="linear", convergenceWindow=20, convergenceValue=c, patienceConvergence=5,
+        convergenceStepTol=0, convergenceGradTol=0))
+    md = m.gc_model(f, "askadam", 12)
+    cfg = askadam().make_cfg(f, None)
+    assert cfg.convergence_model == _abi.GC_CONV_LINEAR and cfg.transform == _abi.GC_TRANSFORM_LINEAR
+    y = np.repeat(rng.uniform(0.1, 0.4, (1, nv)), 12, axis=0).astype(np.float32)
+    z0 = np.stack([np.full(nv, 0.905), np.full(nv, 0.2)]).astype(np.float32)
+    out = orc.askadam(md, cfg, y, z0, trace=True)
+    tr = out["trace"]
+    assert out["n_iteration"] > 100
+    # the constructed sequence: linear with slope -0.015 up to the clamp, then flat
+    e = np.arange(1, 80)
+    np.testing.assert_allclose(np.diff(tr[e]), -0.015, rtol=1e-6)
+    # (R2* still moves, by TE ~ 1e-9 times less)
+    assert np.ptp(tr[-7:]) < 1e-12
+    # the stop rule restated on that sequence
+    buf, pc, stop_at, conv_at = np.ones(20), 0, None, None
+    for k in range(1, len(tr)):
+        buf = np.append(buf[1:], tr[k])
+        conv = -np.polyfit(np.arange(1, 21), buf, 1)[0]
+        pc = 0 if (conv > c or k <= 20) else pc + 1
+        if pc > 5:
+            stop_at, conv_at = k, conv
+            break
+    assert stop_at is not None and stop_at > 100
+    assert out["stop_reason"] == _abi.GC_STOP_CONVERGENCE
+    assert out["n_iteration"] == stop_at
+    assert out["convergence"] == pytest.approx(conv_at, rel=1e-9, abs=1e-15)
+    # a loss flat from the start (learning rate 0) below the buffer's priming
+    # ones: the signal is positive until the buffer holds 20 losses, zero at
+    # iteration 20, which the window guard still excludes, so patience runs
+    # from 21 and the stop is at 20 + 5 + 1
+    cfg0 = askadam().make_cfg(dict(f, initialLearnRate=0.0), None)
+    flat = orc.askadam(md, cfg0, y, z0, trace=True)
+    assert (flat["n_iteration"], flat["stop_reason"]) == (26, _abi.GC_STOP_CONVERGENCE)
+    assert abs(flat["convergence"]) < 1e-14
 
 
 # ---------------------------------------------------------------------------
@@ -412,16 +538,78 @@ def test_mh_posterior_of_analytic_target():
     np.testing.assert_allclose(np.median(o["std"][0]), sigma / np.sqrt(12), rtol=0.06)
 
 
-def test_gw_stretch_density():
-    """zz = ((a-1)U+1)^2/a has density proportional to 1/sqrt(z) on [1/a, a]."""
+def test_gw_native_stretch_density():
+    """The oracle's native Goodman-Weare stream: zz = ((a-1)u+1)^2/a from the
+    stretch uniform has density proportional to 1/sqrt(z) on [1/a, a]
+    (utils/mcmc.m:299), and the acceptance uniform of the same draw is U(0,1)
+    and uncorrelated with it."""
     a = 2.0
-    u = np.random.default_rng(0).uniform(0, 1, 200000)
-    zz = ((a - 1) * u + 1) ** 2 / a
-    hist, edges = np.histogram(zz, bins=20, range=(1 / a, a), density=True)
-    c = (edges[:-1] + edges[1:]) / 2
-    ref = 1 / np.sqrt(c)
-    ref /= np.sum(ref * np.diff(edges))
-    np.testing.assert_allclose(hist, ref, rtol=0.05)
+    n = 60000
+    uz = np.empty(n)
+    ua = np.empty(n)
+    for i in range(n):
+        uz[i], ua[i] = orc.native_gw_uniforms(48463, 1 + i % 997, i // 997, i % 2)
+    zz = ((a - 1) * uz + 1) ** 2 / a
+    # CDF of p(z) ~ 1/sqrt(z) on [1/a, a]: (sqrt(z) - sqrt(1/a)) / (sqrt(a) - sqrt(1/a))
+    cdf = lambda z: (np.sqrt(z) - np.sqrt(1 / a)) / (np.sqrt(a) - np.sqrt(1 / a))
+    assert stats.kstest(zz, cdf).pvalue > 1e-3
+    assert stats.kstest(ua, "uniform").pvalue > 1e-3
+    assert abs(np.corrcoef(uz, ua)[0, 1]) < 0.02
+
+
+def _grid_posterior(y, te, lb, ub, sig):
+    """Posterior mean and SD of (M0, R2*, noise) under logP_Gaussian with a
+    uniform prior on the box, by integration on a grid around the
+    least-squares fit (the noise axis spans its whole box)."""
+    from scipy.optimize import least_squares
+    p = least_squares(lambda q: q[0] * np.exp(-te * q[1]) - y, [1.0, 30.0]).x
+    J = np.stack([np.exp(-te * p[1]), -te * p[0] * np.exp(-te * p[1])], 1)
+    sd = np.sqrt(np.diag(np.linalg.inv(J.T @ J))) * sig * 3
+    axes = [np.linspace(p[0] - 8 * sd[0], p[0] + 8 * sd[0], 161), np.linspace(p[1] - 8 * sd[1], p[1] + 8 * sd[1], 161),
+            np.linspace(lb[2], ub[2], 400)]
+    S = axes[0][:, None, None] * np.exp(-te[None, None, :] * axes[1][None, :, None])
+    SS = np.sum((S - y) ** 2, axis=2)
+    lp = -SS[:, :, None] / (2 * axes[2] ** 2) - te.size * np.log(axes[2])
+    P = np.exp(lp - lp.max())
+    P /= P.sum()
+    out = []
+    for ax, g in enumerate(axes):
+        marg = P.sum(axis=tuple(i for i in range(3) if i != ax))
+        mu = np.sum(marg * g)
+        out.append((mu, np.sqrt(np.sum(marg * (g - mu) ** 2))))
+    return out
+
+
+@pytest.mark.parametrize("redblack", [False, True])
+def test_gw_posterior_of_analytic_target(redblack):
+    """The oracle's plain Goodman-Weare (utils/mcmc.m:192-383) samples the
+    posterior of (M0, R2*, noise) of mono-exponential data, all three
+    variables live, whose moments come from grid integration: posterior means
+    within 0.05 posterior SD and SDs within 3%.  Planted in a copy of the
+    oracle (on the CPU), z^0 instead of z^(Q-1) gives SDs x0.65, z^Q x1.22,
+    and a stretch about the walker instead of its partner x0.46; a walker
+    drawn as its own partner only slows mixing and leaves this target
+    unchanged (the derangement is pinned by test_native_partners_are_derangements)."""
+    from gacelle_amd.mcmc import mcmc as M
+    lb, ub = [0, 0.1, 0.003], [2, 200, 0.05]
+    md = models.gpuR2starMapping(TE12).gc_model(
+        M.check_set_default_basic(dict(modelParams=["M0", "R2star", "noise"], lb=lb, ub=ub)), "mcmc", 12)
+    rng = np.random.default_rng(11)
+    nd, rep, sig = 6, 40, 0.01
+    th = np.stack([rng.uniform(0.7, 1.3, nd), rng.uniform(15, 50, nd)])
+    y0 = (th[0] * np.exp(-np.outer(TE12, th[1])) + sig * rng.standard_normal((12, nd))).astype(np.float32)
+    te = np.array([md.table[t] for t in range(12)], np.float64)
+    ref = [_grid_posterior(y0[:, i].astype(np.float64), te, lb, ub, sig) for i in range(nd)]
+    x0 = np.stack([np.repeat(th[0], rep), np.repeat(th[1], rep), np.full(nd * rep, sig)]).astype(np.float32)
+    cfg = mcmc_cfg(_abi.GC_MCMC_GW, 4000, 5, 1000, W=16, redblack=redblack, Q=3)
+    o = orc.mcmc(md, cfg, np.repeat(y0, rep, axis=1), x0, store=False, nthreads=4)
+    for q in range(3):
+        mu = o["mean"][q].reshape(nd, rep).mean(axis=1)
+        sd = np.sqrt((o["std"][q].reshape(nd, rep) ** 2).mean(axis=1))
+        rm = np.array([r[q][0] for r in ref])
+        rs = np.array([r[q][1] for r in ref])
+        assert np.all(np.abs(mu - rm) < 0.05 * rs), (q, (mu - rm) / rs)
+        np.testing.assert_allclose(sd / rs, 1.0, atol=0.03)
 
 
 # ---------------------------------------------------------------------------

@@ -79,11 +79,14 @@ def test_headline_l1_matched_sign_parity():
     # FP32 device arithmetic vs FP64 along the same subgradient path
     assert max(err_final) <= 1e-4, err_final
     assert max(err_min) <= 1e-4, err_min
-    np.testing.assert_allclose(got["trace"], ref["trace"], rtol=1e-5)
+    # loss trace: each FP32 residual s - y (|r| ~ 0.01 at s ~ 0.5 near the
+    # noise floor) carries ~3e-6 relative cancellation error, and the block
+    # partials are FP32 sums of 256 voxels
+    np.testing.assert_allclose(got["trace"], ref["trace"], rtol=5e-5)
     # the minimum is the argmin of the device's own loss trace, and the
     # oracle's trace has its minimum within the FP32 spread of it
     assert got["trace"][k] == got["trace"].min()
-    assert ref["trace"][k] <= ref["trace"].min() * (1 + 1e-5)
+    assert ref["trace"][k] <= ref["trace"].min() * (1 + 5e-5)
 
 
 def test_headline_l1_unmatched_spread():

@@ -17,6 +17,8 @@ GC_MAX_VOL = 16
 GC_MAX_REG = 4
 GC_MLP_MAX_LAYERS = 6
 GC_MLP_MAX_WIDTH = 64
+GC_NOALLOC_MAX_NV = 1 << 25
+GC_NOALLOC_SCRATCH_BYTES = 4096 + 8 * (GC_NOALLOC_MAX_NV // 256)
 
 # status codes
 GC_OK = 0
@@ -244,6 +246,8 @@ SIGNATURES = {
     "gc_fwd": ([c_void_p, POINTER(gc_model), c_void_p, c_void_p, c_void_p, c_int64], c_int32),
     "gc_fwd_adj": ([c_void_p, POINTER(gc_model), c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                     c_int64, c_float, c_void_p, c_void_p, c_void_p], c_int32),
+    "gc_fwd_adj_noalloc": ([c_void_p, POINTER(gc_model), c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_int64, c_float, c_void_p, c_void_p, c_void_p], c_int32),
     "gc_tv_fwd_adj": ([c_void_p, c_void_p, c_void_p, POINTER(c_int32), c_int32, POINTER(c_double), c_double,
                        c_int64, c_void_p, c_void_p], c_int32),
     "gc_posterior_stats": ([c_void_p, c_void_p, c_int32, c_int32, c_int64, POINTER(c_float), POINTER(c_float), c_int32,

@@ -3,6 +3,7 @@
 // MCMC set-up.  The askadam driver is in gc_askadam.hpp; model-dependent
 // launches are compiled per model variant in ops_*.hip.
 #include <atomic>
+#include <mutex>
 
 #include "gc_dict.hpp"
 #include "gc_host.hpp"
@@ -62,6 +63,51 @@ int select_ops(gc_ctx* ctx, const gc_model& md, const ModelOps** out) {
 
 }  // namespace
 
+// ---- scratch of gc_fwd_adj_noalloc -----------------------------------------
+// What gc_fwd_adj allocates per call, in one fixed array per device: the
+// model description's device copy (first kNoAllocModelBytes) and the FP64
+// loss partial of every 256-voxel block.  In host code `scratch` names a
+// host-side stand-in, not device memory: its device address on each device
+// is looked up once (scratch_setup, with that device current) and only that
+// address is used.
+constexpr size_t kNoAllocModelBytes = 4096;
+static_assert(sizeof(gc_model) <= kNoAllocModelBytes, "gc_model outgrew the scratch model slot");
+static_assert(GC_NOALLOC_MAX_NV % kBlock == 0, "whole blocks");
+constexpr size_t kNoAllocScratchBytes = kNoAllocModelBytes + sizeof(double) * size_t(GC_NOALLOC_MAX_NV / kBlock);
+static_assert(kNoAllocScratchBytes == size_t(GC_NOALLOC_SCRATCH_BYTES), "header and library disagree");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+hipError_t g_scratch_err[kMaxDevices] = {};       // the lookup's error, per device
+
+// Look up the device address of `scratch` on `device` (the current device)
+// once; later calls return the recorded address or error.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device] || g_scratch_err[device] != hipSuccess) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) g_scratch_err[device] = (e != hipSuccess) ? e : hipErrorInvalidSymbol;
+  else g_scratch_dev[device] = static_cast<unsigned char*>(p);
+}
+
+int scratch_of(gc_ctx* ctx, unsigned char** out) {
+  *out = nullptr;
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_err(ctx, GC_ERR_UNSUPPORTED, "gc_fwd_adj_noalloc: device %d beyond %d", ctx->device, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[ctx->device])
+    return set_err(ctx, GC_ERR_HIP, "gc_fwd_adj_noalloc: no scratch address on device %d (%s)", ctx->device,
+                   hipGetErrorString(g_scratch_err[ctx->device]));
+  *out = g_scratch_dev[ctx->device];
+  return GC_OK;
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 extern "C" {
@@ -82,6 +128,7 @@ int gc_ctx_create(int32_t device, gc_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) { delete c; return GC_ERR_HIP; }
   c->stream = nullptr;  // the device's default stream until gc_ctx_set_stream
   c->own_stream = false;
+  scratch_setup(device);  // once per device, with the device current
   g_live_contexts.fetch_add(1);
   *out = c;
   return GC_OK;
@@ -236,6 +283,40 @@ int gc_fwd_adj(gc_ctx* ctx, const gc_model* md, int32_t loss, int32_t transform,
     GC_HIP(ctx, hipGetLastError());
   }
   GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // scratch freed on return
+  return GC_OK;
+}
+
+int gc_fwd_adj_noalloc(gc_ctx* ctx, const gc_model* md, int32_t loss, int32_t transform, const float* x,
+                       const float* y, const float* w, const float* extra, int64_t nv, float scale, float* grad,
+                       double* loss_sum, float* residual) {
+  if (!ctx) return GC_ERR_ARG;
+  if (nv > GC_NOALLOC_MAX_NV)
+    return set_err(ctx, GC_ERR_ARG, "gc_fwd_adj_noalloc: at most %lld voxels per call (got %lld)",
+                   (long long)GC_NOALLOC_MAX_NV, (long long)nv);
+  int rc = check_model(ctx, md);
+  if (rc) return rc;
+  if (!x || !y || nv <= 0 || (md->n_extra > 0 && !extra))
+    return set_err(ctx, GC_ERR_ARG, "gc_fwd_adj_noalloc: bad argument");
+  if (md->model_id == GC_MODEL_MCRMWI && (md->flags & GC_FLAG_MCRMWI_EPG))
+    return set_err(ctx, GC_ERR_UNSUPPORTED, "gc_fwd_adj_noalloc: the EPG-X model needs per-call memory (gc_fwd_adj)");
+  const ModelOps* ops = nullptr;
+  rc = select_ops(ctx, *md, &ops);
+  if (rc) return rc;
+  unsigned char* base = nullptr;
+  rc = scratch_of(ctx, &base);
+  if (rc) return rc;
+  gc_model* dmd = reinterpret_cast<gc_model*>(base);
+  double* part = reinterpret_cast<double*>(base + kNoAllocModelBytes);
+  const unsigned nb = nblocks(nv);
+  // every block writes its own partial, and the reduction reads only those
+  GC_HIP(ctx, hipMemcpyAsync(dmd, md, sizeof(gc_model), hipMemcpyHostToDevice, ctx->stream));
+  rc = ops->fwd_adj(ctx, dmd, loss, transform, x, y, w, extra, nv, scale, grad, loss_sum ? part : nullptr,
+                    residual, nullptr);
+  if (rc) return rc;
+  if (loss_sum) {
+    hipLaunchKernelGGL(k_reduce_rows_d, dim3(1), dim3(kBlock), 0, ctx->stream, part, int64_t(nb), loss_sum);
+    GC_HIP(ctx, hipGetLastError());
+  }
   return GC_OK;
 }
 

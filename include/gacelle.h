@@ -426,6 +426,26 @@ int gc_fwd_adj(gc_ctx* ctx, const gc_model* model, int32_t loss, int32_t transfo
                const float* extra_dev, int64_t nv, float scale, float* grad_dev,
                double* loss_sum_dev, float* residual_dev);
 
+/* ---- the operator pair without allocation --------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, kernels and results as gc_fwd_adj, but
+ * what gc_fwd_adj allocates per call (the device copy of the model
+ * description and one FP64 loss partial per 256-voxel block) lives in a
+ * fixed library array of GC_NOALLOC_SCRATCH_BYTES on each device, held
+ * until the process ends.  Limits: nv <= GC_NOALLOC_MAX_NV (else
+ * GC_ERR_ARG); the EPG-X MCR-MWI model (GC_FLAG_MCRMWI_EPG), whose network
+ * repacking needs per-call memory, is rejected with GC_ERR_UNSUPPORTED.
+ * Every call on a device uses that device's one array: calls ordered on one
+ * stream are safe, calls that overlap on one device (other streams or
+ * threads) race.  Asynchronous on the context's stream (gc_fwd_adj
+ * synchronises). */
+#define GC_NOALLOC_MAX_NV (1 << 25)
+#define GC_NOALLOC_SCRATCH_BYTES (4096 + 8 * (GC_NOALLOC_MAX_NV / 256))
+int gc_fwd_adj_noalloc(gc_ctx* ctx, const gc_model* model, int32_t loss, int32_t transform,
+                       const float* x_dev, const float* y_dev, const float* w_dev, const float* extra_dev,
+                       int64_t nv, float scale, float* grad_dev, double* loss_sum_dev, float* residual_dev);
+
 /* ---- regulariser (utils/spatial_total_variation.m:24-66) ---------------- */
 /* loss = lambda/n_masked * sum_i mask_i * sum_d |D_d img_i| / vsize_d with
  * periodic forward differences; grad_img_dev (optional) receives

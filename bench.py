@@ -114,7 +114,10 @@ def logit_z0(nv, md, device):
     return torch.from_numpy(np.repeat(z, nv, axis=1).astype(np.float32)).to(device).contiguous()
 
 
-def run_askadam(ctx, md, cfg, y, z0):
+def run_askadam(ctx, md, cfg, y, z0, n_mean=0, signs=None):
+    """One whole fit (gc_askadam_run).  n_mean: N of the loss mean (0 = this
+    volume's voxels); signs: a device int32 buffer [iteration, ceil(M/16), nv]
+    that receives the residual sign trace."""
     from gacelle_amd import _abi, runtime
 
     nv = y.shape[1]
@@ -127,14 +130,33 @@ def run_askadam(ctx, md, cfg, y, z0):
     v[0].z_dev = runtime.ptr(z)
     v[0].zmin_dev = runtime.ptr(zmin)
     v[0].resloss_dev = runtime.ptr(rl)
+    v[0].n_mean = int(n_mean)
+    v[0].sign_trace_dev = runtime.ptr(signs)
     ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(cfg), v, 1))
     return z, v[0]
 
 
-def run_mh(ctx, md, y, x0, iterations, reps, seed=48463):
-    from gacelle_amd import _abi, runtime
+def subset_for_oracle(ctx, md, cfg, y, z0, zf, n=10000, seed=7):
+    """A voxel subset of a whole fit, fitted again alone with the whole fit's
+    N (gc_volume.n_mean) and its residual sign trace recorded: with stopping
+    off those voxels follow exactly their trajectories in the whole fit,
+    which is checked bit for bit.  Returns host arrays for the oracle check
+    of the CPU leg."""
+    import torch
 
     nv = y.shape[1]
+    idx = torch.from_numpy(np.sort(np.random.default_rng(seed).choice(nv, min(n, nv), replace=False))).to(y.device)
+    ys, zs = y[:, idx].contiguous(), z0[:, idx].contiguous()
+    signs = torch.zeros((int(cfg.iteration), (md.n_meas + 15) // 16, idx.numel()), dtype=torch.int32, device=y.device)
+    zsub, _ = run_askadam(ctx, md, cfg, ys, zs, n_mean=nv, signs=signs)
+    same = bool(torch.equal(zsub, zf[:, idx]))
+    return dict(y=ys.cpu().numpy(), z0=zs.cpu().numpy(), z=zsub.cpu().numpy(),
+                signs=signs.cpu().numpy().view(np.uint32), n_mean=nv, same_as_whole_fit=same)
+
+
+def mh_cfg(iterations, reps, seed=48463, replay=False):
+    from gacelle_amd import _abi
+
     c = _abi.gc_mcmc_cfg()
     c.algorithm = _abi.GC_MCMC_MH
     c.iteration = iterations
@@ -142,12 +164,22 @@ def run_mh(ctx, md, y, x0, iterations, reps, seed=48463):
     c.n_burnin = int(np.floor(iterations * 0.1))
     c.repetition = reps
     c.n_walker = 1
-    c.rng_mode = _abi.GC_RNG_NATIVE
+    c.rng_mode = _abi.GC_RNG_REPLAY if replay else _abi.GC_RNG_NATIVE
     c.noise_slot = 2
     c.n_var = 3
     c.seed = seed
     for q, s in enumerate((0.01, 1.0, 0.005)):
         c.x_step[q] = s
+    return c
+
+
+def run_mh(ctx, md, y, x0, iterations, reps, seed=48463, normals=None, uniforms=None):
+    """gc_mcmc_run, MH; with normals / uniforms (device tensors in the
+    reference's draw layout) in replay mode."""
+    from gacelle_amd import _abi, runtime
+
+    nv = y.shape[1]
+    c = mh_cfg(iterations, reps, seed, replay=normals is not None)
     io = _abi.gc_mcmc_io()
     mean = ctx.empty((3, nv))
     std = ctx.empty((3, nv))
@@ -156,8 +188,19 @@ def run_mh(ctx, md, y, x0, iterations, reps, seed=48463):
     io.x0_dev = runtime.ptr(x0)
     io.mean_dev = runtime.ptr(mean)
     io.std_dev = runtime.ptr(std)
+    io.normals_dev = runtime.ptr(normals)
+    io.uniforms_dev = runtime.ptr(uniforms)
     ctx.check(ctx.lib.gc_mcmc_run(ctx.handle, ctypes.byref(md), ctypes.byref(c), ctypes.byref(io)))
     return mean, std, io
+
+
+def omp_threads():
+    """Host threads for the CPU legs: the process's CPU share (OMP_NUM_THREADS
+    when set, else every core in the affinity mask), and the host's core count."""
+    ncores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else ncores
+    return min(n, ncores), ncores
 
 
 def theta_of(z, md):
@@ -187,6 +230,54 @@ def timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def bench_c1(ctx, device, iterations, mh_iterations=5000):
+    """The CPU-runnable configuration: 100x100x1 fully masked (10,000 voxels),
+    the same mono-exponential model, askadam (L1, lr 1e-3) and MH (5000
+    iterations, one chain); the CPU path runs the whole volume."""
+    import torch
+
+    dims = (100, 100, 1)
+    rng = np.random.default_rng(5438973)
+    nv = 10000
+    s0 = rng.uniform(0.5, 1.5, nv).astype(np.float32)
+    r2 = rng.uniform(10, 60, nv).astype(np.float32)
+    sigma = float(s0.mean()) / 100.0
+    y = synth_device(s0, r2, sigma, 5438973, device)
+    md = r2s_model([0, 0.1], [2, 200], 2)
+    cfg = askadam_cfg(iterations)
+    z0 = logit_z0(nv, md, device)
+    run_askadam(ctx, md, cfg, y, z0)  # warm-up
+    t_ask, (zf, vol) = timed(lambda: run_askadam(ctx, md, cfg, y, z0))
+    sub = subset_for_oracle(ctx, md, cfg, y, z0, zf, n=nv)
+    md3 = r2s_model([0, 0.1, 0.001], [2, 200, 0.1], 3)
+    x0 = torch.stack([torch.from_numpy(s0), torch.from_numpy(r2),
+                      torch.full((nv,), sigma, dtype=torch.float32)]).to(device).contiguous()
+    run_mh(ctx, md3, y, x0, 50, 1)
+    t_mh, (_, _, io) = timed(lambda: run_mh(ctx, md3, y, x0, mh_iterations, 1))
+    yh, x0h = y.cpu().numpy(), x0.cpu().numpy()
+
+    def cpu_path(nthreads, ncores):
+        from oracle import oracle as orc
+
+        t0 = time.perf_counter()
+        orc.askadam(md, cfg, yh, sub["z0"], precision=1, nthreads=nthreads)
+        t_a = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.mcmc(md3, mh_cfg(mh_iterations, 1), yh, x0h, store=False, precision=1, nthreads=nthreads)
+        t_m = time.perf_counter() - t0
+        return dict(kind="port", cores=nthreads, host_cores=ncores, askadam_voxels_per_s=nv / t_a,
+                    mh_samples_per_s=nv * mh_iterations / t_m,
+                    sample=f"the whole volume, oracle FP32 restatement on {nthreads} threads: askadam {t_a:.2f} s, "
+                           f"MH {t_m:.2f} s")
+
+    return dict(workload=f"mono-exponential R2* 100x100x1 (10000 voxels), 12 echoes 2:4:46 ms, SNR 100: askadam "
+                         f"{iterations} iterations L1/Adam lr 1e-3, MH {mh_iterations} iterations x 1 chain",
+                askadam_seconds=t_ask, voxels_per_s=nv / t_ask, askadam_kernel_ms=vol.kernel_ms,
+                mh_seconds=t_mh, mh_samples_per_s=nv * mh_iterations / t_mh, mh_kernel_ms=io.kernel_ms,
+                _oracle_check=lambda nthreads: oracle_check(md, cfg, sub, ["M0", "R2star"], nthreads),
+                _cpu_baseline=cpu_path)
 
 
 def bench_c3(ctx, device, iterations):
@@ -223,7 +314,7 @@ def bench_c3(ctx, device, iterations):
         z0 = torch.tensor(zc, dtype=torch.float32, device=device)[:, None].repeat(1, nv).contiguous()
         vols.append((y, b1, z0, th))
 
-    def run():
+    def run(c=cfg):
         arr = (_abi.gc_volume * 3)()
         keep = []
         for i, (y, b1, z0, _) in enumerate(vols):
@@ -233,24 +324,53 @@ def bench_c3(ctx, device, iterations):
             arr[i].extra_dev = runtime.ptr(b1)
             arr[i].z_dev = runtime.ptr(z)
             keep.append(z)
-        ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(cfg), arr, 3))
+        ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(c), arr, 3))
         return arr, keep
 
     run()  # warm-up
     t, (arr, zs) = timed(run)
     nv_total = sum(v[0].shape[1] for v in vols)
-    err = {}
-    for p, name in enumerate(("M0", "R1", "R2star")):
-        rel = []
-        for (y, b1, z0, th), z in zip(vols, zs):
-            est = md.lb[p] + (md.ub[p] - md.lb[p]) * torch.sigmoid(z[p])
-            rel.append(torch.abs(est - th[p]) / th[p])
-        err[name] = float(torch.median(torch.cat(rel)))
+
+    def gt_err(zlist):
+        err = {}
+        for p, name in enumerate(("M0", "R1", "R2star")):
+            rel = []
+            for (y, b1, z0, th), z in zip(vols, zlist):
+                est = md.lb[p] + (md.ub[p] - md.lb[p]) * torch.sigmoid(z[p])
+                rel.append(torch.abs(est - th[p]) / th[p])
+            err[name] = float(torch.median(torch.cat(rel)))
+        return err
+
+    err = gt_err(zs)
+    # the same fit converged (lr 1e-2, where the FP64 oracle reaches the error
+    # of a 10x longer lr 1e-3 run): the error floor that the Rician noise at
+    # SNR 80 sets for this protocol
+    _, zs_c = run(askadam().make_cfg(dict(f, initialLearnRate=1e-2), m))
+    floor = gt_err(zs_c)
+    del zs_c
+    # a 4000-voxel piece of the first volume fitted alone with that volume's N
+    # and its sign trace, for the oracle check of the CPU leg
+    y0, b10, z00, _ = vols[0]
+    nv0 = y0.shape[1]
+    idx = torch.from_numpy(np.sort(np.random.default_rng(9).choice(nv0, 4000, replace=False))).to(device)
+    ys, bs, zs0 = y0[:, idx].contiguous(), b10[:, idx].contiguous(), z00[:, idx].contiguous()
+    sg = torch.zeros((iterations, (md.n_meas + 15) // 16, idx.numel()), dtype=torch.int32, device=device)
+    a1 = (_abi.gc_volume * 1)()
+    zsub = zs0.clone()
+    a1[0].nv = idx.numel()
+    a1[0].y_dev, a1[0].extra_dev, a1[0].z_dev = runtime.ptr(ys), runtime.ptr(bs), runtime.ptr(zsub)
+    a1[0].n_mean, a1[0].sign_trace_dev = nv0, runtime.ptr(sg)
+    ctx.check(ctx.lib.gc_askadam_run(ctx.handle, ctypes.byref(md), ctypes.byref(cfg), a1, 1))
+    sub = dict(y=ys.cpu().numpy(), z0=zs0.cpu().numpy(), z=zsub.cpu().numpy(), signs=sg.cpu().numpy().view(np.uint32),
+               n_mean=nv0, same_as_whole_fit=bool(torch.equal(zsub, zs[0][:, idx])))
+    b1h = bs.cpu().numpy()
     return dict(workload="joint R1-R2* 4 FA x 12 TE (TE 2:2:24 ms, TR 30 ms), Rician SNR 80, volumes 0.4M/0.9M/1.6M "
                          f"voxels in one batched launch, {iterations} iterations L1/Adam",
                 voxels=nv_total, seconds=t, voxels_per_s=nv_total / t,
                 voxel_iterations_per_s=nv_total * iterations / t, kernel_ms=arr[0].kernel_ms,
-                launches=arr[0].kernel_launches, median_rel_err=err)
+                launches=arr[0].kernel_launches, median_rel_err=err, median_rel_err_converged_lr1e2=floor,
+                _oracle_check=lambda nthreads: oracle_check(md, cfg, sub, ["M0", "R1", "R2star"], nthreads,
+                                                            extra=b1h))
 
 
 def bench_c4(ctx, device, iterations, nv=500_000, walkers=16):
@@ -309,10 +429,16 @@ def bench_c4(ctx, device, iterations, nv=500_000, walkers=16):
     t, io = timed(lambda: run(c))
     steps = float(nv) * walkers * iterations
     err = {n: float(torch.median(torch.abs(mean[q] - th[q]) / th[q])) for q, n in enumerate(names[:4])}
+    # how wide the posterior is: a parameter whose posterior SD is comparable to
+    # (or larger than) its true value is not identified by these data, and its
+    # posterior mean sits about one SD from the truth by construction
+    width = {n: float(torch.median(std[q] / th[q])) for q, n in enumerate(names[:4])}
+    zdev = {n: float(torch.median((mean[q] - th[q]) / std[q].clamp_min(1e-12))) for q, n in enumerate(names[:4])}
     return dict(workload=f"NEXI (6 shells x 2 Delta, GL64, Rician mean) Goodman-Weare, {nv} voxels x {walkers} walkers x "
                          f"{iterations} steps (simultaneous update, native stream)",
                 seconds=t, walker_steps_per_s=steps / t, kernel_ms=io.kernel_ms, block_threads=io.block_threads,
-                transcendentals_per_s=steps * 3072 / t, posterior_mean_median_rel_err=err)
+                transcendentals_per_s=steps * 3072 / t, posterior_mean_median_rel_err=err,
+                posterior_sd_over_true_median=width, posterior_mean_minus_true_over_sd_median=zdev)
 
 
 def epgx_networks(seed=17, hidden=(64, 64, 64)):
@@ -586,6 +712,34 @@ def cpu_baseline_mh(s0, r2, sigma, iterations, reps, nthreads, nsample, seed=8):
     return time.perf_counter() - t0
 
 
+def _sigmoid_theta(z, md, P):
+    z = np.asarray(z, np.float64)
+    return np.stack([md.lb[p] + (md.ub[p] - md.lb[p]) / (1 + np.exp(-z[p])) for p in range(P)])
+
+
+def oracle_check(md, cfg, sub, names, nthreads, extra=None):
+    """Checker run of the CPU leg: the device fit of a subset (see
+    subset_for_oracle) against the FP64 oracle following the device's residual
+    signs, and against the oracle with its own signs.  Per-parameter relative
+    errors (median, 99.9th percentile, max)."""
+    from oracle import oracle as orc
+
+    P = len(names)
+    kw = dict(extra=extra, n_mean=sub["n_mean"], nthreads=nthreads)
+    t0 = time.perf_counter()
+    matched = orc.askadam(md, cfg, sub["y"], sub["z0"], sign_trace=sub["signs"], **kw)
+    own = orc.askadam(md, cfg, sub["y"], sub["z0"], **kw)
+    th = _sigmoid_theta(sub["z"], md, P)
+    out = {"voxels": int(sub["y"].shape[1]), "subset_equals_whole_fit": sub["same_as_whole_fit"],
+           "oracle_seconds": time.perf_counter() - t0}
+    for key, ref in (("matched_signs", matched), ("own_signs", own)):
+        tr = _sigmoid_theta(ref["z_final"], md, P)
+        e = np.abs(th - tr) / np.abs(tr)
+        out[key] = {n: {"median": float(np.median(e[p])), "p999": float(np.quantile(e[p], 0.999)),
+                        "max": float(e[p].max())} for p, n in enumerate(names)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -599,10 +753,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mh", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--configs", default="c3,c4,c5,c5epg,tvsmall,dict",
-                    help="further workloads to measure (c3, c4, c5, c5epg, tvsmall, dict or none)")
+    ap.add_argument("--configs", default="c1,c3,c4,c5,c5epg,tvsmall,dict",
+                    help="further workloads to measure (c1, c3, c4, c5, c5epg, tvsmall, dict or none)")
     ap.add_argument("--c3-iterations", type=int, default=2000)
-    ap.add_argument("--c4-iterations", type=int, default=500)
+    ap.add_argument("--c4-iterations", type=int, default=5000)
     ap.add_argument("--c5-iterations", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -664,6 +818,8 @@ def main():
     achieved_tflops = units * FLOP_PER_VOXEL_ITER / (k_ms * 1e-3) / 1e12
     trans_rate = units * TRANS_PER_VOXEL_ITER / (k_ms * 1e-3)
     launches = int(np.median(kl))
+    # a 10^4-voxel subset of this fit for the oracle check of the CPU leg
+    sub = None if args.skip_cpu else subset_for_oracle(ctx, md, cfg, y, z0, zf)
 
     # ---- plain per-op tape -----------------------------------------------
     tape = None
@@ -706,31 +862,53 @@ def main():
         mh["posterior_mean_median_rel_err"] = dict(M0=float(np.median(np.abs(m[0] - s0) / s0)),
                                                    R2star=float(np.median(np.abs(m[1] - r2) / r2)))
         log(f"MH: {t_mh:.2f} s, {mh['samples_per_s']:.3e} chain-steps/s")
+        # replay mode (the reference's draw layout, a caller-supplied stream) against
+        # the native stream on a 10^4-voxel subset: the replay draws alone are
+        # 4 x 5000 x 10^4 x (3 + 1) floats = 3.2 GB
+        nsub = min(10000, nv)
+        ys, xs = y[:, :nsub].contiguous(), x0[:, :nsub].contiguous()
+        gen = torch.Generator(device=device)
+        gen.manual_seed(11)
+        normals = torch.randn((args.mh_chains, args.mh_iterations, nsub, 3), generator=gen, device=device)
+        uniforms = torch.rand((args.mh_chains, args.mh_iterations, nsub), generator=gen, device=device)
+        run_mh(ctx, md3, ys, xs, args.mh_iterations, args.mh_chains, normals=normals, uniforms=uniforms)  # warm-up
+        t_rep, (_, _, io_r) = timed(lambda: run_mh(ctx, md3, ys, xs, args.mh_iterations, args.mh_chains,
+                                                   normals=normals, uniforms=uniforms))
+        t_nat, (_, _, io_n) = timed(lambda: run_mh(ctx, md3, ys, xs, args.mh_iterations, args.mh_chains))
+        steps_sub = float(nsub) * args.mh_chains * args.mh_iterations
+        mh["replay_vs_native_subset"] = dict(
+            voxels=nsub, replay_samples_per_s=steps_sub / t_rep, native_samples_per_s=steps_sub / t_nat,
+            replay_kernel_ms=io_r.kernel_ms, native_kernel_ms=io_n.kernel_ms,
+            replay_draw_bytes=int(normals.numel() * 4 + uniforms.numel() * 4))
+        log(f"MH replay vs native ({nsub} voxels): {t_rep * 1e3:.1f} ms vs {t_nat * 1e3:.1f} ms")
+        del normals, uniforms
 
     # ---- further workloads, reported beside the line ---------------------------
     extra = {}
     wanted = [c.strip() for c in args.configs.split(",") if c.strip() and c.strip() != "none"]
-    for name, fn, it in (("c3", bench_c3, args.c3_iterations), ("c4", bench_c4, args.c4_iterations),
+    for name, fn, it in (("c1", bench_c1, args.iterations), ("c3", bench_c3, args.c3_iterations),
+                         ("c4", bench_c4, args.c4_iterations),
                          ("c5", bench_c5, args.c5_iterations),
                          ("c5epg", lambda c, d, it: bench_c5(c, d, it, epg=True), args.c5_iterations // 4),
                          ("tvsmall", bench_tv_small, args.iterations),
                          ("dict", bench_dict, args.steps)):
         if name in wanted:
             extra[name] = fn(ctx, device, it)
-            log(f"{name}: {json.dumps(extra[name])}")
+            log(f"{name}: {json.dumps({k: v for k, v in extra[name].items() if not k.startswith('_')})}")
             torch.cuda.empty_cache()
 
     # ---- CPU path -------------------------------------------------------------
     cpu = None
+    oracle_rel_err = None
     if not args.skip_cpu:
-        ncores = len(os.sched_getaffinity(0))
-        nthreads = min(ncores, 16)
+        nthreads, ncores = omp_threads()
         # about 10 s of host work on 16 threads
         nsample = 16000 * nthreads
         t_cpu = cpu_baseline_askadam(s0, r2, sigma, args.iterations, nthreads, nsample)
-        cpu = dict(value=nsample / t_cpu, unit="voxels/s", cores=nthreads, kind="port",
+        cpu = dict(value=nsample / t_cpu, unit="voxels/s", cores=nthreads, host_cores=ncores, kind="port",
                    sample=f"askadam {args.iterations} iterations on {nsample} voxels of the same workload "
-                          f"(oracle FP32 restatement, OpenMP {nthreads} threads): {t_cpu:.1f} s")
+                          f"(oracle FP32 restatement, OpenMP {nthreads} threads = the process's CPU share, "
+                          f"OMP_NUM_THREADS, of {ncores} host cores in its affinity mask): {t_cpu:.1f} s")
         log(f"CPU askadam: {t_cpu:.2f} s for {nsample} voxels")
         if mh is not None:
             ns_mh = 4000 * nthreads
@@ -738,6 +916,22 @@ def main():
             cpu["mh_samples_per_s"] = ns_mh * args.mh_chains * args.mh_iterations / t_cmh
             cpu["mh_sample"] = f"MH {args.mh_chains}x{args.mh_iterations} on {ns_mh} voxels: {t_cmh:.1f} s"
             log(f"CPU MH: {t_cmh:.2f} s for {ns_mh} voxels")
+        # checker runs of the FP64 oracle on device results (not timed as a baseline)
+        if sub is not None:
+            oracle_rel_err = oracle_check(md, cfg, sub, ["M0", "R2star"], nthreads)
+            log(f"oracle check (headline subset): {json.dumps(oracle_rel_err)}")
+        for name, w in extra.items():
+            chk = w.pop("_oracle_check", None)
+            if chk is not None:
+                w["oracle_rel_err"] = chk(nthreads)
+                log(f"oracle check ({name}): {json.dumps(w['oracle_rel_err'])}")
+            cb = w.pop("_cpu_baseline", None)
+            if cb is not None:
+                w["cpu_baseline"] = cb(nthreads, ncores)
+                log(f"CPU path ({name}): {json.dumps(w['cpu_baseline'])}")
+    for w in extra.values():
+        w.pop("_oracle_check", None)
+        w.pop("_cpu_baseline", None)
 
     traffic = measured_traffic() if tuple(args.dims) == (256, 256, 128) and args.iterations == 2000 else None
     out = {
@@ -783,7 +977,7 @@ def main():
             "unit_of_work": "voxel-iteration",
         },
         "cpu_baseline": cpu,
-        "fused": {"final_loss": fused_loss, "median_rel_err": err_fused},
+        "fused": {"final_loss": fused_loss, "median_rel_err": err_fused, "oracle_rel_err": oracle_rel_err},
         "tape": tape,
         "mcmc": mh,
         "workloads": extra,

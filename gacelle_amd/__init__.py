@@ -6,7 +6,7 @@ from . import _abi, utils
 from ._abi import GacelleError, load_library
 from .askadam import askadam
 from .mcmc import mcmc
-from .models import gpuJointR1R2starMapping, gpuNEXI, gpuNEXIrice, gpuR2starMapping
+from .models import gpuJointR1R2starMapping, gpuMCRMWI, gpuNEXI, gpuNEXIrice, gpuR2starMapping
 
 __all__ = [
     "askadam",
@@ -15,6 +15,7 @@ __all__ = [
     "gpuJointR1R2starMapping",
     "gpuNEXI",
     "gpuNEXIrice",
+    "gpuMCRMWI",
     "utils",
     "GacelleError",
     "load_library",

@@ -144,6 +144,14 @@ class askadam:
         parameters[, extraData]), ...]."""
         model = _resolve_model(userfuncCell)
         fitting = self.check_set_default_basic(fitting)
+        lam = [float(x) for x in np.atleast_1d(fitting["lambda"])]
+        if not fitting["isOptimiseMemory"] and lam and lam[0] > 0:
+            # With isOptimiseMemory false the reference optimises whole N-D
+            # maps: voxels outside the mask next to masked ones enter the TV
+            # term and are learned (utils/spatial_total_variation.m:32-48).
+            # Only the masked (G-D) form is implemented.
+            raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "isOptimiseMemory = false with a regulariser (whole-grid "
+                                                        "parameter maps) is not supported; use the default true")
         ctx = runtime.context()
         lib = ctx.lib
         start = time.time()

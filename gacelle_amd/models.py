@@ -995,7 +995,19 @@ class gpuNEXI(_ModelBase):
         return out
 
 
-gpuNEXIrice = gpuNEXI  # the Rician variant is gpuNEXI(..., rician=True)
+class gpuNEXIrice(gpuNEXI):
+    """NEXI/gpuNEXIrice.m: the NEXI model whose forward signal carries the
+    Rician first-moment correction (FWD, NEXI/gpuNEXIrice.m:635-660) with
+    the noise map extraData['sigma'], normalised like the data.
+
+    gpuNEXIrice(b, Delta, Nav) as the constructor of :31;
+    estimate(dwi, mask, extraData, fitting, pars0) as :104;
+    fit(dwi, mask, fitting, extraData, pars0) as :203 (gpuNEXI.fit takes
+    pars0 before extradata).  quad selects the radial quadrature as for
+    gpuNEXI ('trapz14' is the reference's rule)."""
+
+    def __init__(self, b, Delta, Nav=None, quad="trapz14"):
+        super().__init__(b, Delta, Nav, quad=quad, rician=True)
 
 
 def mlp_pack(net, n_out):

@@ -1401,6 +1401,46 @@ int orc_surface_tv(const double* img, const uint8_t* mask, int64_t ngrid, const 
   return 0;
 }
 
+// askadam.rescale_parameters (utils/askadam.m:1077-1089) on theta [P x nv]:
+// sigmoid: clamp to [lb + 1e-4 (ub-lb), ub - 1e-4 (ub-lb)], rescale01, logit;
+// linear: rescale01 (:1092-1094).  FP64.
+int orc_rescale_parameters(const gc_model* md, int32_t transform, const double* theta, double* z, int64_t nv) {
+  for (int p = 0; p < md->n_par; ++p) {
+    const double lb = md->lb[p], ub = md->ub[p];
+    for (int64_t v = 0; v < nv; ++v) {
+      const double x = theta[size_t(p) * nv + v];
+      double out;
+      if (transform == GC_TRANSFORM_SIGMOID) {
+        const double eb = 1e-4 * (ub - lb);
+        const double c = std::max(std::min(x, ub - eb), lb + eb);
+        const double t = (c - lb) / (ub - lb);
+        out = std::log(t / (1 - t));
+      } else if (transform == GC_TRANSFORM_LINEAR) {
+        out = (x - lb) / (ub - lb);
+      } else {
+        out = x;
+      }
+      z[size_t(p) * nv + v] = out;
+    }
+  }
+  return 0;
+}
+
+// askadam.unscale_parameters (utils/askadam.m:1063-1073): lb + (ub-lb)
+// sigmoid(z), or unscale01 z (ub-lb) + lb (:1097-1099).  FP64.
+int orc_unscale_parameters(const gc_model* md, int32_t transform, const double* z, double* theta, int64_t nv) {
+  for (int p = 0; p < md->n_par; ++p) {
+    const double lb = md->lb[p], ub = md->ub[p];
+    for (int64_t v = 0; v < nv; ++v) {
+      const double x = z[size_t(p) * nv + v];
+      theta[size_t(p) * nv + v] = transform == GC_TRANSFORM_SIGMOID ? lb + (ub - lb) * (1 / (1 + std::exp(-x)))
+                                  : transform == GC_TRANSFORM_LINEAR ? x * (ub - lb) + lb
+                                                                     : x;
+    }
+  }
+  return 0;
+}
+
 // askadam; precision 0 = double (oracle), 1 = float (CPU-path baseline).
 // sign_trace (optional, L1): [iteration][ceil(M/16)][nv] residual sign codes
 // (gc_volume.sign_trace_dev layout) that replace the oracle's own L1 signs;

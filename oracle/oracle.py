@@ -69,6 +69,8 @@ def lib():
         L.orc_norm_inv.argtypes = [c_double]
         L.orc_norm_inv.restype = c_double
         L.orc_native_mh_draws.argtypes = [c_uint64, c_uint32, c_uint64, c_int32, POINTER(c_double), POINTER(c_double)]
+        L.orc_rescale_parameters.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_int64]
+        L.orc_unscale_parameters.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_int64]
         L.orc_native_gw_uniforms.argtypes = [c_uint64, c_uint32, c_uint64, c_int32, POINTER(c_double),
                                              POINTER(c_double)]
         L.orc_dict_search.argtypes = [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
@@ -136,6 +138,22 @@ def native_mh_draws(seed, it, chain, Q):
     u = c_double()
     assert lib().orc_native_mh_draws(seed, it, chain, Q, n, ctypes.byref(u)) == 0
     return np.array(n), u.value
+
+
+def rescale_parameters(md, transform, theta):
+    """askadam.rescale_parameters (utils/askadam.m:1077-1089), FP64: theta [P, nv] -> z."""
+    th = np.ascontiguousarray(theta, np.float64)
+    z = np.zeros_like(th)
+    assert lib().orc_rescale_parameters(ctypes.byref(md), transform, _p(th), _p(z), th.shape[1]) == 0
+    return z
+
+
+def unscale_parameters(md, transform, z):
+    """askadam.unscale_parameters (utils/askadam.m:1063-1073), FP64: z [P, nv] -> theta."""
+    zz = np.ascontiguousarray(z, np.float64)
+    th = np.zeros_like(zz)
+    assert lib().orc_unscale_parameters(ctypes.byref(md), transform, _p(zz), _p(th), zz.shape[1]) == 0
+    return th
 
 
 def native_gw_uniforms(seed, it, chain, sub=0):

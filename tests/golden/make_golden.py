@@ -246,6 +246,47 @@ def mcmc_fixtures(rng):
              **golden_io.pack("model", md), **golden_io.pack("cfg", cfg))
 
 
+def gw_c4_fixtures(rng):
+    """Goodman-Weare replay at the NEXI benchmark settings: 64-point
+    Gauss-Legendre quadrature, the Rician mean correction with its sigma row,
+    16 walkers, simultaneous and red-black updates, Rician data."""
+    nv, it, R, W = 12, 30, 1, 16
+    md = nexi_md("gl64", rician=True, solver="mcmc")
+    fa = rng.uniform(0.2, 0.8, nv)
+    th = np.stack([fa, rng.uniform(1.5, 3, nv), rng.uniform(0.5, 1.5, nv), (1 - fa) / rng.uniform(2, 50, nv),
+                   np.full(nv, 0.02)])
+    sigma = np.full((1, nv), 0.02, np.float32)
+    md0 = nexi_md("gl64", solver="mcmc")
+    s = orc.fwd_value(md0, th)
+    y = np.sqrt((s + 0.02 * rng.standard_normal(s.shape)) ** 2 + (0.02 * rng.standard_normal(s.shape)) ** 2)
+    y = y.astype(np.float32)
+    for redblack in (False, True):
+        cfg = mcmc_cfg(_abi.GC_MCMC_GW, it, 2, 4, R, W, (), redblack)
+        cfg.noise_slot, cfg.n_var = 4, 5
+        normals = rng.standard_normal((W, nv, 5)).astype(np.float32)
+        uniforms = rng.uniform(0, 1, (R, it, 2, W, nv)).astype(np.float32)
+        partners = np.zeros((R, it, W), np.int32)
+        h = W // 2
+        for r in range(R):
+            for k in range(it):
+                if redblack:
+                    partners[r, k, :h] = rng.integers(h, W, h)
+                    partners[r, k, h:] = rng.integers(0, h, W - h)
+                else:
+                    while True:
+                        p = rng.permutation(W)
+                        if np.all(p != np.arange(W)):
+                            break
+                    partners[r, k] = p
+        x0 = th.astype(np.float32)
+        o = orc.mcmc(md, cfg, y, x0, extra=sigma, normals=normals, uniforms=uniforms, partners=partners, trace=True,
+                     margin=True)
+        save(f"gw_replay_nexi_gl64_rician_w16_{'redblack' if redblack else 'simultaneous'}", y=y, x0=x0,
+             extra=sigma, normals=normals, uniforms=uniforms, partners=partners, posterior=o["posterior"],
+             mean=o["mean"], std=o["std"], accept=o["accept"], trace=o["trace"], margin=o["margin"],
+             **golden_io.pack("model", md), **golden_io.pack("cfg", cfg))
+
+
 def tv_fixture(rng):
     dims = (7, 6, 5)
     mask = (rng.random(dims) < 0.75).astype(np.uint8)
@@ -265,6 +306,7 @@ def main():
     askadam_fixtures(rng)
     mcmc_fixtures(rng)
     tv_fixture(rng)
+    gw_c4_fixtures(np.random.default_rng(20261204))
 
 
 if __name__ == "__main__":

@@ -14,9 +14,13 @@ GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def pack(prefix: str, st) -> dict:
+    """Numeric fields only: the network descriptors (gc_model.mlp) carry
+    device pointers and are not part of a fixture."""
     out = {}
     for name, ty in st._fields_:
         v = getattr(st, name)
+        if isinstance(v, ctypes.Array) and issubclass(v._type_, ctypes.Structure):
+            continue
         if isinstance(v, ctypes.Array):
             v = np.array(list(v))
         out[f"{prefix}.{name}"] = np.asarray(v)

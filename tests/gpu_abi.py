@@ -223,3 +223,23 @@ def tv(img, mask, mode, vsize, lam, nmasked):
     c.check(c.lib.gc_tv_fwd_adj(c.handle, runtime.ptr(imgd), runtime.ptr(maskd), dims, mode, vs, float(lam),
                                 int(nmasked), ctypes.byref(loss), runtime.ptr(g)))
     return loss.value, runtime.to_host(g).reshape(dims3, order="F")
+
+
+def rescale(md, transform, theta):
+    """gc_rescale_parameters: theta [P, nv] -> z."""
+    c = ctx()
+    th = c.to_device(np.ascontiguousarray(theta, np.float32))
+    z = c.empty(th.shape)
+    c.check(c.lib.gc_rescale_parameters(c.handle, ctypes.byref(md), transform, runtime.ptr(th), runtime.ptr(z),
+                                        th.shape[1]))
+    return runtime.to_host(z)
+
+
+def unscale(md, transform, z):
+    """gc_unscale_parameters: z [P, nv] -> theta."""
+    c = ctx()
+    zd = c.to_device(np.ascontiguousarray(z, np.float32))
+    th = c.empty(zd.shape)
+    c.check(c.lib.gc_unscale_parameters(c.handle, ctypes.byref(md), transform, runtime.ptr(zd), runtime.ptr(th),
+                                        zd.shape[1]))
+    return runtime.to_host(th)

@@ -18,7 +18,8 @@ ADJ = ["adj_r2star12_L1", "adj_r2star12_L2", "adj_r2star12_huber", "adj_r2star12
 ASK = ["askadam_r2star12_l1_ema", "askadam_r2star12_l2_tol", "askadam_joint12x4_l2"]
 # stops decided by thresholds that FP32 rounding can move by a few iterations
 ASK_MARGINAL = {"askadam_r2star12_l1_ema"}
-GW = ["gw_replay_nexi_simultaneous", "gw_replay_nexi_redblack"]
+GW = ["gw_replay_nexi_simultaneous", "gw_replay_nexi_redblack", "gw_replay_nexi_gl64_rician_w16_simultaneous",
+      "gw_replay_nexi_gl64_rician_w16_redblack"]
 TV = ["tv_2d", "tv_3d"]
 
 
@@ -86,8 +87,8 @@ def test_oracle_askadam_fixture(name):
 @pytest.mark.parametrize("name", GW + ["mh_replay_r2star12"])
 def test_oracle_mcmc_fixture(name):
     d = gio.load(name)
-    o = orc.mcmc(gio.model(d), gio.mcmc_cfg(d), d["y"], d["x0"], normals=d["normals"], uniforms=d["uniforms"],
-                 partners=gio.opt(d, "partners"), trace=True)
+    o = orc.mcmc(gio.model(d), gio.mcmc_cfg(d), d["y"], d["x0"], extra=gio.opt(d, "extra"), normals=d["normals"],
+                 uniforms=d["uniforms"], partners=gio.opt(d, "partners"), trace=True)
     np.testing.assert_array_equal(o["trace"], d["trace"])
     np.testing.assert_allclose(o["posterior"], d["posterior"], rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(o["mean"], d["mean"], rtol=1e-12)
@@ -186,8 +187,8 @@ def test_gpu_mcmc_replay_fixture(name):
 
     d = gio.load(name)
     cfg = gio.mcmc_cfg(d)
-    g = gpu_abi.mcmc(gio.model(d), cfg, d["y"], d["x0"], normals=d["normals"], uniforms=d["uniforms"],
-                     partners=gio.opt(d, "partners"), trace=True)
+    g = gpu_abi.mcmc(gio.model(d), cfg, d["y"], d["x0"], extra=gio.opt(d, "extra"), normals=d["normals"],
+                     uniforms=d["uniforms"], partners=gio.opt(d, "partners"), trace=True)
     # MH chains are independent: rounding stays at FP32 level.  A GW stretch
     # x_p + z (x - x_p) carries the partner's rounding into every walker, so
     # FP32/FP64 differences grow along the ensemble's 30 iterations.

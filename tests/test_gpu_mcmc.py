@@ -208,3 +208,52 @@ def test_unfilled_posterior_slots_are_zero(algo):
     assert np.all(g["posterior"][:, :, 8] == 0) and np.all(np.isfinite(g["posterior"]))
     p = g["posterior"].astype(np.float64)  # [Q, R, Ns, W, nv]
     np.testing.assert_allclose(g["mean"], p.transpose(0, 4, 1, 2, 3).reshape(3, nv, -1).mean(-1), rtol=1e-5)
+
+
+@pytest.mark.parametrize("redblack", [False, True])
+def test_nexi_gl64_rician_gw_statistics_match_oracle(redblack):
+    """Goodman-Weare at the NEXI benchmark settings (64-point Gauss-Legendre
+    quadrature, Rician mean with its sigma row, 16 walkers; utils/mcmc.m:
+    192-383, NEXI/gpuNEXIrice.m:635-660), native streams, against the FP64
+    oracle's chains.  The exchange rate is barely identified at this SNR, so
+    posteriors are broad and the Monte-Carlo error is calibrated, not
+    assumed: the oracle run twice with different seeds gives the spread of
+    posterior means that chance alone produces, and the device's departure
+    from the oracle must be of that size (median |d| at most 1.5x), unbiased,
+    with posterior SDs that agree."""
+    from gacelle_amd import models
+    from gacelle_amd.mcmc import mcmc as M
+
+    b = np.repeat(np.arange(1, 7, dtype=float), 2)
+    D = np.tile([20.0, 30.0], 6)
+    m = models.gpuNEXI(b, D, quad="gl64", rician=True)
+    names = ["fa", "Da", "De", "ra", "noise"]
+    f = M.check_set_default_basic(dict(modelParams=names, lb=list(m.lb[:4]) + [0.01], ub=list(m.ub[:4]) + [0.1]))
+    md = m.gc_model(f, "mcmc", None)
+    md0 = models.gpuNEXI(b, D, quad="gl64").gc_model(
+        M.check_set_default_basic(dict(modelParams=names[:4], lb=list(m.lb[:4]), ub=list(m.ub[:4]))), "mcmc", None)
+    md0.n_par = 5
+    rng = np.random.default_rng(44)
+    nv = 96
+    fa = rng.uniform(0.2, 0.8, nv)
+    th = np.stack([fa, rng.uniform(1.5, 3, nv), rng.uniform(0.5, 1.5, nv), (1 - fa) / rng.uniform(2, 50, nv),
+                   np.full(nv, 0.02)])
+    s = orc.fwd_value(md0, th)
+    y = np.sqrt((s + 0.02 * rng.standard_normal(s.shape)) ** 2 + (0.02 * rng.standard_normal(s.shape)) ** 2)
+    y = y.astype(np.float32)
+    sig = np.full((1, nv), 0.02, np.float32)
+    x0 = th.astype(np.float32)
+    cfg = mcmc_cfg(_abi.GC_MCMC_GW, 2000, 10, 200, rep=1, W=16, redblack=redblack, Q=5)
+    cfg.noise_slot = 4
+    g = gpu_abi.mcmc(md, cfg, y, x0, extra=sig, store=False)
+    o = orc.mcmc(md, cfg, y, x0, extra=sig, store=False, nthreads=16)
+    cfg2 = mcmc_cfg(_abi.GC_MCMC_GW, 2000, 10, 200, rep=1, W=16, redblack=redblack, Q=5, seed=777)
+    cfg2.noise_slot = 4
+    o2 = orc.mcmc(md, cfg2, y, x0, extra=sig, store=False, nthreads=16)
+    for q in range(5):
+        chance = np.median(np.abs(o2["mean"][q] - o["mean"][q]))
+        dev = g["mean"][q] - o["mean"][q]
+        assert np.median(np.abs(dev)) <= 1.5 * chance, (names[q], np.median(np.abs(dev)), chance)
+        assert abs(np.median(dev)) <= 0.5 * chance, (names[q], np.median(dev), chance)
+        ratio = g["std"][q] / o["std"][q]
+        assert 0.8 < np.median(ratio) < 1.25, (names[q], np.median(ratio))

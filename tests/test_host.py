@@ -129,3 +129,21 @@ def test_mcmc_global_detection_and_errors():
         mcmc().optimisation(data, mask, None, bad, dict(f, algorithm="GW"), models.gpuR2starMapping(te))
     g = mcmc.check_global_constant(dict(a=np.ones(3), b=np.float32(1), c=np.ones(1)), ["a", "b", "c"])
     assert list(g) == [False, True, True]
+
+
+def test_whole_grid_parameters_with_a_regulariser_are_refused():
+    """isOptimiseMemory = false with TV (utils/spatial_total_variation.m:32-48
+    on whole-grid maps) is refused before any device work, not silently fitted
+    in the masked form."""
+    import pytest
+
+    from gacelle_amd import GacelleError, _abi, models
+    from gacelle_amd.askadam import askadam
+
+    m = models.gpuR2starMapping(np.arange(2, 47, 4) * 1e-3)
+    f = {"modelParams": ["M0", "R2star"], "lb": [0, 0.1], "ub": [2, 200], "lambda": [1e-3], "regmap": ["R2star"],
+         "isOptimiseMemory": False}
+    with pytest.raises(GacelleError) as e:
+        askadam().optimisation(np.zeros((2, 2, 1, 12), np.float32), np.ones((2, 2, 1), bool), None,
+                               {"M0": np.ones((2, 2, 1)), "R2star": np.ones((2, 2, 1))}, f, m)
+    assert e.value.code == _abi.GC_ERR_UNSUPPORTED

@@ -219,3 +219,41 @@ def test_nexi_rician_prepare_masks():
         assert mk[idx] == (idx not in bad), idx
     np.testing.assert_allclose(exn["sigma"], sigma)    # no b=0 volume: b0 = 1
     assert np.all(np.isfinite(d2))
+
+
+def test_gpunexirice_is_the_rician_model_with_the_reference_argument_order(monkeypatch):
+    """gpuNEXIrice(b, Delta).fit(dwi, mask, fitting, extraData) (NEXI/
+    gpuNEXIrice.m:31, :203): the model carries the Rician mean correction,
+    extraData reaches the solver as the FWD's extra data (sigma) and is not
+    read as pars0; estimate keeps the (dwi, mask, extraData, fitting) order."""
+    import gacelle_amd as ga
+    from gacelle_amd import _abi
+
+    b = np.repeat(np.arange(1, 7, dtype=float), 2)
+    D = np.tile([20.0, 30.0], 6)
+    m = ga.gpuNEXIrice(b, D)
+    assert isinstance(m, models.gpuNEXI) and m.rician
+    f = m.check_set_default({})
+    m.updateProperty(f)
+    f["modelParams"], f["lb"], f["ub"] = list(m.modelParams), list(m.lb), list(m.ub)
+    assert m.gc_model(f, "askadam", None).flags & _abi.GC_FLAG_NEXI_RICIAN
+    assert not (models.gpuNEXI(b, D).gc_model(f, "askadam", None).flags & _abi.GC_FLAG_NEXI_RICIAN)
+    seen = {}
+
+    def fake_solve(self, data, mask, w, pars0, fitting, *fwd_args):
+        seen["pars0"], seen["fwd_args"] = pars0, fwd_args
+        return {}
+
+    monkeypatch.setattr(models.gpuNEXI, "_solve", fake_solve)
+    dwi = np.full((2, 2, 1, 12), 0.5, np.float32)
+    mask = np.ones((2, 2, 1), bool)
+    extra = {"sigma": np.full((2, 2, 1), 0.02, np.float32)}
+    m.fit(dwi, mask, {"start": "default"}, extra)
+    assert seen["fwd_args"] == (extra,)
+    assert set(seen["pars0"]) >= {"fa", "Da", "De", "ra"}
+    np.testing.assert_allclose(np.asarray(seen["pars0"]["fa"]), 0.4)   # the fixed start, not sigma
+    # gpuNEXI keeps its own order fit(dwi, mask, fitting, pars0, extradata)
+    p0 = {k: np.full((2, 2, 1), v, np.float32) for k, v in zip(models.gpuNEXI.modelParams, [0.3, 2, 1, 0.05, 0.2, 0.01])}
+    models.gpuNEXI(b, D).fit(dwi, mask, {"start": "default"}, p0, None)
+    np.testing.assert_allclose(np.asarray(seen["pars0"]["fa"]), 0.3)
+    assert "gpuMCRMWI" in ga.__all__ and ga.gpuMCRMWI is models.gpuMCRMWI

@@ -802,3 +802,26 @@ def test_robust_flags_the_unfittable_voxels():
     drops = [e for e in range(21, 200) if (e - 1) % 5 == 0]
     assert np.all(r[drops] < 0.5), r[drops].max()
     np.testing.assert_array_equal(r[:21], 1.0)
+
+
+def test_rescale_unscale_parameters():
+    """askadam.rescale_parameters / unscale_parameters (utils/askadam.m:
+    1063-1099): sigmoid maps theta inside [lb + eb, ub - eb] (eb = 1e-4 of
+    the range) to its logit and back exactly; outside, theta is clamped to
+    those points (z = -/+ logit(1 - 1e-4) = -/+ 9.2102...); linear is the
+    affine map to [0, 1] and back."""
+    md = _r2s_md()
+    lb, ub = np.array([md.lb[0], md.lb[1]]), np.array([md.ub[0], md.ub[1]])
+    rng = np.random.default_rng(1)
+    th = np.stack([rng.uniform(0.01, 1.99, 50), rng.uniform(1, 199, 50)])
+    for tr in (_abi.GC_TRANSFORM_SIGMOID, _abi.GC_TRANSFORM_LINEAR):
+        z = orc.rescale_parameters(md, tr, th)
+        np.testing.assert_allclose(orc.unscale_parameters(md, tr, z), th, rtol=1e-12)
+    zl = orc.rescale_parameters(md, _abi.GC_TRANSFORM_LINEAR, th)
+    np.testing.assert_allclose(zl, (th - lb[:, None]) / (ub - lb)[:, None], rtol=1e-15)
+    edge = np.array([[-1.0, 0.0, 2.0, 5.0], [0.0, 0.1, 200.0, 1e4]])
+    ze = orc.rescale_parameters(md, _abi.GC_TRANSFORM_SIGMOID, edge)
+    k = np.log((1 - 1e-4) / 1e-4)
+    np.testing.assert_allclose(ze, [[-k, -k, k, k], [-k, -k, k, k]], rtol=1e-9)
+    np.testing.assert_allclose(orc.unscale_parameters(md, _abi.GC_TRANSFORM_SIGMOID, np.zeros((2, 1)))[:, 0],
+                               (lb + ub) / 2, rtol=1e-15)

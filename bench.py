@@ -441,20 +441,6 @@ def bench_c4(ctx, device, iterations, nv=500_000, walkers=16):
                 posterior_sd_over_true_median=width, posterior_mean_minus_true_over_sd_median=zdev)
 
 
-def epgx_networks(seed=17, hidden=(64, 64, 64)):
-    """Random EPG-X networks (dlnet.parameters layout) of the given hidden
-    widths: the reference's trained networks are MATLAB objects this build
-    does not read, so the EPG-X workload times networks of that shape."""
-    rng = np.random.default_rng(seed)
-
-    def net(n_out, scale):
-        w = [11, *hidden, n_out]
-        return {"parameters": {f"fc{l}": {"Weights": rng.normal(0, scale / np.sqrt(w[l - 1]), (w[l], w[l - 1])),
-                                          "Bias": rng.normal(0, 0.05, w[l])} for l in range(1, len(w))}}
-
-    return net(1, 0.3), net(2, 0.05)
-
-
 def bench_c5(ctx, device, iterations, dims=(128, 128, 64), epg=False):
     """MCR-MWI complex 3-pool GRE (BM exchange), 24 echoes x 4 flip angles,
     complex Gaussian SNR 150, askadam with 3D TV on MWF (one launch per
@@ -467,11 +453,9 @@ def bench_c5(ctx, device, iterations, dims=(128, 128, 64), epg=False):
 
     te = np.arange(2, 49, 2) * 1e-3
     m = models.gpuMCRMWI(te, 38e-3, np.array([5.0, 10.0, 20.0, 50.0]))
-    opts = dict(isComplex=True, isFitExchange=True)
-    if epg:
-        nets = epgx_networks()
-        opts.update(epgx_phase_ann=nets[0], epgx_mag_ann=nets[1])
-    f = m.check_set_default(opts)
+    # epg: the reference's default T1 path with its bundled trained networks
+    # (11-20-30-40-45-60-65-75-1 / -2); otherwise the BM steady state
+    f = m.check_set_default(dict(isComplex=True, isFitExchange=True, isEPG=bool(epg)))
     m.updateProperty(f)
     f.update(modelParams=list(m.modelParams), lb=list(m.lb), ub=list(m.ub), iteration=iterations, tol=0,
              convergenceValue=-1, convergenceStepTol=0, convergenceGradTol=0, TVmode="3D", voxelSize=[1, 1, 1],
@@ -533,13 +517,15 @@ def bench_c5(ctx, device, iterations, dims=(128, 128, 64), epg=False):
     t, v = timed(lambda: run(cfg))
     vi = float(nv) * iterations
     bytes_per_vi = 4 * md.n_meas + 2 * 3 * md.n_par * 4 + 6 * 4  # y, (z, m, v) r/w, TV neighbour reads
-    path = "EPG-X MLP (networks 11-64-64-64-1 / -2, random weights) on BM" if epg else "BM"
+    path = "EPG-X MLP (the reference's trained networks 11-20-30-40-45-60-65-75-1 / -2) on BM" if epg else "BM"
     extra_out = {}
     if epg:
-        # per (voxel, flip angle): forward passes of both networks and one
-        # reverse pass per output (3), each sum_l w_l w_(l-1) FMAs
-        fma = lambda w: sum(w[l] * w[l - 1] for l in range(1, len(w)))
-        per_sample = 2 * fma([11, 64, 64, 64, 1]) + 3 * fma([11, 64, 64, 64, 2])
+        # per (voxel, flip angle): the forward pass of each network (sum over
+        # its layers of w_l w_(l-1) FMAs) and one reverse pass per output
+        # (the same sum without the last layer, whose row is used directly)
+        hid = [11, 20, 30, 40, 45, 60, 65, 75]
+        fwd = lambda w: sum(w[l] * w[l - 1] for l in range(1, len(w)))
+        per_sample = fwd(hid + [1]) + fwd(hid) + fwd(hid + [2]) + 2 * fwd(hid)
         extra_out = dict(epgx_flop_per_voxel_iteration=2 * 4 * per_sample)
     return dict(workload=f"MCR-MWI complex {path} 24 TE x 4 FA, {dims[0]}x{dims[1]}x{dims[2]} ({nv} voxels), 3D TV "
                          f"on MWF lambda 1e-3, {iterations} iterations L1/Adam, one step launch per iteration",

@@ -10,13 +10,13 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 7
+GC_ABI_VERSION = 8
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
 GC_MAX_REG = 4
-GC_MLP_MAX_LAYERS = 6
-GC_MLP_MAX_WIDTH = 64
+GC_MLP_MAX_LAYERS = 8
+GC_MLP_MAX_WIDTH = 80
 GC_NOALLOC_MAX_NV = 1 << 25
 GC_NOALLOC_SCRATCH_BYTES = 4096 + 8 * (GC_NOALLOC_MAX_NV // 256)
 

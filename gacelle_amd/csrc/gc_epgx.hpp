@@ -6,172 +6,217 @@
 // derivatives of their three outputs with respect to (MVF, R1IEW, kIEWM,
 // R2sIW), written as the 15 aux rows that McrmwiModel<..., EPG> reads.
 //
-// One thread per voxel runs its flip angles one after the other.  A
-// network's weights are the same for every lane, so they are read at
-// wave-uniform addresses (scalar loads) from a padded copy the library
-// builds: per layer a [WM][WM] weight block (row = output unit, zero
-// beyond the layer's widths) and a [WM] bias.  Activations live in
-// registers (WM each for the current and next layer); the signs of the
-// hidden pre-activations are kept as 64-bit masks in LDS for the reverse
-// passes, one per network output.
+// Every layer of a network is a product [units x samples] = W [units x in] x
+// A [in x samples], run on the FP32 matrix cores (v_mfma_f32_16x16x4_f32).
+// A wavefront holds kEpgxNB column blocks of 16 samples.  For that
+// instruction lane l holds D rows 4 (l/16) + r (r = 0..3) of column l % 16,
+// and it supplies B row l/16 of column l % 16: so instruction r of an input
+// tile t takes, as its B operand, register r of the previous layer's output
+// tile t, with B row g standing for unit 16 t + 4 g + r.  Activations and the
+// reverse-pass gradients therefore stay in registers from layer to layer;
+// only the weights (the A operand) are read, one float per lane and
+// instruction, from tables the library arranges in exactly that order.
 #pragma once
 #include "gc_kernels.hpp"
 
 namespace gc {
 
-// the networks as the kernels read them
-struct EpgxNets {
-  const float* w[2];                         // padded weights of the phase / magnitude network
-  int32_t nl[2];                             // layers
-  int32_t width[2][GC_MLP_MAX_LAYERS + 1];   // units per layer (width[.][0] = 11)
-  float alpha[2];                            // leaky ReLU scale
+constexpr int kEpgxMaxTiles = (GC_MLP_MAX_WIDTH + 15) / 16;  // 16-unit tiles of the widest layer
+constexpr int kEpgxNB = 2;                                   // column blocks of 16 samples per wavefront
+constexpr int kEpgxSamples = 16 * kEpgxNB;                   // samples (voxels) per wavefront
+// workgroups of the pre-pass per 256-voxel block of the askadam kernels: a
+// workgroup has one wavefront per flip angle over kEpgxSamples voxels
+constexpr int kEpgxGroupsPerBlock = kBlock / kEpgxSamples;
+
+// One layer of a network in the table buffer (offsets in floats).
+//  fwd  [nto][nti][4][64]: W[16 to + l%16][16 ti + 4 (l/16) + r]
+//  rev  [nti][nto][4][64]: W[16 to + 4 (l/16) + r][16 ti + l%16]
+//  bias [nto][4][64]:      b[16 to + 4 (l/16) + r]
+// (zero outside the layer's widths)
+struct EpgxLayer {
+  int32_t fwd, rev, bias;
+  int32_t nti, nto;
 };
 
-template <int WM>
-constexpr size_t epgx_layer_floats() { return size_t(WM) * WM + WM; }
+struct EpgxNets {
+  const float* tab;                        // every table of both networks
+  EpgxLayer layer[2][GC_MLP_MAX_LAYERS];
+  int32_t last[2];   // the last layer's rows [nout][nti][4][64]: W_L[o][16 ti + 4 (l/16) + r]
+  int32_t nl[2];     // layers
+  int32_t nout[2];   // outputs (1 phase, 2 magnitude)
+  float alpha[2];    // leaky ReLU scale
+};
 
-// Forward pass of one network on a[] (inputs, zero beyond width[0]); on
-// return a[] holds the outputs.  The sign mask of hidden layer l goes to
-// msk[l * kBlock] (this thread's column of an LDS table).  Zero rows beyond
-// a layer's width and zero chunks of 16 inputs are skipped by uniform
-// branches.  (Straight-line row groups, which let the scalar loads run
-// further ahead, raise the 64-unit kernel above 256 registers: one wave per
-// SIMD, measured no faster.)
-template <int WM>
-GC_DEV void mlp_fwd(const float* __restrict__ W, int nl, const int32_t* width, float alpha, float (&a)[WM],
-                    uint64_t* msk) {
+using f32x4 = __attribute__((ext_vector_type(4))) float;
+
+GC_DEV f32x4 mma16(float a, float b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// LDS of one wavefront: the 11 features of each of its samples and their
+// derivatives with respect to the 4 parameters ([f * 5 + 0] value,
+// [f * 5 + 1 + q] d/dq), and the sign masks of the hidden pre-activations
+// (bit 4 t + r of word [layer][block] of a lane: unit 16 t + 4 (l/16) + r of
+// its column is positive).
+struct EpgxLds {
+  float xs[55][kEpgxSamples];
+  uint32_t msk[GC_MLP_MAX_LAYERS][kEpgxNB][64];
+};
+
+// The 11 features of one sample with their derivatives, into column c of xs.
+GC_DEV void epgx_features(float mvf, float r1iew, float kfr, float r2siw, float fa, float tr, float t1mw,
+                          EpgxLds& L, int c) {
+  using D4 = DualF<4>;
+  const D4 fm = D4::var(mvf, 0), r1 = D4::var(r1iew, 1), kf = D4::var(kfr, 2), r2 = D4::var(r2siw, 3);
+  float sa, ca;
+  sincosf(fa, &sa, &ca);
+  const D4 kmw = (kf * (1.f - fm)) / fm;
+  D4 x[11];
+  x[0] = fm;
+  x[1] = dexp(-tr * r1);                       // exp(-TR / T1iew)
+  x[2] = dexp(-tr * kf);                       // exp(-TR kiewmw)
+  x[3] = D4::cst(sa);
+  x[4] = 1.f - ca * x[1];                      // 1 - E1 cos(alpha)
+  x[5] = dexp(-tr * kmw);                      // exp(-TR kmwiew)
+  x[6] = dexp(-tr * (r1 + kf));
+  x[7] = dexp(-tr * ((1.f / t1mw) + kmw));
+  x[8] = (sa * ((1.f - fm) * (1.f - x[1]))) / (1.f - ca * x[1]);   // Bloch steady state of IEW
+  const float e1m = __expf(-tr / t1mw);
+  x[9] = (sa * (1.f - e1m) / (1.f - ca * e1m)) * fm;                // ... of M
+  x[10] = dexp(-tr * r2);                      // exp(-TR / T2iew)
+#pragma unroll
+  for (int f = 0; f < 11; ++f) {
+    L.xs[f * 5][c] = x[f].v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) L.xs[f * 5 + 1 + q][c] = x[f].d[q];
+  }
+}
+
+// Forward pass of network `net` over the wavefront's column blocks: act
+// holds the input tiles on entry (features in tile 0) and the output tile on
+// return.  Hidden layers record their sign masks in L.msk.
+GC_DEV void epgx_forward(const EpgxNets& N, int net, f32x4 (&act)[kEpgxMaxTiles][kEpgxNB], EpgxLds& L, int lane) {
+  const float* __restrict__ tab = N.tab;
+  const int nl = N.nl[net];
+  const float alpha = N.alpha[net];
   for (int l = 0; l < nl; ++l) {
-    const float* __restrict__ Wl = W + size_t(l) * epgx_layer_floats<WM>();
-    const int wi = width[l], wo = width[l + 1];
-    float z[WM];
+    const EpgxLayer Ly = N.layer[net][l];
+    f32x4 acc[kEpgxMaxTiles][kEpgxNB];
 #pragma unroll
-    for (int j = 0; j < WM; ++j) {
-      float t = 0.f;
-      if (j < wo) {
-        t = Wl[WM * WM + j];
+    for (int to = 0; to < kEpgxMaxTiles; ++to) {
+      f32x4 b = {0.f, 0.f, 0.f, 0.f};
+      if (to < Ly.nto) {
 #pragma unroll
-        for (int i0 = 0; i0 < WM; i0 += 16) {
-          if (i0 < wi) {
+        for (int r = 0; r < 4; ++r) b[r] = tab[Ly.bias + (to * 4 + r) * 64 + lane];
+      }
 #pragma unroll
-            for (int i = i0; i < i0 + 16; ++i) t = fmaf(Wl[j * WM + i], a[i], t);
+      for (int k = 0; k < kEpgxNB; ++k) acc[to][k] = b;
+    }
+#pragma unroll
+    for (int ti = 0; ti < kEpgxMaxTiles; ++ti) {
+      if (ti < Ly.nti) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+          for (int to = 0; to < kEpgxMaxTiles; ++to) {
+            if (to < Ly.nto) {
+              const float a = tab[Ly.fwd + ((to * Ly.nti + ti) * 4 + r) * 64 + lane];
+#pragma unroll
+              for (int k = 0; k < kEpgxNB; ++k) acc[to][k] = mma16(a, act[ti][k][r], acc[to][k]);
+            }
           }
         }
       }
-      z[j] = t;
     }
     if (l + 1 < nl) {
-      uint64_t m = 0;
+      // leaky ReLU; padded units stay zero (zero weights and bias)
 #pragma unroll
-      for (int j = 0; j < WM; ++j) {
-        const bool pos = z[j] > 0.f;
-        m |= uint64_t(pos) << j;
-        a[j] = pos ? z[j] : alpha * z[j];
+      for (int k = 0; k < kEpgxNB; ++k) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int to = 0; to < kEpgxMaxTiles; ++to) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float z = acc[to][k][r];
+            const bool pos = z > 0.f;
+            m |= uint32_t(pos) << (to * 4 + r);
+            act[to][k][r] = pos ? z : alpha * z;
+          }
+        }
+        L.msk[l][k][lane] = m;
       }
-      msk[l * kBlock] = m;
     } else {
 #pragma unroll
-      for (int j = 0; j < WM; ++j) a[j] = z[j];
+      for (int to = 0; to < kEpgxMaxTiles; ++to)
+#pragma unroll
+        for (int k = 0; k < kEpgxNB; ++k) act[to][k] = acc[to][k];
     }
   }
 }
 
-// Reverse pass for output o: g[0..width[0]) = d out_o / d input.
-template <int WM>
-GC_DEV void mlp_rev(const float* __restrict__ W, int nl, const int32_t* width, float alpha, int o,
-                    const uint64_t* msk, float (&g)[WM]) {
+// Reverse pass of network `net` for output o: on return g[0] holds
+// d out_o / d feature for units 4 (l/16) + r of the lane's columns.
+GC_DEV void epgx_reverse(const EpgxNets& N, int net, int o, f32x4 (&g)[kEpgxMaxTiles][kEpgxNB], const EpgxLds& L,
+                         int lane) {
+  const float* __restrict__ tab = N.tab;
+  const int nl = N.nl[net];
+  const float alpha = N.alpha[net];
+  // d out_o / d (input of the last layer) = row o of its weights, through the
+  // derivative of the leaky ReLU that produced that input
+  const EpgxLayer Lz = N.layer[net][nl - 1];
 #pragma unroll
-  for (int j = 0; j < WM; ++j) g[j] = (j == o) ? 1.f : 0.f;
-  for (int l = nl - 1; l >= 0; --l) {
-    const float* __restrict__ Wl = W + size_t(l) * epgx_layer_floats<WM>();
-    const int wi = width[l], wo = width[l + 1];
-    float gi[WM];
+  for (int t = 0; t < kEpgxMaxTiles; ++t) {
 #pragma unroll
-    for (int i = 0; i < WM; ++i) gi[i] = 0.f;
+    for (int k = 0; k < kEpgxNB; ++k) g[t][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (t < Lz.nti) {
 #pragma unroll
-    for (int j = 0; j < WM; ++j) {
-      if (j < wo) {
+      for (int k = 0; k < kEpgxNB; ++k) {
+        const uint32_t m = (nl >= 2) ? L.msk[nl - 2][k][lane] : 0xffffffffu;
 #pragma unroll
-        for (int i0 = 0; i0 < WM; i0 += 16) {
-          if (i0 < wi) {
+        for (int r = 0; r < 4; ++r) {
+          const float w = tab[N.last[net] + ((o * Lz.nti + t) * 4 + r) * 64 + lane];
+          g[t][k][r] = ((m >> (t * 4 + r)) & 1u) ? w : alpha * w;
+        }
+      }
+    }
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    const EpgxLayer Ly = N.layer[net][l];
+    f32x4 acc[kEpgxMaxTiles][kEpgxNB];
 #pragma unroll
-            for (int i = i0; i < i0 + 16; ++i) gi[i] = fmaf(Wl[j * WM + i], g[j], gi[i]);
+    for (int ti = 0; ti < kEpgxMaxTiles; ++ti)
+#pragma unroll
+      for (int k = 0; k < kEpgxNB; ++k) acc[ti][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int to = 0; to < kEpgxMaxTiles; ++to) {
+      if (to < Ly.nto) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+          for (int ti = 0; ti < kEpgxMaxTiles; ++ti) {
+            if (ti < Ly.nti) {
+              const float a = tab[Ly.rev + ((ti * Ly.nto + to) * 4 + r) * 64 + lane];
+#pragma unroll
+              for (int k = 0; k < kEpgxNB; ++k) acc[ti][k] = mma16(a, g[to][k][r], acc[ti][k]);
+            }
           }
         }
       }
     }
-    if (l > 0) {
-      const uint64_t m = msk[(l - 1) * kBlock];
 #pragma unroll
-      for (int i = 0; i < WM; ++i) g[i] = ((m >> i) & 1ull) ? gi[i] : alpha * gi[i];
-    } else {
+    for (int ti = 0; ti < kEpgxMaxTiles; ++ti) {
 #pragma unroll
-      for (int i = 0; i < WM; ++i) g[i] = gi[i];
-    }
-  }
-}
-
-// One (voxel, flip angle) sample: rows k * nfa + fi of aux (stride nv) get
-// {ph, dph/d(MVF, R1IEW, kIEWM, R2sIW), dIEW, d.../d(...), dMW, d.../d(...)}.
-// fa = nominal flip angle x B1 [rad]; msk and xs point at this thread's
-// columns of the LDS tables (sign masks; the 11 features and their 4
-// derivatives, kept out of registers while the networks run).
-template <int WM>
-GC_DEV void epgx_sample(const EpgxNets& N, float mvf, float r1iew, float kfr, float r2siw, float fa, float tr,
-                        float t1mw, uint64_t* msk, float* xs, float* __restrict__ aux, int64_t rstride) {
-  {
-    using D4 = DualF<4>;
-    const D4 fm = D4::var(mvf, 0), r1 = D4::var(r1iew, 1), kf = D4::var(kfr, 2), r2 = D4::var(r2siw, 3);
-    float sa, ca;
-    sincosf(fa, &sa, &ca);
-    const D4 kmw = (kf * (1.f - fm)) / fm;
-    D4 x[11];
-    x[0] = fm;
-    x[1] = dexp(-tr * r1);                       // exp(-TR / T1iew)
-    x[2] = dexp(-tr * kf);                       // exp(-TR kiewmw)
-    x[3] = D4::cst(sa);
-    x[4] = 1.f - ca * x[1];                      // 1 - E1 cos(alpha)
-    x[5] = dexp(-tr * kmw);                      // exp(-TR kmwiew)
-    x[6] = dexp(-tr * (r1 + kf));
-    x[7] = dexp(-tr * ((1.f / t1mw) + kmw));
-    x[8] = (sa * ((1.f - fm) * (1.f - x[1]))) / (1.f - ca * x[1]);   // Bloch steady state of IEW
-    const float e1m = __expf(-tr / t1mw);
-    x[9] = (sa * (1.f - e1m) / (1.f - ca * e1m)) * fm;                // ... of M
-    x[10] = dexp(-tr * r2);                      // exp(-TR / T2iew)
+      for (int k = 0; k < kEpgxNB; ++k) {
+        const uint32_t m = (l >= 1) ? L.msk[l - 1][k][lane] : 0xffffffffu;
 #pragma unroll
-    for (int f = 0; f < 11; ++f) {
-      xs[(f * 5) * kBlock] = x[f].v;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) xs[(f * 5 + 1 + q) * kBlock] = x[f].d[q];
-    }
-  }
-  int k = 0;
-#pragma unroll 1
-  for (int net = 0; net < 2; ++net) {
-    float a[WM];
-#pragma unroll
-    for (int i = 0; i < WM; ++i) a[i] = (i < 11) ? xs[((i < 11 ? i : 0) * 5) * kBlock] : 0.f;
-    mlp_fwd<WM>(N.w[net], N.nl[net], N.width[net], N.alpha[net], a, msk);
-    const int nout = N.width[net][N.nl[net]];
-    const float val[2] = {a[0], a[1]};
-#pragma unroll 1
-    for (int o = 0; o < nout; ++o) {
-      float g[WM];
-      mlp_rev<WM>(N.w[net], N.nl[net], N.width[net], N.alpha[net], o, msk, g);
-      aux[int64_t(k) * rstride] = val[o];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float s = 0.f;
-#pragma unroll
-        for (int f = 0; f < 11; ++f) s = fmaf(g[f], xs[(f * 5 + 1 + q) * kBlock], s);
-        aux[int64_t(k + 1 + q) * rstride] = s;
+        for (int r = 0; r < 4; ++r) {
+          const float v = acc[ti][k][r];
+          g[ti][k][r] = ((m >> (ti * 4 + r)) & 1u) ? v : alpha * v;
+        }
       }
-      k += 5;
     }
   }
 }
 
-// MVF and the network inputs of MCR-MWI's parameter slots
-// (S0, MWF, IWF, R1IEW, kIEWM, R2sMW, R2sIW, ...), as McrmwiModel::amplitudes
+// MVF of MCR-MWI's parameter slots (S0, MWF, IWF, ...), as McrmwiModel::amplitudes
 GC_DEV float epgx_mvf(const float* th, float rho) {
   const float S0MW = th[0] * th[1];
   const float S0IW = th[0] * (1.f - th[1]) * th[2];
@@ -180,66 +225,126 @@ GC_DEV float epgx_mvf(const float* th, float rho) {
   return fmaxf(((1.f / rho) * S0MW) / tv, 1e-8f);
 }
 
-// One voxel's flip angles.  The pointers are __restrict__ so that the
-// weight loads are known not to be clobbered by the aux stores: the weights
-// then come through scalar loads (a plain pointer to aux lets every store
-// alias them, and the loads turn into per-lane vector loads).
-template <int WM, class TF>
-GC_DEV void epgx_voxel(const gc_model& md, const EpgxNets& N, const float* __restrict__ z,
-                       const float* __restrict__ ex, float* __restrict__ aux, int64_t nv, int64_t v, TF&& transform,
-                       uint64_t* msk, float* xs) {
-  float th[7];
+// One wavefront: flip angle fi of voxels v0 .. v0 + kEpgxSamples - 1 of a
+// volume.  Lanes l < kEpgxSamples compute the features of sample l (voxel
+// v0 + l); the matrix passes treat sample 16 k + j as column j of block k.
+// All lanes of the wavefront run every step (samples beyond nv are computed
+// on the last voxel and not stored).
+template <class TF>
+GC_DEV void epgx_wave(const gc_model& md, const EpgxNets& N, const float* __restrict__ z,
+                      const float* __restrict__ ex, float* __restrict__ aux, int64_t nv, int64_t v0, int fi,
+                      TF&& transform, EpgxLds& L) {
+  const int lane = threadIdx.x & 63;
+  const int nfa = md.i[1];
+  if (lane < kEpgxSamples) {
+    const int64_t v = v0 + lane;
+    const int64_t vv = v < nv ? v : nv - 1;
+    float th[7];
 #pragma unroll
-  for (int p = 0; p < 7; ++p) th[p] = transform(z[int64_t(p) * nv + v], md.lb[p], md.ub[p]);
-  const float mvf = epgx_mvf(th, md.f[3]);
-  const int nte = md.i[0], nfa = md.i[1];
-  const float b1 = ex[v];
-  for (int fi = 0; fi < nfa; ++fi)
-    epgx_sample<WM>(N, mvf, th[3], th[4], th[6], md.table[nte + fi] * b1, md.f[0], md.f[4], msk, xs,
-                    aux + int64_t(fi) * nv + v, int64_t(nfa) * nv);
+    for (int p = 0; p < 7; ++p) th[p] = transform(z[int64_t(p) * nv + vv], md.lb[p], md.ub[p]);
+    const float mvf = epgx_mvf(th, md.f[3]);
+    const int f = fi < nfa ? fi : nfa - 1;
+    epgx_features(mvf, th[3], th[4], th[6], md.table[md.i[0] + f] * ex[vv], md.f[0], md.f[4], L, lane);
+  }
+  __syncthreads();
+  const int g = lane >> 4, j = lane & 15;
+  int row = 0;  // aux row of the next output
+#pragma unroll 1
+  for (int net = 0; net < 2; ++net) {
+    f32x4 act[kEpgxMaxTiles][kEpgxNB];
+#pragma unroll
+    for (int t = 0; t < kEpgxMaxTiles; ++t)
+#pragma unroll
+      for (int k = 0; k < kEpgxNB; ++k) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = 16 * t + 4 * g + r;
+          act[t][k][r] = u < 11 ? L.xs[(u < 11 ? u : 0) * 5][16 * k + j] : 0.f;
+        }
+      }
+    epgx_forward(N, net, act, L, lane);
+    // outputs: unit o of the last layer sits in register o of lanes with g = 0
+    float val[kEpgxNB][2];
+#pragma unroll
+    for (int k = 0; k < kEpgxNB; ++k) {
+      val[k][0] = act[0][k][0];
+      val[k][1] = act[0][k][1];
+    }
+#pragma unroll 1
+    for (int o = 0; o < N.nout[net]; ++o) {
+      f32x4 gr[kEpgxMaxTiles][kEpgxNB];
+      epgx_reverse(N, net, o, gr, L, lane);
+#pragma unroll
+      for (int k = 0; k < kEpgxNB; ++k) {
+        const int64_t v = v0 + 16 * k + j;
+        float d[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          float s = 0.f;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int u = 4 * g + r;
+            if (u < 11) s = fmaf(gr[0][k][r], L.xs[u * 5 + 1 + q][16 * k + j], s);
+          }
+          // the four lane groups hold units 0-3, 4-7, 8-11, 12-15 of one column
+          s += __shfl_xor(s, 16);
+          s += __shfl_xor(s, 32);
+          d[q] = s;
+        }
+        if (g == 0 && v < nv && fi < nfa) {
+          float* __restrict__ out = aux + int64_t(fi) * nv + v;
+          const int64_t rs = int64_t(nfa) * nv;
+          out[int64_t(row) * rs] = o == 0 ? val[k][0] : val[k][1];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) out[int64_t(row + 1 + q) * rs] = d[q];
+        }
+      }
+      row += 5;
+    }
+  }
 }
 
-// the step form: aux rows of every volume's voxels from z_in (same block
-// mapping as k_askadam_step)
-template <int WM, int TR>
+// the step form: aux rows of every active volume's voxels from z_in.
+// Workgroup b of volume i (counted from kEpgxGroupsPerBlock blk0,
+// kEpgxSamples voxels each) has one wavefront per flip angle.
+template <int TR>
 __global__ __launch_bounds__(kBlock) void k_epgx_step(const gc_model* __restrict__ mdp, const StepParams sp,
                                                       const EpgxNets N) {
   int vi = 0;
   for (int i = 1; i < sp.n_vol; ++i)
-    if (int(blockIdx.x) >= sp.vols[i].blk0) vi = i;
+    if (int(blockIdx.x) >= kEpgxGroupsPerBlock * sp.vols[i].blk0) vi = i;
   const StepVol& V = sp.vols[vi];
-  if (!V.active) return;
-  const int64_t nv = V.nv;
-  const int64_t v = int64_t(blockIdx.x - V.blk0) * kBlock + threadIdx.x;
-  if (v >= nv) return;
-  __shared__ uint64_t msk[GC_MLP_MAX_LAYERS][kBlock];
-  __shared__ float xs[55][kBlock];
-  epgx_voxel<WM>(*mdp, N, V.z_in, V.ex, V.aux, nv, v,
-                 [](float z, float lb, float ub) {
-                   float t, d;
-                   transform_fwd_t<TR>(z, lb, ub, t, d);
-                   return t;
-                 },
-                 &msk[0][threadIdx.x], &xs[0][threadIdx.x]);
+  if (!V.active) return;  // uniform over the block
+  const int64_t v0 = int64_t(blockIdx.x - kEpgxGroupsPerBlock * V.blk0) * kEpgxSamples;
+  if (v0 >= V.nv) return;  // uniform over the block
+  __shared__ EpgxLds L[kBlock / 64];
+  const int w = threadIdx.x >> 6;
+  epgx_wave(*mdp, N, V.z_in, V.ex, V.aux, V.nv, v0, w,
+            [](float zz, float lb, float ub) {
+              float t, d;
+              transform_fwd_t<TR>(zz, lb, ub, t, d);
+              return t;
+            },
+            L[w]);
 }
 
 // the one-shot form (gc_fwd, gc_fwd_adj, the askadam outputs): aux rows
 // from parameters x in the given transform space
-template <int WM>
-__global__ __launch_bounds__(kBlock) void k_epgx_eval(const gc_model* __restrict__ mdp, const float* __restrict__ x,
-                                                      const float* __restrict__ ex, int64_t nv, int32_t transform,
-                                                      float* __restrict__ aux, const EpgxNets N) {
-  const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (v >= nv) return;
-  __shared__ uint64_t msk[GC_MLP_MAX_LAYERS][kBlock];
-  __shared__ float xs[55][kBlock];
-  epgx_voxel<WM>(*mdp, N, x, ex, aux, nv, v,
-                 [transform](float z, float lb, float ub) {
-                   float t, d;
-                   transform_fwd(transform, z, lb, ub, t, d);
-                   return t;
-                 },
-                 &msk[0][threadIdx.x], &xs[0][threadIdx.x]);
+__global__ __launch_bounds__(kBlock) static void k_epgx_eval(const gc_model* __restrict__ mdp,
+                                                             const float* __restrict__ x,
+                                                             const float* __restrict__ ex, int64_t nv,
+                                                             int32_t transform, float* __restrict__ aux,
+                                                             const EpgxNets N) {
+  const int64_t v0 = int64_t(blockIdx.x) * kEpgxSamples;
+  __shared__ EpgxLds L[kBlock / 64];
+  const int w = threadIdx.x >> 6;
+  epgx_wave(*mdp, N, x, ex, aux, nv, v0, w,
+            [transform](float zz, float lb, float ub) {
+              float t, d;
+              transform_fwd(transform, zz, lb, ub, t, d);
+              return t;
+            },
+            L[w]);
 }
 
 }  // namespace gc

@@ -320,12 +320,12 @@ struct ModelOps {
 };
 
 // ---- EPG-X networks (MCR-MWI isEPG) ------------------------------------------
-// The two networks of a gc_model in the padded layout the k_epgx kernels
-// read (gc_epgx.hpp), built once per call; wm = 16, 32 or 64 units.
+// The two networks of a gc_model as the matrix-core weight tables the k_epgx
+// kernels read (gc_epgx.hpp), built once per call.
 struct EpgxDev {
   DevBuf wbuf;
   EpgxNets nets{};
-  int wm = 0;
+  bool ready = false;
 };
 int epgx_prepare(gc_ctx* ctx, const gc_model& md, EpgxDev& out);
 // aux rows from z_in of every active volume of a step launch (epgx.hip)

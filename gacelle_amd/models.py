@@ -13,11 +13,15 @@ from __future__ import annotations
 
 import copy
 import ctypes
+import os
 
 import numpy as np
 
 from . import _abi, dwi, runtime, segment, utils
 from ._abi import GacelleError
+
+# the reference's trained EPG-X networks of MCR-MWI as plain arrays
+_DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 class _ModelBase:
@@ -1108,10 +1112,12 @@ class gpuMCRMWI(_ModelBase):
             dimwi.setdefault(k, True)
         f2["DIMWI"] = dimwi
         f2.setdefault("isFitExchange", True)
-        # The reference defaults to the EPG-X path with its bundled networks
-        # (MATLAB dlnet objects this library does not read): here isEPG is on
-        # when the caller passes the networks, otherwise the BM path runs.
-        f2.setdefault("isEPG", fitting.get("epgx_phase_ann") is not None and fitting.get("epgx_mag_ann") is not None)
+        # the EPG-X path with the bundled networks by default (MCRMWI/gpuMCRMWI.m:
+        # 1094, :1104-1105): the reference's trained networks, extracted into
+        # gacelle_amd/data (extract_epgx_networks.py)
+        f2.setdefault("isEPG", True)
+        f2.setdefault("epgx_phase_ann", os.path.join(_DATA_DIR, "mcrmwi_epgx_phase.npz"))
+        f2.setdefault("epgx_mag_ann", os.path.join(_DATA_DIR, "mcrmwi_epgx_magn.npz"))
         f2.setdefault("isComplex", True)
         if data is not None and not np.iscomplexobj(data):
             f2["isComplex"] = False
@@ -1352,9 +1358,9 @@ class gpuMCRMWI(_ModelBase):
         if not all(dimwi.get(k, True) for k in ("isFitFreqMW", "isFitFreqIW", "isFitR2sEW", "isFitIWF")):
             raise GacelleError(_abi.GC_ERR_UNSUPPORTED,
                                "gpuMCRMWI: the HCFM paths are not available (DIMWI parameters fitted)")
-        epg = bool(fitting.get("isEPG", False))
-        if epg and not fitting.get("isFitExchange", True):
-            raise GacelleError(_abi.GC_ERR_ARG, "gpuMCRMWI: isEPG needs isFitExchange")
+        # isEPG selects the T1 model only under isFitExchange (MCRMWI/gpuMCRMWI.m:
+        # 663-664); without exchange the Bloch steady state runs
+        epg = bool(fitting.get("isEPG", False)) and bool(fitting.get("isFitExchange", True))
         md = self._base_model(fitting)
         md.flags = self._flags(fitting)
         if epg:
@@ -1364,8 +1370,7 @@ class gpuMCRMWI(_ModelBase):
                 if fitting.get(key) is None:
                     raise GacelleError(_abi.GC_ERR_ARG, f"gpuMCRMWI: isEPG needs fitting['{key}']: the network as "
                                        "{'parameters': {'fc1': {'Weights', 'Bias'}, ...}} or an .npz of fc<l>_Weights "
-                                       "/ fc<l>_Bias (the reference's .mat files hold MATLAB dlnet objects, which "
-                                       "this library does not read)")
+                                       "/ fc<l>_Bias")
                 widths, alpha, packed = mlp_pack(fitting[key], n_out=1 if n == 0 else 2)
                 m = md.mlp[n]
                 m.n_layer = len(widths) - 1

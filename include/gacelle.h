@@ -34,13 +34,13 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 7
+#define GC_ABI_VERSION 8
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
 #define GC_MAX_REG 4    /* regularised parameter maps             */
-#define GC_MLP_MAX_LAYERS 6  /* fully connected layers per network     */
-#define GC_MLP_MAX_WIDTH 64  /* units per layer (inputs included)      */
+#define GC_MLP_MAX_LAYERS 8  /* fully connected layers per network     */
+#define GC_MLP_MAX_WIDTH 80  /* units per layer (inputs included)      */
 
 /* ---- status codes (mirroring the MATLAB error identifiers) ------------- */
 enum {

@@ -17,11 +17,15 @@ pytestmark = pytest.mark.gpu
 
 
 def _epg(cplx=True, widths=((11, 24, 24, 1), (11, 24, 24, 2)), seed=7):
-    rng = np.random.default_rng(seed)
-    nets = (random_net(rng, list(widths[0])), random_net(rng, list(widths[1]), scale=0.1))
+    """widths None: the reference's trained networks, the model's default."""
     m = models.gpuMCRMWI(TE24, 38e-3, FA4)
-    f = m.check_set_default(dict(isComplex=cplx, isFitExchange=True, epgx_phase_ann=nets[0],
-                                 epgx_mag_ann=nets[1]))
+    if widths is None:
+        f = m.check_set_default(dict(isComplex=cplx, isFitExchange=True))
+    else:
+        rng = np.random.default_rng(seed)
+        nets = (random_net(rng, list(widths[0])), random_net(rng, list(widths[1]), scale=0.1))
+        f = m.check_set_default(dict(isComplex=cplx, isFitExchange=True, epgx_phase_ann=nets[0],
+                                     epgx_mag_ann=nets[1]))
     assert f["isEPG"]
     m.updateProperty(f)
     f["modelParams"], f["lb"], f["ub"] = list(m.modelParams), list(m.lb), list(m.ub)
@@ -53,10 +57,12 @@ def _extra(m, nv, rng):
 
 
 @pytest.mark.parametrize("widths", [((11, 24, 24, 1), (11, 24, 24, 2)), ((11, 40, 40, 40, 1), (11, 64, 2)),
-                                    ((11, 8, 1), (11, 12, 2))])
+                                    ((11, 8, 1), (11, 12, 2)), ((11, 80, 80, 80, 80, 1), (11, 80, 2)),
+                                    None])
 def test_epgx_forward_matches_oracle(widths):
-    """gc_fwd runs the pre-pass then the EPG model (16-, 32- and 64-unit
-    network classes)."""
+    """gc_fwd runs the pre-pass then the EPG model: networks of one to five
+    16-unit tiles per layer (the widest the ABI takes, 80 units), and the
+    reference's trained 8-layer networks (None)."""
     print("epgx forward", widths, flush=True)
     m, f, md = _epg(True, widths)
     rng = np.random.default_rng(4)
@@ -74,11 +80,12 @@ def test_epgx_forward_matches_oracle(widths):
     assert np.abs(orc.fwd_value(bm, th, ex) - o).max() / scale > 1e-3
 
 
+@pytest.mark.parametrize("nets", ["random", "reference"])
 @pytest.mark.parametrize("cplx", [True, False])
 @pytest.mark.parametrize("loss", [_abi.GC_LOSS_L1, _abi.GC_LOSS_L2])
-def test_epgx_adjoint_matches_oracle_gradient(cplx, loss):
-    print("epgx adjoint", cplx, loss, flush=True)
-    m, f, md = _epg(cplx)
+def test_epgx_adjoint_matches_oracle_gradient(cplx, loss, nets):
+    print("epgx adjoint", cplx, loss, nets, flush=True)
+    m, f, md = _epg(cplx, None if nets == "reference" else ((11, 24, 24, 1), (11, 24, 24, 2)))
     rng = np.random.default_rng(5)
     nv = 200
     th = _theta(m, f, nv, rng)
@@ -101,12 +108,12 @@ def test_epgx_adjoint_matches_oracle_gradient(cplx, loss):
         assert np.quantile(err, 0.99) < 1e-4 and err.max() < 5e-3, (p, err.max())
 
 
-@pytest.mark.parametrize("graphs", [1, 0])
-def test_askadam_epgx_matches_oracle(graphs):
+@pytest.mark.parametrize("graphs,nets", [(1, "random"), (0, "random"), (1, "reference")])
+def test_askadam_epgx_matches_oracle(graphs, nets):
     """60 Adam iterations of the step form with the pre-pass before every
-    step kernel (graph replay and direct launches)."""
-    print("epgx askadam", graphs, flush=True)
-    m, f, md = _epg(True)
+    step kernel (graph replay and direct launches; the reference's networks)."""
+    print("epgx askadam", graphs, nets, flush=True)
+    m, f, md = _epg(True, None if nets == "reference" else ((11, 24, 24, 1), (11, 24, 24, 2)))
     dims = (8, 7, 3)
     rng = np.random.default_rng(21)
     mask = rng.random(dims) < 0.8

@@ -19,7 +19,7 @@ FA4 = np.array([5.0, 10.0, 20.0, 50.0])
 
 def _mcrmwi(cplx=True, exch=True, te=TE24, lam=0.0, tv="3D"):
     m = models.gpuMCRMWI(te, 38e-3, FA4)
-    f = m.check_set_default(dict(isComplex=cplx, isFitExchange=exch))
+    f = m.check_set_default(dict(isComplex=cplx, isFitExchange=exch, isEPG=False))
     m.updateProperty(f)
     f["modelParams"] = list(m.modelParams)
     f["lb"], f["ub"] = list(m.lb), list(m.ub)

@@ -617,7 +617,7 @@ def test_gw_posterior_of_analytic_target(redblack):
 # solve of the two-pool exchange equations, and its no-exchange limit
 def _mcrmwi_md(exch, cplx, te=(1e-6, 2e-6, 3e-6)):
     m = models.gpuMCRMWI(np.asarray(te), 38e-3, np.array([5.0, 10.0, 20.0, 50.0]))
-    f = m.check_set_default(dict(isComplex=cplx, isFitExchange=exch))
+    f = m.check_set_default(dict(isComplex=cplx, isFitExchange=exch, isEPG=False))
     m.updateProperty(f)
     f["modelParams"], f["lb"], f["ub"] = list(m.modelParams), list(m.lb), list(m.ub)
     return m, f, m.gc_model(f, "askadam", None)

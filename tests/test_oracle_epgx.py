@@ -65,9 +65,13 @@ def np_mlp(net, x):
 
 
 def _model(nets, cplx=True):
+    """nets: (phase, magnitude) networks, None for the BM path, or "default"
+    for the model's defaults (the reference's trained networks)."""
     m = models.gpuMCRMWI(TE24, 38e-3, FA4)
     fit = dict(isComplex=cplx, isFitExchange=True)
-    if nets is not None:
+    if nets is None:
+        fit["isEPG"] = False
+    elif nets != "default":
         fit.update(epgx_phase_ann=nets[0], epgx_mag_ann=nets[1])
     f = m.check_set_default(fit)
     m.updateProperty(f)
@@ -97,6 +101,53 @@ def test_epgx_sample_matches_numpy_restatement_and_differences():
         a2[key] -= h
         fd = (np_out(a1) - np_out(a2)) / (2 * h)
         np.testing.assert_allclose(out[[1 + q, 6 + q, 11 + q]], fd, rtol=1e-5, atol=1e-9, err_msg=key)
+
+
+def _npz_net(path):
+    with np.load(path, allow_pickle=False) as d:
+        p, l = {}, 1
+        while f"fc{l}_Weights" in d:
+            p[f"fc{l}"] = {"Weights": d[f"fc{l}_Weights"].astype(float), "Bias": d[f"fc{l}_Bias"].astype(float)}
+            l += 1
+    return {"parameters": p, "alpha": 0.01}
+
+
+def test_default_model_runs_the_reference_networks():
+    """With no networks given, gpuMCRMWI takes the EPG-X path (isEPG = true,
+    MCRMWI/gpuMCRMWI.m:1094) with the reference's trained networks
+    (:1104-1105, 8 layers 11-20-30-40-45-60-65-75 then 1 / 2 outputs, as
+    extracted into gacelle_amd/data).  The oracle's features, both networks
+    and the derivatives of the three outputs match the numpy restatement of
+    the feature script and of mlp_model_leakyRelu on those weights, and
+    central differences of it, at samples across the parameter ranges."""
+    import os
+
+    m, f, md = _model("default")
+    assert f["isEPG"] and md.flags & _abi.GC_FLAG_MCRMWI_EPG
+    widths = [11, 20, 30, 40, 45, 60, 65, 75]
+    for n, nout in ((0, 1), (1, 2)):
+        assert md.mlp[n].n_layer == 8
+        assert [md.mlp[n].width[l] for l in range(9)] == widths + [nout]
+    nets = [_npz_net(os.path.join(models._DATA_DIR, f"mcrmwi_epgx_{k}.npz")) for k in ("phase", "magn")]
+    for args in (dict(mvf=0.12, r1iew=1.1, kfr=2.5, r2siw=22.0, fa=np.deg2rad(20.0) * 1.05),
+                 dict(mvf=0.05, r1iew=0.8, kfr=1.0, r2siw=15.0, fa=np.deg2rad(5.0) * 0.9),
+                 dict(mvf=0.3, r1iew=1.6, kfr=5.0, r2siw=30.0, fa=np.deg2rad(50.0) * 1.1)):
+        out = orc.epgx_sample(md, **args)
+
+        def np_out(a):
+            x = np_features(a["mvf"], 1 / a["r1iew"], a["kfr"], a["fa"], float(np.float32(38e-3)), 1 / a["r2siw"],
+                            float(np.float32(234e-3)))
+            return np.concatenate([np_mlp(nets[0], x), np_mlp(nets[1], x)])
+
+        ref = np_out(args)
+        np.testing.assert_allclose(out[[0, 5, 10]], ref, rtol=1e-10, atol=1e-12)
+        for q, key in enumerate(("mvf", "r1iew", "kfr", "r2siw")):
+            h = 1e-6 * max(abs(args[key]), 1)
+            a1, a2 = dict(args), dict(args)
+            a1[key] += h
+            a2[key] -= h
+            fd = (np_out(a1) - np_out(a2)) / (2 * h)
+            np.testing.assert_allclose(out[[1 + q, 6 + q, 11 + q]], fd, rtol=2e-5, atol=1e-8, err_msg=key)
 
 
 @pytest.mark.parametrize("cplx", [True, False])

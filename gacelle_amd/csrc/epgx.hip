@@ -51,34 +51,40 @@ int epgx_prepare(gc_ctx* ctx, const gc_model& md, EpgxDev& out) {
       EpgxLayer& Ly = N.layer[n][l - 1];
       Ly.nti = nti;
       Ly.nto = nto;
+      // every entry: the four floats (r = 0..3) of one lane, contiguous
       Ly.fwd = int32_t(tab.size());
       for (int to = 0; to < nto; ++to)
         for (int ti = 0; ti < nti; ++ti)
-          for (int r = 0; r < 4; ++r)
-            for (int ln = 0; ln < 64; ++ln) tab.push_back(W(16 * to + (ln & 15), 16 * ti + 4 * (ln >> 4) + r));
+          for (int ln = 0; ln < 64; ++ln)
+            for (int r = 0; r < 4; ++r) tab.push_back(W(16 * to + (ln & 15), 16 * ti + 4 * (ln >> 4) + r));
       Ly.rev = int32_t(tab.size());
       for (int ti = 0; ti < nti; ++ti)
         for (int to = 0; to < nto; ++to)
-          for (int r = 0; r < 4; ++r)
-            for (int ln = 0; ln < 64; ++ln) tab.push_back(W(16 * to + 4 * (ln >> 4) + r, 16 * ti + (ln & 15)));
+          for (int ln = 0; ln < 64; ++ln)
+            for (int r = 0; r < 4; ++r) tab.push_back(W(16 * to + 4 * (ln >> 4) + r, 16 * ti + (ln & 15)));
       Ly.bias = int32_t(tab.size());
       for (int to = 0; to < nto; ++to)
-        for (int r = 0; r < 4; ++r)
-          for (int ln = 0; ln < 64; ++ln) tab.push_back(B(16 * to + 4 * (ln >> 4) + r));
+        for (int ln = 0; ln < 64; ++ln)
+          for (int r = 0; r < 4; ++r) tab.push_back(B(16 * to + 4 * (ln >> 4) + r));
       if (l == m.n_layer) {
         N.last[n] = int32_t(tab.size());
         for (int o = 0; o < no; ++o)
           for (int ti = 0; ti < nti; ++ti)
-            for (int r = 0; r < 4; ++r)
-              for (int ln = 0; ln < 64; ++ln) tab.push_back(W(o, 16 * ti + 4 * (ln >> 4) + r));
+            for (int ln = 0; ln < 64; ++ln)
+              for (int r = 0; r < 4; ++r) tab.push_back(W(o, 16 * ti + 4 * (ln >> 4) + r));
       }
       src = bsrc + no;
     }
   }
+  // zero tail: epgx_reverse reads whole kEpgxMaxTiles rows of the last tables
+  tab.resize(tab.size() + size_t(kEpgxMaxTiles) * 64 * 4, 0.f);
+  // (at most 2 x 8 layers x 3 tables x 25 tile pairs x 1 KiB: far below the
+  // buffer resource's 32-bit range)
   GC_HIP(ctx, out.wbuf.alloc(sizeof(float) * tab.size()));
   GC_HIP(ctx, hipMemcpyAsync(out.wbuf.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
   GC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // tab is freed on return
   N.tab = out.wbuf.as<float>();
+  N.tab_bytes = int32_t(tab.size() * sizeof(float));
   out.ready = true;
   return GC_OK;
 }

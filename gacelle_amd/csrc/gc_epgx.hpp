@@ -28,11 +28,16 @@ constexpr int kEpgxSamples = 16 * kEpgxNB;                   // samples (voxels)
 // workgroup has one wavefront per flip angle over kEpgxSamples voxels
 constexpr int kEpgxGroupsPerBlock = kBlock / kEpgxSamples;
 
-// One layer of a network in the table buffer (offsets in floats).
-//  fwd  [nto][nti][4][64]: W[16 to + l%16][16 ti + 4 (l/16) + r]
-//  rev  [nti][nto][4][64]: W[16 to + 4 (l/16) + r][16 ti + l%16]
-//  bias [nto][4][64]:      b[16 to + 4 (l/16) + r]
-// (zero outside the layer's widths)
+// One layer of a network in the table buffer (offsets in floats; every entry
+// is one f32x4 per lane, the four weights a lane feeds to the four
+// instructions r = 0..3 of an input tile, so a lane fetches them with one
+// 16-byte load):
+//  fwd  [nto][nti][64]: W[16 to + l%16][16 ti + 4 (l/16) + r]
+//  rev  [nti][nto][64]: W[16 to + 4 (l/16) + r][16 ti + l%16]
+//  bias [nto][64]:      b[16 to + 4 (l/16) + r]
+// (zero outside the layer's widths).  The buffer ends with kEpgxMaxTiles x 64
+// zero entries, so reading a whole kEpgxMaxTiles row of the last table stays
+// inside it.
 struct EpgxLayer {
   int32_t fwd, rev, bias;
   int32_t nti, nto;
@@ -40,8 +45,9 @@ struct EpgxLayer {
 
 struct EpgxNets {
   const float* tab;                        // every table of both networks
+  int32_t tab_bytes;                       // its size
   EpgxLayer layer[2][GC_MLP_MAX_LAYERS];
-  int32_t last[2];   // the last layer's rows [nout][nti][4][64]: W_L[o][16 ti + 4 (l/16) + r]
+  int32_t last[2];   // the last layer's rows [nout][nti][64]: W_L[o][16 ti + 4 (l/16) + r]
   int32_t nl[2];     // layers
   int32_t nout[2];   // outputs (1 phase, 2 magnitude)
   float alpha[2];    // leaky ReLU scale
@@ -90,130 +96,166 @@ GC_DEV void epgx_features(float mvf, float r1iew, float kfr, float r2siw, float 
   }
 }
 
+using EpgxRsrc = __amdgpu_buffer_rsrc_t;
+
+// The table buffer as a raw buffer resource (gfx9 word 3: 32-bit data
+// format); reads past its end return zero.
+GC_DEV EpgxRsrc epgx_rsrc(const EpgxNets& N) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(N.tab), 0, N.tab_bytes, 0x00020000);
+}
+
+// Entry e of the table that starts `base` floats into the buffer: one
+// 16-byte buffer load with the lane's offset in a vector register shared by
+// every load and the table offset in a scalar register.
+GC_DEV f32x4 tab_entry(EpgxRsrc rs, int lane, int32_t base, int e) {
+  return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, lane * 16, base * 4 + e * 1024, 0));
+}
+
+// One forward layer with NTI input and NTO output tiles, known at compile
+// time: the layer is one straight block, so the scheduler issues its weight
+// loads well ahead of the matrix instructions that use them.  Hidden layers
+// apply the leaky ReLU and record their sign masks in msk.
+template <int NTI, int NTO>
+GC_DEV void epgx_fwd_layer(EpgxRsrc rs, const EpgxLayer& Ly, bool hidden, float alpha,
+                           f32x4 (&act)[kEpgxMaxTiles][kEpgxNB], uint32_t (&msk)[kEpgxNB][64], int lane) {
+  f32x4 acc[NTO][kEpgxNB];
+#pragma unroll
+  for (int to = 0; to < NTO; ++to) {
+    const f32x4 b = tab_entry(rs, lane, Ly.bias, to);
+#pragma unroll
+    for (int k = 0; k < kEpgxNB; ++k) acc[to][k] = b;
+  }
+#pragma unroll
+  for (int ti = 0; ti < NTI; ++ti) {
+    f32x4 a[NTO];
+#pragma unroll
+    for (int to = 0; to < NTO; ++to) a[to] = tab_entry(rs, lane, Ly.fwd, to * NTI + ti);
+    // consecutive instructions go to different accumulators
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int to = 0; to < NTO; ++to)
+#pragma unroll
+        for (int k = 0; k < kEpgxNB; ++k) acc[to][k] = mma16(a[to][r], act[ti][k][r], acc[to][k]);
+  }
+  if (hidden) {
+    // leaky ReLU; padded units stay zero (zero weights and bias)
+#pragma unroll
+    for (int k = 0; k < kEpgxNB; ++k) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int to = 0; to < NTO; ++to) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float z = acc[to][k][r];
+          const bool pos = z > 0.f;
+          m |= uint32_t(pos) << (to * 4 + r);
+          act[to][k][r] = pos ? z : alpha * z;
+        }
+      }
+      msk[k][lane] = m;
+    }
+  } else {
+#pragma unroll
+    for (int to = 0; to < NTO; ++to)
+#pragma unroll
+      for (int k = 0; k < kEpgxNB; ++k) act[to][k] = acc[to][k];
+  }
+}
+
+// One reverse layer (the transpose of a forward layer with NTI input and NTO
+// output tiles): g holds d out / d (output of the layer) on entry and
+// d out / d (input of the layer) on return, through the derivative of the
+// leaky ReLU whose signs msk holds (nullptr: the network's inputs).
+template <int NTI, int NTO>
+GC_DEV void epgx_rev_layer(EpgxRsrc rs, const EpgxLayer& Ly, float alpha,
+                           f32x4 (&g)[kEpgxMaxTiles][kEpgxNB], const uint32_t (*msk)[64], int lane) {
+  f32x4 acc[NTI][kEpgxNB];
+#pragma unroll
+  for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+    for (int k = 0; k < kEpgxNB; ++k) acc[ti][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int to = 0; to < NTO; ++to) {
+    f32x4 a[NTI];
+#pragma unroll
+    for (int ti = 0; ti < NTI; ++ti) a[ti] = tab_entry(rs, lane, Ly.rev, ti * NTO + to);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+        for (int k = 0; k < kEpgxNB; ++k) acc[ti][k] = mma16(a[ti][r], g[to][k][r], acc[ti][k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kEpgxNB; ++k) {
+    const uint32_t m = msk ? msk[k][lane] : 0xffffffffu;
+#pragma unroll
+    for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = acc[ti][k][r];
+        g[ti][k][r] = ((m >> (ti * 4 + r)) & 1u) ? v : alpha * v;
+      }
+  }
+}
+
+// The layer kernels above for the layer's tile counts (uniform branches over
+// the kEpgxMaxTiles^2 shapes).
+template <int S = 0>
+GC_DEV void epgx_fwd_any(EpgxRsrc rs, const EpgxLayer& Ly, bool hidden, float alpha,
+                         f32x4 (&act)[kEpgxMaxTiles][kEpgxNB], uint32_t (&msk)[kEpgxNB][64], int lane) {
+  constexpr int NTI = S / kEpgxMaxTiles + 1, NTO = S % kEpgxMaxTiles + 1;
+  if (Ly.nti == NTI && Ly.nto == NTO)
+    epgx_fwd_layer<NTI, NTO>(rs, Ly, hidden, alpha, act, msk, lane);
+  else if constexpr (S + 1 < kEpgxMaxTiles * kEpgxMaxTiles)
+    epgx_fwd_any<S + 1>(rs, Ly, hidden, alpha, act, msk, lane);
+}
+
+template <int S = 0>
+GC_DEV void epgx_rev_any(EpgxRsrc rs, const EpgxLayer& Ly, float alpha,
+                         f32x4 (&g)[kEpgxMaxTiles][kEpgxNB], const uint32_t (*msk)[64], int lane) {
+  constexpr int NTI = S / kEpgxMaxTiles + 1, NTO = S % kEpgxMaxTiles + 1;
+  if (Ly.nti == NTI && Ly.nto == NTO)
+    epgx_rev_layer<NTI, NTO>(rs, Ly, alpha, g, msk, lane);
+  else if constexpr (S + 1 < kEpgxMaxTiles * kEpgxMaxTiles)
+    epgx_rev_any<S + 1>(rs, Ly, alpha, g, msk, lane);
+}
+
 // Forward pass of network `net` over the wavefront's column blocks: act
 // holds the input tiles on entry (features in tile 0) and the output tile on
 // return.  Hidden layers record their sign masks in L.msk.
 GC_DEV void epgx_forward(const EpgxNets& N, int net, f32x4 (&act)[kEpgxMaxTiles][kEpgxNB], EpgxLds& L, int lane) {
-  const float* __restrict__ tab = N.tab;
   const int nl = N.nl[net];
-  const float alpha = N.alpha[net];
-  for (int l = 0; l < nl; ++l) {
-    const EpgxLayer Ly = N.layer[net][l];
-    f32x4 acc[kEpgxMaxTiles][kEpgxNB];
-#pragma unroll
-    for (int to = 0; to < kEpgxMaxTiles; ++to) {
-      f32x4 b = {0.f, 0.f, 0.f, 0.f};
-      if (to < Ly.nto) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) b[r] = tab[Ly.bias + (to * 4 + r) * 64 + lane];
-      }
-#pragma unroll
-      for (int k = 0; k < kEpgxNB; ++k) acc[to][k] = b;
-    }
-#pragma unroll
-    for (int ti = 0; ti < kEpgxMaxTiles; ++ti) {
-      if (ti < Ly.nti) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-#pragma unroll
-          for (int to = 0; to < kEpgxMaxTiles; ++to) {
-            if (to < Ly.nto) {
-              const float a = tab[Ly.fwd + ((to * Ly.nti + ti) * 4 + r) * 64 + lane];
-#pragma unroll
-              for (int k = 0; k < kEpgxNB; ++k) acc[to][k] = mma16(a, act[ti][k][r], acc[to][k]);
-            }
-          }
-        }
-      }
-    }
-    if (l + 1 < nl) {
-      // leaky ReLU; padded units stay zero (zero weights and bias)
-#pragma unroll
-      for (int k = 0; k < kEpgxNB; ++k) {
-        uint32_t m = 0;
-#pragma unroll
-        for (int to = 0; to < kEpgxMaxTiles; ++to) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float z = acc[to][k][r];
-            const bool pos = z > 0.f;
-            m |= uint32_t(pos) << (to * 4 + r);
-            act[to][k][r] = pos ? z : alpha * z;
-          }
-        }
-        L.msk[l][k][lane] = m;
-      }
-    } else {
-#pragma unroll
-      for (int to = 0; to < kEpgxMaxTiles; ++to)
-#pragma unroll
-        for (int k = 0; k < kEpgxNB; ++k) act[to][k] = acc[to][k];
-    }
-  }
+  const EpgxRsrc rs = epgx_rsrc(N);
+  for (int l = 0; l < nl; ++l) epgx_fwd_any(rs, N.layer[net][l], l + 1 < nl, N.alpha[net], act, L.msk[l], lane);
 }
 
 // Reverse pass of network `net` for output o: on return g[0] holds
 // d out_o / d feature for units 4 (l/16) + r of the lane's columns.
 GC_DEV void epgx_reverse(const EpgxNets& N, int net, int o, f32x4 (&g)[kEpgxMaxTiles][kEpgxNB], const EpgxLds& L,
                          int lane) {
-  const float* __restrict__ tab = N.tab;
   const int nl = N.nl[net];
   const float alpha = N.alpha[net];
   // d out_o / d (input of the last layer) = row o of its weights, through the
-  // derivative of the leaky ReLU that produced that input
-  const EpgxLayer Lz = N.layer[net][nl - 1];
+  // derivative of the leaky ReLU that produced that input.  A whole row of
+  // kEpgxMaxTiles entries is read (the table buffer's zero tail keeps it in
+  // bounds) so that the loads need no branch.
+  const int nti = N.layer[net][nl - 1].nti;
+  const EpgxRsrc rs = epgx_rsrc(N);
+  const int32_t row = N.last[net] + o * nti * 256;
 #pragma unroll
-  for (int t = 0; t < kEpgxMaxTiles; ++t) {
+  for (int k = 0; k < kEpgxNB; ++k) {
+    const uint32_t m = (nl >= 2) ? L.msk[nl - 2][k][lane] : 0xffffffffu;
 #pragma unroll
-    for (int k = 0; k < kEpgxNB; ++k) g[t][k] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (t < Lz.nti) {
+    for (int t = 0; t < kEpgxMaxTiles; ++t) {
+      const f32x4 w = tab_entry(rs, lane, row, t);
 #pragma unroll
-      for (int k = 0; k < kEpgxNB; ++k) {
-        const uint32_t m = (nl >= 2) ? L.msk[nl - 2][k][lane] : 0xffffffffu;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float w = tab[N.last[net] + ((o * Lz.nti + t) * 4 + r) * 64 + lane];
-          g[t][k][r] = ((m >> (t * 4 + r)) & 1u) ? w : alpha * w;
-        }
-      }
+      for (int r = 0; r < 4; ++r)
+        g[t][k][r] = t < nti ? (((m >> (t * 4 + r)) & 1u) ? w[r] : alpha * w[r]) : 0.f;
     }
   }
-  for (int l = nl - 2; l >= 0; --l) {
-    const EpgxLayer Ly = N.layer[net][l];
-    f32x4 acc[kEpgxMaxTiles][kEpgxNB];
-#pragma unroll
-    for (int ti = 0; ti < kEpgxMaxTiles; ++ti)
-#pragma unroll
-      for (int k = 0; k < kEpgxNB; ++k) acc[ti][k] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int to = 0; to < kEpgxMaxTiles; ++to) {
-      if (to < Ly.nto) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-#pragma unroll
-          for (int ti = 0; ti < kEpgxMaxTiles; ++ti) {
-            if (ti < Ly.nti) {
-              const float a = tab[Ly.rev + ((ti * Ly.nto + to) * 4 + r) * 64 + lane];
-#pragma unroll
-              for (int k = 0; k < kEpgxNB; ++k) acc[ti][k] = mma16(a, g[to][k][r], acc[ti][k]);
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int ti = 0; ti < kEpgxMaxTiles; ++ti) {
-#pragma unroll
-      for (int k = 0; k < kEpgxNB; ++k) {
-        const uint32_t m = (l >= 1) ? L.msk[l - 1][k][lane] : 0xffffffffu;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float v = acc[ti][k][r];
-          g[ti][k][r] = ((m >> (ti * 4 + r)) & 1u) ? v : alpha * v;
-        }
-      }
-    }
-  }
+  for (int l = nl - 2; l >= 0; --l) epgx_rev_any(rs, N.layer[net][l], alpha, g, l >= 1 ? L.msk[l - 1] : nullptr, lane);
 }
 
 // MVF of MCR-MWI's parameter slots (S0, MWF, IWF, ...), as McrmwiModel::amplitudes
@@ -308,7 +350,7 @@ GC_DEV void epgx_wave(const gc_model& md, const EpgxNets& N, const float* __rest
 // Workgroup b of volume i (counted from kEpgxGroupsPerBlock blk0,
 // kEpgxSamples voxels each) has one wavefront per flip angle.
 template <int TR>
-__global__ __launch_bounds__(kBlock) void k_epgx_step(const gc_model* __restrict__ mdp, const StepParams sp,
+__global__ __launch_bounds__(kBlock, 3) void k_epgx_step(const gc_model* __restrict__ mdp, const StepParams sp,
                                                       const EpgxNets N) {
   int vi = 0;
   for (int i = 1; i < sp.n_vol; ++i)
@@ -330,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_epgx_step(const gc_model* __restrict
 
 // the one-shot form (gc_fwd, gc_fwd_adj, the askadam outputs): aux rows
 // from parameters x in the given transform space
-__global__ __launch_bounds__(kBlock) static void k_epgx_eval(const gc_model* __restrict__ mdp,
+__global__ __launch_bounds__(kBlock, 3) static void k_epgx_eval(const gc_model* __restrict__ mdp,
                                                              const float* __restrict__ x,
                                                              const float* __restrict__ ex, int64_t nv,
                                                              int32_t transform, float* __restrict__ aux,

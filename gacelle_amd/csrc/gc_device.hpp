@@ -10,6 +10,12 @@ namespace gc {
 
 constexpr float kPiF = 3.14159265358979323846f;
 
+// Device global memory, said so: loads through such a pointer compile to
+// global_load (a plain pointer read from a descriptor in memory gives
+// flat_load, which also waits on the LDS counter).
+using gfloat = __attribute__((address_space(1))) const float;
+GC_DEV gfloat* as_global(const float* p) { return (gfloat*)p; }
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11), from the paper's round
 // function: two 32x32->64 multiplies per round, Weyl key schedule.

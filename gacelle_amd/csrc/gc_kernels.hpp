@@ -514,6 +514,78 @@ struct step_waves {
   static constexpr int value = f<Model>(nullptr);
 };
 
+// measurement streams of a model's loss calls (MeasAhead); 0: the model
+// declares no call order
+template <class Model>
+struct meas_streams {
+  template <class M>
+  static constexpr int f(decltype(M::kMeasStreams)*) { return M::kMeasStreams; }
+  template <class M>
+  static constexpr int f(...) { return 0; }
+  static constexpr int value = f<Model>(nullptr);
+};
+
+// The step kernel's reads of one voxel's measurements (and weights).  A
+// model that declares kMeasStreams = S calls the loss in a fixed order: S
+// streams of len = n_meas / S calls (stream c: c len + k, k ascending),
+// interleaved call by call (k of every stream, then k + 1, ...).  The reader
+// then keeps the values of the next kAhead calls in flight, with no branch:
+// each call takes the oldest and loads the one kAhead calls later (kAhead /
+// S steps further in the same stream).  Models without a declared order
+// read each value when called.
+template <class Model, bool HAS_W, int S = meas_streams<Model>::value>
+struct MeasAhead {
+  static constexpr int kAhead = 4;
+  static_assert(S == 1 || S == 2, "one or two interleaved measurement streams");
+  const gc_model& md;
+  gfloat* y;
+  gfloat* w;
+  int64_t nv, vv;
+  int32_t n;
+  float yq[kAhead], wq[kAhead];
+
+  GC_DEV MeasAhead(const gc_model& m, const float* yp, const float* wp, int64_t nv_, int64_t vv_)
+      : md(m), y(as_global(yp)), w(as_global(wp)), nv(nv_), vv(vv_), n(m.n_meas) {
+    const int32_t len = n / S;
+#pragma unroll
+    for (int i = 0; i < kAhead; ++i) {
+      const int32_t r = (i % S) * len + i / S;  // the i-th call
+      yq[i] = ld(y, r);
+      if constexpr (HAS_W) wq[i] = ld(w, r);
+    }
+  }
+  GC_DEV float ld(gfloat* p, int32_t r) const {
+    r = r < n ? r : n - 1;  // reads past the last call stay in the array
+    return p[int64_t(Model::meas(md, r)) * nv + vv];
+  }
+  GC_DEV void get(int32_t r, float& yv, float& wv) {
+    yv = yq[0];
+    wv = HAS_W ? wq[0] : 1.f;
+#pragma unroll
+    for (int i = 0; i + 1 < kAhead; ++i) {
+      yq[i] = yq[i + 1];
+      if constexpr (HAS_W) wq[i] = wq[i + 1];
+    }
+    yq[kAhead - 1] = ld(y, r + kAhead / S);
+    if constexpr (HAS_W) wq[kAhead - 1] = ld(w, r + kAhead / S);
+  }
+};
+
+template <class Model, bool HAS_W>
+struct MeasAhead<Model, HAS_W, 0> {
+  const gc_model& md;
+  gfloat* y;
+  gfloat* w;
+  int64_t nv, vv;
+  GC_DEV MeasAhead(const gc_model& m, const float* yp, const float* wp, int64_t nv_, int64_t vv_)
+      : md(m), y(as_global(yp)), w(as_global(wp)), nv(nv_), vv(vv_) {}
+  GC_DEV void get(int32_t r, float& yv, float& wv) {
+    const int64_t m = Model::meas(md, r);
+    yv = y[m * nv + vv];
+    wv = HAS_W ? w[m * nv + vv] : 1.f;
+  }
+};
+
 template <class Model, int LOSS, bool HAS_W, int TR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(step_waves<Model>::value)))
 void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
@@ -530,17 +602,20 @@ void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
   const bool valid = v < nv;
   const int64_t vv = valid ? v : (nv - 1);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the first measurements are read while the parameters are prepared
+  MeasAhead<Model, HAS_W> meas(md, V.y, V.w, nv, vv);
 
   float ex[kMaxExtra];
 #pragma unroll
   for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
-  float z[NP], m1[NP], m2[NP], th[NP], dth[NP];
+  // only theta lives through the model evaluation: z is read again after it
+  // and the transform's derivative formed again from it (the same
+  // arithmetic, so the same bits), and the optimiser state is read there
+  float th[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
-    z[p] = V.z_in[int64_t(p) * nv + vv];
-    m1[p] = V.m_in[int64_t(p) * nv + vv];
-    m2[p] = V.v_in[int64_t(p) * nv + vv];
-    transform_fwd_t<TR>(z[p], md.lb[p], md.ub[p], th[p], dth[p]);
+    float d;
+    transform_fwd_t<TR>(V.z_in[int64_t(p) * nv + vv], md.lb[p], md.ub[p], th[p], d);
   }
   float dead_z2 = 0.f;
   for (int p = NP; p < md.n_par; ++p) {
@@ -557,19 +632,28 @@ void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
   model_adjoint<Model>(
       md, th, ex,
       [&](int r, float s) -> float {
-        const int64_t m = Model::meas(md, r);
+        float yv, wr;
+        meas.get(r, yv, wr);
         const bool fin = isfinite(s);
         const float sv = fin ? s : 0.f;
-        const float wv = (HAS_W ? V.w[m * nv + vv] : 1.f) * wm;
+        const float wv = wr * wm;
         float l, dl;
-        loss_fn<LOSS>(sv - V.y[m * nv + vv], wv, l, dl);
+        loss_fn<LOSS>(sv - yv, wv, l, dl);
         lsum += l;
         return fin ? dl : 0.f;
       },
       gth, AuxRef{V.aux ? V.aux + vv : nullptr, nv});
-  float g[NP];
+  // a compiler-only barrier: without it the loads below would be merged with
+  // the ones above and z held in registers through the model
+  asm volatile("" ::: "memory");
+  float z[NP], g[NP], dth[NP];
 #pragma unroll
-  for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * V.inv_count;
+  for (int p = 0; p < NP; ++p) {
+    z[p] = V.z_in[int64_t(p) * nv + vv];
+    float t;
+    transform_fwd_t<TR>(z[p], md.lb[p], md.ub[p], t, dth[p]);
+    g[p] = gth[p] * dth[p] * V.inv_count;
+  }
   const float pvl = lsum / float(md.n_meas);
   if (V.pvl_out != nullptr && valid) V.pvl_out[v] = pvl;
   float msum = (V.mconv != nullptr && V.mconv[vv]) ? pvl : 0.f;
@@ -654,6 +738,12 @@ void k_askadam_step(const gc_model* __restrict__ mdp, const StepParams sp) {
   if (V.z_last != nullptr && valid) {
 #pragma unroll
     for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
+  }
+  float m1[NP], m2[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    m1[p] = V.m_in[int64_t(p) * nv + vv];
+    m2[p] = V.v_in[int64_t(p) * nv + vv];
   }
   if (V.update) {
 #pragma unroll

@@ -410,7 +410,7 @@ GC_DEV DualF<N> dmax(const DualF<N>& a, float c) {
 struct AuxRef {
   const float* p = nullptr;
   int64_t stride = 0;
-  GC_DEV float operator[](int k) const { return p[int64_t(k) * stride]; }
+  GC_DEV float operator[](int k) const { return as_global(p)[int64_t(k) * stride]; }
 };
 
 // EPG-X path (EPG, needs EXCH; MCRMWI/gpuMCRMWI.m:664-681): the MLP
@@ -432,6 +432,11 @@ struct McrmwiModel {
   static constexpr int NE = 3 + MF;
   static constexpr bool kRelJ = false;
   static constexpr bool kFusedAdj = true, kStreamOnly = true;
+  // The order of eval_adj's loss calls, which the step kernel reads ahead
+  // by: flip angle by flip angle, echo by echo, the real part (call
+  // t + nte fi) then, with complex data, the imaginary part (call
+  // t + nte fi + nte nfa); every call made, none skipped.
+  static constexpr int kMeasStreams = CPLX ? 2 : 1;
   using D = DualF<NA>;
   GC_DEV static void post_grad(const float*, float*) {}
   GC_DEV static bool valid(const gc_model& md, int r) { return r < md.n_meas; }
@@ -530,9 +535,17 @@ struct McrmwiModel {
     const float freq[3] = {th[NA + 3], th[NA + 4], freqEW};
     float fbkg = 0.f, pini = 0.f;
     if constexpr (CPLX) {
-      fbkg = th[NA + 5 + fi] + ex[3 + fi];
-      // one initial phase per flip angle (rows 3+MF.., n_fa == MF) or one per voxel
-      pini = th[NA + 5 + MF] + ((md.flags & GC_FLAG_MCRMWI_PINI_FA) ? ex[3 + MF + fi] : ex[1]);
+      // one initial phase per flip angle (rows 3+MF.., n_fa == MF) or one per
+      // voxel; uniform selects over the MF slots keep the indices static
+      const bool pfa = (md.flags & GC_FLAG_MCRMWI_PINI_FA) != 0;
+      float pex = ex[1];
+#pragma unroll
+      for (int j = 0; j < MF; ++j)
+        if (j == fi) {
+          fbkg = th[NA + 5 + j] + ex[3 + j];
+          if (pfa) pex = ex[3 + MF + j];
+        }
+      pini = th[NA + 5 + MF] + pex;
     }
     const bool uni = (md.flags & GC_FLAG_UNIFORM_TE) != 0;
     constexpr float k2pi = 6.283185307179586f;
@@ -548,6 +561,9 @@ struct McrmwiModel {
         sincosf(dte * w[p], &qs[p], &qc[p]);
       }
     }
+    // two echoes per trip: the step kernel's read-ahead queue (two values
+    // per stream) then returns to the same registers every trip
+#pragma unroll 2
     for (int t = 0; t < nte; ++t) {
       const float te = md.table[t];
       if (!uni) {
@@ -649,7 +665,12 @@ struct McrmwiModel {
       for (int q = 0; q < NA; ++q)
         g[q] += gamp[0] * aMW.d[q] + gamp[1] * aIW.d[q] + gamp[2] * aEW.d[q] + (EPG ? gph * ph.d[q] : 0.f);
       if constexpr (EPG) g[NA + 1] += gamp[0] * r2[0] + gamp[1] * r2[1] + gamp[2] * r2[2] + gph * r2[3];  // R2sIW
-      if constexpr (CPLX) g[NA + 5 + fi] += gbkg;
+      if constexpr (CPLX) {
+        // (a uniform select over the MF slots: a runtime index would put g in scratch)
+#pragma unroll
+        for (int j = 0; j < MF; ++j)
+          if (j == fi) g[NA + 5 + j] += gbkg;
+      }
     }
   }
 };

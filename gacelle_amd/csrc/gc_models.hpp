@@ -522,10 +522,10 @@ struct McrmwiModel {
     aEW = aIEW * (1.f - iwf);
   }
 
-  // Visit the echoes of flip angle fi with each pool's decay e_p =
-  // exp(-te r2s_p) and phase (cos, sin); g(t, te, e[3], c[3], s[3]).  With
-  // equally spaced echoes both are advanced by recurrence (one multiply
-  // and one complex rotation per pool and echo).
+  // Visit the echoes of flip angle fi with each pool's complex factor
+  // w_p = exp(-te r2s_p) (cos + i sin)(phase_p(te)); g(t, te, wr[3], wi[3]).
+  // With equally spaced echoes w_p is advanced by recurrence (one complex
+  // multiply per pool and echo).
   template <class G>
   GC_DEV static void per_echo(const gc_model& md, const float* th, const float* ex, int fi, G&& g, float ph = 0.f) {
     const int nte = md.i[0];
@@ -547,42 +547,51 @@ struct McrmwiModel {
         }
       pini = th[NA + 5 + MF] + pex;
     }
-    const bool uni = (md.flags & GC_FLAG_UNIFORM_TE) != 0;
     constexpr float k2pi = 6.283185307179586f;
-    float w[3], e[3], cs[3], sn[3], qe[3], qc[3], qs[3];
-    const float te0 = md.table[0], dte = md.f[8];
+    float om[3], wr[3], wi[3];
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      w[p] = k2pi * (freq[p] + fbkg) * B0g;
-      if (uni) {
-        e[p] = __expf(-te0 * r2s[p]);
-        sincosf(te0 * w[p] + pini + (p > 0 ? ph : 0.f), &sn[p], &cs[p]);
-        qe[p] = __expf(-dte * r2s[p]);
-        sincosf(dte * w[p], &qs[p], &qc[p]);
+    for (int p = 0; p < 3; ++p) om[p] = k2pi * (freq[p] + fbkg) * B0g;
+    if (md.flags & GC_FLAG_UNIFORM_TE) {
+      // a loop of its own: two echoes per trip, so the step kernel's
+      // read-ahead queue (two values per stream) returns to the same
+      // registers every trip
+      const float te0 = md.table[0], dte = md.f[8];
+      float qr[3], qi[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        float e, c, sn;
+        e = __expf(-te0 * r2s[p]);
+        sincosf(te0 * om[p] + pini + (p > 0 ? ph : 0.f), &sn, &c);
+        wr[p] = e * c;
+        wi[p] = e * sn;
+        e = __expf(-dte * r2s[p]);
+        sincosf(dte * om[p], &sn, &c);
+        qr[p] = e * c;
+        qi[p] = e * sn;
       }
-    }
-    // two echoes per trip: the step kernel's read-ahead queue (two values
-    // per stream) then returns to the same registers every trip
 #pragma unroll 2
-    for (int t = 0; t < nte; ++t) {
-      const float te = md.table[t];
-      if (!uni) {
+      for (int t = 0; t < nte; ++t) {
+        g(t, md.table[t], wr, wi);
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-          e[p] = __expf(-te * r2s[p]);
-          sincosf(te * w[p] + pini + (p > 0 ? ph : 0.f), &sn[p], &cs[p]);
+          const float r2 = wr[p] * qr[p] - wi[p] * qi[p];
+          const float i2 = wr[p] * qi[p] + wi[p] * qr[p];
+          wr[p] = r2;
+          wi[p] = i2;
         }
       }
-      g(t, te, e, cs, sn);
-      if (uni) {
+    } else {
+      for (int t = 0; t < nte; ++t) {
+        const float te = md.table[t];
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-          e[p] *= qe[p];
-          const float c2 = cs[p] * qc[p] - sn[p] * qs[p];
-          const float s2 = sn[p] * qc[p] + cs[p] * qs[p];
-          cs[p] = c2;
-          sn[p] = s2;
+          float c, sn;
+          const float e = __expf(-te * r2s[p]);
+          sincosf(te * om[p] + pini + (p > 0 ? ph : 0.f), &sn, &c);
+          wr[p] = e * c;
+          wi[p] = e * sn;
         }
+        g(t, te, wr, wi);
       }
     }
   }
@@ -596,11 +605,11 @@ struct McrmwiModel {
       D aMW, aIW, aEW, ph = D::cst(0.f);
       float r2[4];
       amplitudes(md, th, ex, fi, aMW, aIW, aEW, aux, &ph, r2);
-      const float amp[3] = {aMW.v, aIW.v, aEW.v};
-      per_echo(md, th, ex, fi, [&](int t, float, const float* e, const float* c, const float* s) {
-        const float d[3] = {amp[0] * e[0], amp[1] * e[1], amp[2] * e[2]};
-        const float re = (d[0] * c[0] + d[1] * c[1] + d[2] * c[2]) * ff;
-        const float im = (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) * ff;
+      // the per-voxel scale folded into the amplitudes
+      const float A[3] = {ff * aMW.v, ff * aIW.v, ff * aEW.v};
+      per_echo(md, th, ex, fi, [&](int t, float, const float* wr, const float* wi) {
+        const float re = A[0] * wr[0] + A[1] * wr[1] + A[2] * wr[2];
+        const float im = A[0] * wi[0] + A[1] * wi[1] + A[2] * wi[2];
         if constexpr (CPLX) {
           f(t + nte * fi, re, nullptr);
           f(t + nte * fi + nte * nfa, im, nullptr);
@@ -613,6 +622,12 @@ struct McrmwiModel {
 
   // dloss(r, s) returns d loss / d s_r (0 when the measurement carries no
   // gradient); g[NP] accumulates d loss / d theta.
+  //
+  // With S = sum_p A_p w_p (A_p = ff a_p) and R = dre + i dim the loss
+  // gradient at one echo: d/dA_p = Re(conj(R) w_p) =: u_p, d/dr2s_p =
+  // -te A_p u_p, and d/dphase_p = A_p Im(conj(R) w_p) =: A_p v_p.  The echo
+  // loop accumulates only the amplitude-free sums of u_p, te u_p, v_p and
+  // te v_p; the amplitudes multiply them once per flip angle.
   template <class F>
   GC_DEV static void eval_adj(const gc_model& md, const float* th, const float* ex, F&& dloss, float* g,
                               AuxRef aux = {}) {
@@ -624,14 +639,14 @@ struct McrmwiModel {
       D aMW, aIW, aEW, ph = D::cst(0.f);
       float r2[4] = {0.f, 0.f, 0.f, 0.f};
       amplitudes(md, th, ex, fi, aMW, aIW, aEW, aux, &ph, r2);
-      const float amp[3] = {aMW.v, aIW.v, aEW.v};
-      float gamp[3] = {0.f, 0.f, 0.f};  // d loss / d a_p at this flip angle
-      float gbkg = 0.f;                  // d loss / d (phase slope common to all pools) / w0
-      float gph = 0.f;                   // d loss / d (IEW phase)
-      per_echo(md, th, ex, fi, [&](int t, float te, const float* e, const float* c, const float* s) {
-        const float d[3] = {amp[0] * e[0], amp[1] * e[1], amp[2] * e[2]};
-        const float re = (d[0] * c[0] + d[1] * c[1] + d[2] * c[2]) * ff;
-        const float im = (d[0] * s[0] + d[1] * s[1] + d[2] * s[2]) * ff;
+      const float A[3] = {ff * aMW.v, ff * aIW.v, ff * aEW.v};
+      float su[3] = {0.f, 0.f, 0.f}, stu[3] = {0.f, 0.f, 0.f};  // sum u_p, sum te u_p
+      float stv[3] = {0.f, 0.f, 0.f};                           // sum te v_p
+      // sum A_p v_p over all pools (pini) and over IW and EW (the EPG phase)
+      float sav = 0.f, sav12 = 0.f;
+      per_echo(md, th, ex, fi, [&](int t, float te, const float* wr, const float* wi) {
+        const float re = A[0] * wr[0] + A[1] * wr[1] + A[2] * wr[2];
+        const float im = A[0] * wi[0] + A[1] * wi[1] + A[2] * wi[2];
         float dre, dim;
         if constexpr (CPLX) {
           dre = dloss(t + nte * fi, re);
@@ -643,33 +658,46 @@ struct McrmwiModel {
           dre = dm * re * im_;
           dim = dm * im * im_;
         }
-        if (dre == 0.f && dim == 0.f) return;
-        const float w0 = k2pi * te * B0g;
-        float vsum = 0.f;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-          const float u = ff * (dre * c[p] + dim * s[p]);   // d loss / d d_p
-          const float v = ff * d[p] * (dim * c[p] - dre * s[p]);  // d loss / d phase_p
-          gamp[p] += u * e[p];
-          g[NA + p] += u * d[p] * (-te);                    // R2s of pool p
-          if (p < 2) g[NA + 3 + p] += v * w0;              // freqMW, freqIW
-          if (EPG && p > 0) gph += v;
-          vsum += v;
-        }
-        if constexpr (CPLX) {
-          gbkg += vsum * w0;
-          g[NA + 5 + MF] += vsum;                           // dpini
+          const float u = dre * wr[p] + dim * wi[p];
+          const float v = dim * wr[p] - dre * wi[p];
+          su[p] += u;
+          stu[p] += te * u;
+          stv[p] += te * v;
+          if (EPG && p > 0)
+            sav12 += A[p] * v;
+          else
+            sav += A[p] * v;
         }
       }, ph.v);
+      // a non-finite amplitude made the signal non-finite at every echo, so
+      // every loss derivative was zero and the pool adds nothing (the sums
+      // weighted by A_p inside the loop are then NaN: they are dropped)
+      const float w0 = k2pi * B0g;
+      const bool allfin = isfinite(A[0]) && isfinite(A[1]) && isfinite(A[2]);
+      float gamp[3], gbkg = 0.f;
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const bool fin = isfinite(A[p]);
+        const float Ap = fin ? A[p] : 0.f;
+        gamp[p] = fin ? ff * su[p] : 0.f;                 // d loss / d a_p
+        g[NA + p] -= Ap * stu[p];                          // R2s of pool p
+        if (p < 2) g[NA + 3 + p] += Ap * w0 * stv[p];      // freqMW, freqIW
+        gbkg += Ap * stv[p];
+      }
+      const float gph = (EPG && allfin) ? sav12 : 0.f;     // IEW phase
+      const float gpini = allfin ? sav + sav12 : 0.f;
 #pragma unroll
       for (int q = 0; q < NA; ++q)
         g[q] += gamp[0] * aMW.d[q] + gamp[1] * aIW.d[q] + gamp[2] * aEW.d[q] + (EPG ? gph * ph.d[q] : 0.f);
       if constexpr (EPG) g[NA + 1] += gamp[0] * r2[0] + gamp[1] * r2[1] + gamp[2] * r2[2] + gph * r2[3];  // R2sIW
       if constexpr (CPLX) {
+        g[NA + 5 + MF] += gpini;                           // dpini
         // (a uniform select over the MF slots: a runtime index would put g in scratch)
 #pragma unroll
         for (int j = 0; j < MF; ++j)
-          if (j == fi) g[NA + 5 + j] += gbkg;
+          if (j == fi) g[NA + 5 + j] += w0 * gbkg;
       }
     }
   }

@@ -10,7 +10,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_uint8, c_uint32, c_uint64, c_void_p
 
-GC_ABI_VERSION = 8
+GC_ABI_VERSION = 9
 GC_MAX_PAR = 24
 GC_MAX_TABLE = 256
 GC_MAX_VOL = 16
@@ -60,7 +60,7 @@ GC_STOP_NONE, GC_STOP_CONVERGENCE, GC_STOP_TOL, GC_STOP_STEP, GC_STOP_GRAD = 0, 
 GC_MCMC_MH, GC_MCMC_GW = 0, 1
 GC_ENSEMBLE_SIMULTANEOUS, GC_ENSEMBLE_REDBLACK = 0, 1
 GC_RNG_NATIVE, GC_RNG_REPLAY = 0, 1
-GC_OPT_GRAPHS, GC_OPT_MCMC_BLOCK = 1, 2
+GC_OPT_GRAPHS, GC_OPT_MCMC_BLOCK, GC_OPT_ASK_BLOCK = 1, 2, 3
 
 STOP_MESSAGES = {
     GC_STOP_NONE: "",
@@ -178,6 +178,7 @@ class gc_volume(ctypes.Structure):
         ("reserved1", c_int32),
         ("n_mean", c_int64),
         ("sign_trace_dev", c_void_p),
+        ("block_threads", c_int32),
     ]
 
 

@@ -173,6 +173,11 @@ int gc_ctx_set_option(gc_ctx* ctx, int32_t option, int64_t value) {
         return set_err(ctx, GC_ERR_ARG, "GC_OPT_MCMC_BLOCK takes 0 (autotune), 64, 128 or 256");
       ctx->mcmc_block = int(value);
       return GC_OK;
+    case GC_OPT_ASK_BLOCK:
+      if (value != 0 && value != 64 && value != 128 && value != 256)
+        return set_err(ctx, GC_ERR_ARG, "GC_OPT_ASK_BLOCK takes 0 (autotune), 64, 128 or 256");
+      ctx->ask_block = int(value);
+      return GC_OK;
     default:
       return set_err(ctx, GC_ERR_ARG, "unknown context option %d", option);
   }

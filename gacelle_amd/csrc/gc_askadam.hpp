@@ -11,6 +11,8 @@
 // through HBM): it carries the spatial TV regulariser, whose stencil needs
 // the neighbours' previous parameters, and models whose data are streamed.
 #pragma once
+#include <chrono>
+
 #include "gc_host.hpp"
 
 namespace gc {
@@ -19,7 +21,8 @@ namespace gc {
 struct VolRun {
   int64_t nv = 0;
   double nmean = 0;     // N of the loss mean (gc_volume.n_mean, else nv)
-  int nblk = 0;
+  int nblk = 0;      // blocks of kBlock voxels (the volume's padding)
+  int nprod = 0;     // partial-sum rows: wavefronts (blocked form) or blocks (step form)
   DevBuf set[3][3];  // [state set][z, m, v]
   DevBuf zlast[2];
   DevBuf partials;
@@ -86,6 +89,12 @@ struct AskDriver {
   int n_vol;
   int K = 0;
   int total_blk = 0;
+  int blk_threads = kBlock;     // threads per block of the blocked kernel
+  double tune_launch_ms = -1.0;  // measured cost of one launch cycle (ms)
+  double tune_vox_ms = 0.0;     // measured kernel time per voxel-iteration (ms)
+  // times the blocked kernel on the first call for a model variant
+  // (tune_blocked, bound to the model by the operator table)
+  int (*tuner)(AskDriver&) = nullptr;
   const gc_model* dmd = nullptr;
   bool has_w = false;
   std::vector<VolRun> run;
@@ -234,20 +243,35 @@ struct AskDriver {
       nv_total += vols[i].nv;
       if (vols[i].w_dev) has_w = true;
     }
+    if (!step_mode && tuner) {
+      const int rct = tuner(*this);
+      if (rct) return rct;
+    }
     // Iterations per launch.  When a stop can land inside a block, blocks of
     // ~2.5e8 voxel-iterations (at most 256 iterations) bound the replay that
     // resolves it.  Without stopping criteria the host has nothing to decide
     // between blocks: as many iterations per launch as the block partials
     // (4 bytes per block, iteration and sum) allow within 1 GiB.
+    //
+    // With a measured launch cost c (ms per launch cycle) and iteration time
+    // t (ms per iteration of this call), blocks of K iterations cost about
+    // (N / K) c in launches over N iterations, and a stop inside a block
+    // about 1.5 K t (the rest of the block and the replay to the stop), so K
+    // = sqrt(N c / (1.5 t)), within 16 .. 256.
+    const int rows_per_blk = step_mode ? 1 : kBlock / 64;
     int64_t nblk_total = 0;
     for (int i = 0; i < n_vol; ++i) nblk_total += (vols[i].nv + kBlock - 1) / kBlock;
-    const int64_t row_bytes = 4 * int64_t(npv_count());
+    const int64_t row_bytes = 4 * int64_t(npv_count()) * rows_per_blk;
     const int k_mem =
         int(std::max<int64_t>(16, (int64_t(1) << 30) / (row_bytes * std::max<int64_t>(nblk_total, 1))));
     if (cfg.block_iters > 0) K = cfg.block_iters;
     else if (stopping_possible()) {
-      const double k = 2.5e8 / double(std::max<int64_t>(nv_total, 1));
-      K = int(std::min(256.0, std::max(16.0, k)));
+      double k = 2.5e8 / double(std::max<int64_t>(nv_total, 1));
+      if (!step_mode && tune_launch_ms > 0 && tune_vox_ms > 0) {
+        const double t = tune_vox_ms * double(std::max<int64_t>(nv_total, 1));
+        k = std::sqrt(double(std::max(cfg.iteration, 1)) * tune_launch_ms / (1.5 * t));
+      }
+      K = int(std::lround(std::min(256.0, std::max(16.0, k))));
     } else {
       K = std::min(k_mem, 8192);
     }
@@ -270,11 +294,12 @@ struct AskDriver {
         return set_err(ctx, GC_ERR_UNSUPPORTED, "volume %d: the sign trace needs the L1 loss and the blocked form", i);
       R.reg_offset = prior ? vols[i].reg_offset : 0.0;
       R.nblk = int((R.nv + kBlock - 1) / kBlock);
+      R.nprod = R.nblk * rows_per_blk;
       const size_t pbytes = size_t(P) * R.nv * sizeof(float);
       for (int s = 0; s < 3; ++s)
         for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.set[s][j].alloc(pbytes));
       for (int s = 0; s < 2; ++s) GC_HIP(ctx, R.zlast[s].alloc(pbytes));
-      GC_HIP(ctx, R.partials.alloc(size_t(K) * npv_count() * R.nblk * sizeof(float)));
+      GC_HIP(ctx, R.partials.alloc(size_t(K) * npv_count() * R.nprod * sizeof(float)));
       if (step_mode)
         for (int s = 0; s < 2; ++s)
           for (int j = 0; j < 3; ++j) GC_HIP(ctx, R.pp[s][j].alloc(pbytes));
@@ -314,7 +339,7 @@ struct AskDriver {
         hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, R.ones.as<float>(), n, 1.0f);
       }
       R.cbuf.assign(std::max(cfg.convergence_window, 1), 1.0);
-      R.chunks = int((R.nblk + kReduceChunk - 1) / kReduceChunk);
+      R.chunks = int((R.nprod + kReduceChunk - 1) / kReduceChunk);
       max_chunks = std::max(max_chunks, R.chunks);
       total_blk += R.nblk;
     }
@@ -496,7 +521,7 @@ struct AskDriver {
       if (n_ep[i] <= 0) continue;
       const int nrows = n_ep[i] * npv_count();
       hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, (nrows + 63) / 64), dim3(kBlock), 0, ctx->stream,
-                         run[i].partials.as<float>(), int64_t(run[i].nblk), nrows, int64_t(kReduceChunk),
+                         run[i].partials.as<float>(), int64_t(run[i].nprod), nrows, int64_t(kReduceChunk),
                          d_sums.as<double>() + size_t(i) * K * kSum * max_chunks);
       GC_HIP(ctx, hipGetLastError());
     }
@@ -569,20 +594,169 @@ struct AskDriver {
     int rce = next_events(ev_a, ev_b);
     if (rce) return rce;
     // events attached to the dispatch itself time the kernel alone
-    if constexpr (LOSS == GC_LOSS_L1) {
-      if (want_signs)
-        hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W, true>), dim3(total_blk), dim3(kBlock), 0, ctx->stream,
-                              ev_a, ev_b, 0, dmd, ap);
-      else
-        hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
-                              ev_b, 0, dmd, ap);
-    } else {
-      hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), dim3(total_blk), dim3(kBlock), 0, ctx->stream, ev_a,
-                            ev_b, 0, dmd, ap);
-    }
-    GC_HIP(ctx, hipGetLastError());
+    const int rcl = launch_fused<Model, LOSS, HAS_W>(want_signs, total_blk, blk_threads, ap, ev_a, ev_b);
+    if (rcl) return rcl;
     ++kernel_launches;
     return reduce_partials(n_ep);
+  }
+
+  // k_askadam over nblk256 blocks of kBlock voxel slots, in blocks of bs
+  // threads (64, 128 or 256), with the wavefronts' LDS rings when only the
+  // loss is summed
+  template <class Model, int LOSS, bool HAS_W>
+  int launch_fused(bool signs, int nblk256, int bs, const AskParams& ap, hipEvent_t ev_a, hipEvent_t ev_b) {
+    if (bs != 64 && bs != 128 && bs != 256) return set_err(ctx, GC_ERR_ARG, "askadam block of %d threads", bs);
+    const dim3 grid(unsigned(int64_t(nblk256) * (kBlock / bs)));
+    const size_t lds = ap.need_norms ? 0 : size_t(bs / 64) * kRing * 64 * sizeof(float);
+    if constexpr (LOSS == GC_LOSS_L1) {
+      if (signs)
+        hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W, true>), grid, dim3(bs), lds, ctx->stream, ev_a, ev_b, 0,
+                              dmd, ap);
+      else
+        hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), grid, dim3(bs), lds, ctx->stream, ev_a, ev_b, 0, dmd,
+                              ap);
+    } else {
+      (void)signs;
+      hipExtLaunchKernelGGL((k_askadam<Model, LOSS, HAS_W>), grid, dim3(bs), lds, ctx->stream, ev_a, ev_b, 0, dmd,
+                            ap);
+    }
+    GC_HIP(ctx, hipGetLastError());
+    return GC_OK;
+  }
+
+  // Autotuner of the blocked kernel (run once per model variant and
+  // context, before the call's buffers exist): times k_askadam at each block
+  // size on the first min(nv, 2^20) voxels of volume 0, reading the call's
+  // data and starting point and writing only buffers of its own, and the
+  // fixed cost of one launch cycle (launch, k_reduce_rows, copy of the sums
+  // to the host, synchronisation).  Strides by the trial's voxel count keep
+  // every read inside the call's arrays.  Block size and K change no result:
+  // every wavefront sums its own 64 voxels, and the host replays any block.
+  template <class Model, int LOSS, bool HAS_W>
+  int tune_blocked() {
+    if constexpr (Model::kStreamOnly) {
+      return GC_OK;
+    } else {
+      const int npv = npv_count();
+      const std::array<int64_t, 8> key{md.model_id, md.flags, md.n_meas, md.n_par,
+                                       LOSS,        HAS_W ? 1 : 0, npv, cfg.transform};
+      const int fixed = ctx->ask_block;
+      auto it = ctx->ask_tuned.find(key);
+      if (it != ctx->ask_tuned.end()) {
+        blk_threads = fixed ? fixed : it->second.threads;
+        tune_launch_ms = it->second.launch_ms;
+        tune_vox_ms = it->second.ms_per_voxel_iter;
+        return GC_OK;
+      }
+      constexpr int kTrialMax = 64;  // iterations of the longest trial launch
+      const int64_t nvt = std::min<int64_t>(vols[0].nv, int64_t(1) << 20);
+      const int nbt = int((nvt + kBlock - 1) / kBlock);
+      const int P = md.n_par;
+      const size_t pbytes = sizeof(float) * size_t(P) * size_t(nvt);
+      DevBuf zt, mt, vt, pt, est, dvt, dsum;
+      GC_HIP(ctx, zt.alloc(pbytes));
+      GC_HIP(ctx, mt.alloc(pbytes));
+      GC_HIP(ctx, vt.alloc(pbytes));
+      GC_HIP(ctx, pt.alloc(sizeof(float) * size_t(kTrialMax) * npv * nbt * (kBlock / 64)));
+      GC_HIP(ctx, est.alloc(sizeof(EpochScal) * kTrialMax));
+      GC_HIP(ctx, dvt.alloc(sizeof(VolDev)));
+      const int chunks = int((int64_t(nbt) * (kBlock / 64) + kReduceChunk - 1) / kReduceChunk);
+      GC_HIP(ctx, dsum.alloc(sizeof(double) * npv * chunks));
+      GC_HIP(ctx, hipMemsetAsync(mt.p, 0, pbytes, ctx->stream));
+      GC_HIP(ctx, hipMemsetAsync(vt.p, 0, pbytes, ctx->stream));
+      std::vector<EpochScal> hes(kTrialMax);
+      for (int k = 0; k < kTrialMax; ++k) hes[k] = epoch_scal(1 + k);
+      GC_HIP(ctx, hipMemcpyAsync(est.p, hes.data(), sizeof(EpochScal) * kTrialMax, hipMemcpyHostToDevice,
+                                 ctx->stream));
+      VolDev d{};
+      d.y = vols[0].y_dev;
+      d.w = HAS_W ? (vols[0].w_dev ? vols[0].w_dev : vols[0].y_dev) : nullptr;  // any finite weights
+      d.ex = vols[0].extra_dev;
+      d.z_in = vols[0].z_dev;
+      d.m_in = mt.as<float>();
+      d.v_in = vt.as<float>();
+      d.z_out = zt.as<float>();
+      d.m_out = mt.as<float>();
+      d.v_out = vt.as<float>();
+      d.partials = pt.as<float>();
+      d.es = est.as<EpochScal>();
+      d.nv = nvt;
+      d.blk0 = 0;
+      d.nblk = nbt;
+      d.epoch0 = 1;
+      d.iteration = INT32_MAX;
+      d.inv_count = float(1.0 / (double(md.n_meas) * double(nvt)));
+      AskParams ap{};
+      ap.vols = dvt.as<VolDev>();
+      ap.n_vol = 1;
+      ap.transform = cfg.transform;
+      ap.need_norms = npv == 4 ? 1 : 0;
+      ap.o = opt_params();
+      hipEvent_t e0, e1;
+      GC_HIP(ctx, hipEventCreate(&e0));
+      GC_HIP(ctx, hipEventCreate(&e1));
+      std::vector<double> hsum(size_t(npv) * chunks);
+      // one trial launch of n iterations at bs threads: kernel ms, and with
+      // cycle = true the wall time of the whole launch cycle
+      auto trial = [&](int bs, int n, bool cycle, float* ms, double* wall) -> int {
+        d.n_epochs = n;
+        const auto t0 = std::chrono::steady_clock::now();
+        GC_HIP(ctx, hipMemcpyAsync(dvt.p, &d, sizeof(VolDev), hipMemcpyHostToDevice, ctx->stream));
+        int e = launch_fused<Model, LOSS, HAS_W>(false, nbt, bs, ap, e0, e1);
+        if (e) return e;
+        if (cycle) {
+          hipLaunchKernelGGL(k_reduce_rows, dim3(chunks, (n * npv + 63) / 64), dim3(kBlock), 0, ctx->stream,
+                             pt.as<float>(), int64_t(nbt) * (kBlock / 64), n * npv, int64_t(kReduceChunk),
+                             dsum.as<double>());
+          GC_HIP(ctx, hipGetLastError());
+          GC_HIP(ctx, hipMemcpyAsync(hsum.data(), dsum.p, sizeof(double) * hsum.size(), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        }
+        GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (wall) *wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        GC_HIP(ctx, hipEventElapsedTime(ms, e0, e1));
+        return GC_OK;
+      };
+      int rc = GC_OK;
+      std::vector<int> cand;
+      if (fixed) cand.push_back(fixed);
+      else cand = {64, 128, 256};
+      float ms = 0.f;
+      int n = 4;
+      rc = trial(cand.back(), n, false, &ms, nullptr);  // also loads the kernel
+      while (!rc && ms < 1.0f && n < kTrialMax) {
+        n = std::min(kTrialMax, 2 * n);
+        rc = trial(cand.back(), n, false, &ms, nullptr);
+      }
+      int best = cand.back();
+      float best_ms = ms;
+      for (size_t c = 0; !rc && c + 1 < cand.size(); ++c) {
+        rc = trial(cand[c], n, false, &ms, nullptr);  // warm-up of this launch shape
+        if (!rc) rc = trial(cand[c], n, false, &ms, nullptr);
+        if (!rc && ms < best_ms) {
+          best_ms = ms;
+          best = cand[c];
+        }
+      }
+      // the launch cycle of one iteration: wall time less kernel time,
+      // median of three
+      std::vector<double> cyc;
+      for (int r = 0; !rc && r < 3; ++r) {
+        double wall = 0.0;
+        rc = trial(best, 1, true, &ms, &wall);
+        cyc.push_back(std::max(0.0, wall - double(ms)));
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (rc) return rc;
+      std::sort(cyc.begin(), cyc.end());
+      gc_ctx::AskTune t{best, std::max(cyc[1], 1e-3), double(best_ms) / (double(n) * double(nvt))};
+      ctx->ask_tuned[key] = t;
+      blk_threads = fixed ? fixed : best;
+      tune_launch_ms = t.launch_ms;
+      tune_vox_ms = t.ms_per_voxel_iter;
+      return GC_OK;
+    }
   }
 
   // The step form: one k_askadam_step launch per iteration.  Iteration k of
@@ -1080,6 +1254,7 @@ struct AskDriver {
       vols[i].kernel_ms = kernel_ms;
       vols[i].kernel_launches = kernel_launches;
       vols[i].iters_per_launch = K;
+      vols[i].block_threads = step_mode ? kBlock : blk_threads;
     }
     return GC_OK;
   }

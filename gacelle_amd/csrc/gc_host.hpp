@@ -35,6 +35,16 @@ struct gc_ctx {
   // sampler block sizes picked by the autotuner, per model variant and
   // chain layout
   std::map<std::array<int64_t, 9>, int> mcmc_tuned;
+  int ask_block = 0;            // GC_OPT_ASK_BLOCK: blocked askadam threads per block (0 = autotuned)
+  // the blocked askadam kernel timed by the autotuner, per model variant:
+  // its fastest block size, the fixed cost of one launch cycle (launch,
+  // partial reduction, copy back, synchronisation) and its time per
+  // voxel-iteration
+  struct AskTune {
+    int threads;
+    double launch_ms, ms_per_voxel_iter;
+  };
+  std::map<std::array<int64_t, 8>, AskTune> ask_tuned;
 };
 
 

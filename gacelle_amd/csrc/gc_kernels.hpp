@@ -154,10 +154,11 @@ struct VolDev {
   float* m_out;
   float* v_out;
   float* z_last;     // z (pre-update) at the launch's last iteration, or null
-  float* partials;   // [nblk][n_epochs][npv], npv = 4 or 1: a block's sums are contiguous
+  float* partials;   // [4 nblk][n_epochs][npv], npv = 4 or 1: the sums of each
+                     // wavefront (64 voxels) contiguous
   const EpochScal* es;
   int64_t nv;
-  int32_t blk0, nblk;
+  int32_t blk0, nblk;  // in blocks of kBlock voxels (the volumes' padding)
   int32_t epoch0, n_epochs, iteration, pad;
   float inv_count;   // 1/(M*nv)
   float pad2[3];
@@ -220,6 +221,14 @@ struct ask_waves {
   static constexpr int value = f<Model>(nullptr);
 };
 
+// Launched with 64, 128 or 256 threads per block (picked by a timed trial or
+// set with GC_OPT_ASK_BLOCK); the volumes are padded to kBlock voxels, so a
+// block never straddles two.  Every wavefront writes its own per-iteration
+// sums (a wavefront covers the same 64 voxels whatever the block size), so
+// the results do not depend on the block size.  The loss-only sums wait in
+// a wavefront's own ring of dynamic LDS (kRing x 64 floats) and are reduced
+// once per kRing iterations; no block barrier is needed.
+//
 // SIGNS (L1 only): also write the sign code of every differentiated residual
 // to V.signs (1 = positive, 2 = negative, 0 = zero or a non-finite model
 // signal), so that an FP64 replay can follow the same subgradient path.  The
@@ -230,15 +239,16 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   const gc_model& md = *mdp;
   constexpr int NP = Model::NP;
   constexpr int MR = Model::MR;
-  const int vi = find_volume(ap.vols, ap.n_vol, blockIdx.x);
+  const int64_t g0 = int64_t(blockIdx.x) * blockDim.x;  // first voxel slot of the block
+  const int vi = find_volume(ap.vols, ap.n_vol, int(g0 / kBlock));
   const VolDev V = ap.vols[vi];
   if (V.n_epochs <= 0) return;
-  const int lblk = blockIdx.x - V.blk0;
   const int64_t nv = V.nv;
-  const int64_t v = int64_t(lblk) * kBlock + threadIdx.x;
+  const int64_t v = g0 - int64_t(V.blk0) * kBlock + threadIdx.x;
   const bool valid = v < nv;
   const int64_t vv = valid ? v : (nv - 1);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t lw = v >> 6;  // the wavefront's partial row within the volume
 
   float yr[MR];
   float wr[HAS_W ? MR : 1];
@@ -277,13 +287,14 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   // the constant address space lets the wave-uniform loads go scalar.
   using ConstEpochScal = __attribute__((address_space(4))) const EpochScal;
   ConstEpochScal* es_table = reinterpret_cast<ConstEpochScal*>(reinterpret_cast<uintptr_t>(V.es));
-  // Per-iteration block sums.  With the norms (step and gradient stopping
-  // criteria) each wavefront reduces its four sums with DPP and cross-row
-  // permutes; for the loss alone every thread parks its value in an LDS ring
-  // that the block reduces once per kRing iterations (one row of 16 lanes per
-  // iteration), which keeps the reduction off the per-iteration VALU path.
-  __shared__ float red[kRing][kBlock / 64][4];
-  __shared__ float redl[kRing][kBlock];
+  // Per-iteration wavefront sums.  With the norms (step and gradient
+  // stopping criteria) each wavefront reduces its four sums with DPP and
+  // cross-row permutes and stores them; for the loss alone every thread parks
+  // its value in the wavefront's LDS ring, reduced once per kRing iterations
+  // (four lanes per iteration), which keeps the reduction off the
+  // per-iteration VALU path.
+  extern __shared__ float gc_ask_ring[];
+  float* const ring = gc_ask_ring + wave * (kRing * 64);
   const float inv_count = V.inv_count;
   const XRow xr = xrow_addr(lane);
 
@@ -377,43 +388,36 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
         }
       }
       if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
-      const int slot = k & (kRing - 1);
       if constexpr (norms) {
+        // four sums per iteration: loss, step^2, |z|^2, |g|^2
         const float tot = wave_sum4(lsum, step, den, gn, lane, xr);
-        if (lane < 4) red[slot][wave][wave_sum4_index(lane)] = tot;
+        if (lane < 4) V.partials[((lw * V.n_epochs + k) << 2) + wave_sum4_index(lane)] = tot;
       } else {
-        redl[slot][threadIdx.x] = lsum;
-      }
-      if (slot == kRing - 1 || k == V.n_epochs - 1) {
-        __syncthreads();
-        const int cnt = slot + 1;
-        const int kbase = k - slot;
-        if constexpr (norms) {
-          // four sums per iteration: loss, step^2, |z|^2, |g|^2
-          if (threadIdx.x < (cnt << 2)) {
-            const int kk = threadIdx.x >> 2, j = threadIdx.x & 3;
-            float s = 0.f;
-#pragma unroll
-            for (int wv = 0; wv < kBlock / 64; ++wv) s += red[kk][wv][j];
-            V.partials[((int64_t(lblk) * V.n_epochs + kbase + kk) << 2) + j] = s;
-          }
-        } else {
-          // row kk of 16 lanes sums iteration kk's 256 values, 16 per lane
-          static_assert(kRing * 16 == kBlock, "one row of 16 lanes per ring slot");
-          const int kk = threadIdx.x >> 4, part = threadIdx.x & 15;
+        const int slot = k & (kRing - 1);
+        ring[slot * 64 + lane] = lsum;
+        if (slot == kRing - 1 || k == V.n_epochs - 1) {
+          // lanes 4 kk .. 4 kk + 3 sum ring slot kk, 16 values each; LDS
+          // operations of one wavefront complete in order, so the slot
+          // stores above are visible to these loads
+          static_assert(kRing * 4 == 64, "four lanes per ring slot");
+          __builtin_amdgcn_wave_barrier();
+          const int cnt = slot + 1;
+          const int kbase = k - slot;
+          const int kk = lane >> 2, part = lane & 3;
           float s = 0.f;
           if (kk < cnt) {
-            const float4* p4 = reinterpret_cast<const float4*>(&redl[kk][part * 16]);
+            const float4* p4 = reinterpret_cast<const float4*>(&ring[kk * 64 + part * 16]);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               const float4 q = p4[i];
               s += (q.x + q.y) + (q.z + q.w);
             }
           }
-          s = row_sum16(s);
-          if (part == 0 && kk < cnt) V.partials[int64_t(lblk) * V.n_epochs + kbase + kk] = s;
+          s += dpp<kDppXor1>(s);
+          s += dpp<kDppXor2>(s);
+          if (part == 0 && kk < cnt) V.partials[lw * V.n_epochs + kbase + kk] = s;
+          __builtin_amdgcn_wave_barrier();
         }
-        __syncthreads();
       }
     }
   };

@@ -52,6 +52,15 @@ struct OpsImpl {
 
   static int askadam(gc_ctx* ctx, const gc_model* md, const gc_askadam_cfg* cfg, gc_volume* vols, int n_vol) {
     AskDriver drv(ctx, *md, *cfg, vols, n_vol);
+    drv.tuner = [](AskDriver& d) -> int {
+      return with_loss(d.ctx, d.cfg.loss, [&](auto lc) -> int {
+        constexpr int LOSS = decltype(lc)::value;
+        return with_bool(d.has_w, [&](auto hw) -> int {
+          constexpr bool HW = decltype(hw)::value;
+          return d.template tune_blocked<Model, LOSS, HW>();
+        });
+      });
+    };
     int rc = drv.setup(Model::kStreamOnly, has_aux<Model>::value);
     if (rc) return rc;
     return with_loss(ctx, cfg->loss, [&](auto lc) -> int {

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define GC_ABI_VERSION 8
+#define GC_ABI_VERSION 9
 #define GC_MAX_PAR 24   /* parameter slots per voxel            */
 #define GC_MAX_TABLE 256 /* acquisition table entries             */
 #define GC_MAX_VOL 16   /* volumes per batched askadam launch     */
@@ -277,6 +277,9 @@ typedef struct gc_volume {
    * [iteration][ceil(M/16)][nv] of uint32, row e-1 for epoch e.  Lets an
    * FP64 replay follow the same L1 subgradient path. */
   uint32_t* sign_trace_dev;
+  int32_t block_threads;     /* out: threads per block of the fused launches
+                                (GC_OPT_ASK_BLOCK or the autotuner's pick;
+                                256 in the per-iteration form)            */
 } gc_volume;
 
 /* ---- MCMC configuration: POD mirror of mcmc.check_set_default_basic
@@ -386,8 +389,17 @@ const char* gc_last_error(const gc_ctx* ctx);
  *  GC_OPT_MCMC_BLOCK  threads per block of the MCMC samplers: 0 (default)
  *                     = picked per model and chain layout by a short timed
  *                     trial on the first call (cached in the context), or
- *                     64, 128 or 256.  Results do not depend on it. */
-enum { GC_OPT_GRAPHS = 1, GC_OPT_MCMC_BLOCK = 2 };
+ *                     64, 128 or 256.  Results do not depend on it.
+ *  GC_OPT_ASK_BLOCK   threads per block of the blocked askadam kernel (the
+ *                     form without a regulariser): 0 (default) = picked per
+ *                     model variant by a short timed trial on the first call
+ *                     (cached in the context), or 64, 128 or 256.  The same
+ *                     trial measures the cost of one launch cycle and of one
+ *                     voxel-iteration, from which the iterations per launch
+ *                     (K) are set when stopping criteria are on and
+ *                     gc_askadam_cfg.block_iters is 0.  Results depend on
+ *                     neither. */
+enum { GC_OPT_GRAPHS = 1, GC_OPT_MCMC_BLOCK = 2, GC_OPT_ASK_BLOCK = 3 };
 int gc_ctx_set_option(gc_ctx* ctx, int32_t option, int64_t value);
 
 /* ---- layout helpers (utils/utils.m:157-163, 194-198, 299-315) ----------- */

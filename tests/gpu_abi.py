@@ -125,7 +125,8 @@ def askadam(md, cfg, vols, trace=False, signs=False):
                  resloss_min=runtime.to_host(rlm), residual=runtime.to_host(rs), residual_min=runtime.to_host(rsm),
                  loss=a.loss, loss_fidelity=a.loss_fidelity, loss_reg=a.loss_reg, min_loss=a.min_loss,
                  min_iteration=a.min_iteration, n_iteration=a.n_iteration, stop_reason=a.stop_reason,
-                 convergence=a.convergence, n_launch=a.n_launch)
+                 convergence=a.convergence, n_launch=a.n_launch, block_threads=a.block_threads,
+                 iters_per_launch=a.iters_per_launch)
         if tr is not None:
             o["trace"] = np.array(tr[: a.n_iteration + 1])
         if sg is not None:

@@ -1,7 +1,8 @@
-"""GPU tests of two launch choices that must not change results: the
-step-form iteration blocks replayed from captured hipGraphs (against one
-launch per iteration) and the threads per block of the MH / GW samplers
-(fixed sizes against each other and against the autotuner's pick)."""
+"""GPU tests of launch choices that must not change results: the step-form
+iteration blocks replayed from captured hipGraphs (against one launch per
+iteration), the threads per block of the MH / GW samplers and of the blocked
+askadam kernel (fixed sizes against each other and against the autotuner's
+pick), and the autotuner's iterations per launch."""
 import numpy as np
 import pytest
 
@@ -23,6 +24,7 @@ def _with_option(option, value, fn):
     finally:
         c.set_option(_abi.GC_OPT_GRAPHS, 1)
         c.set_option(_abi.GC_OPT_MCMC_BLOCK, 0)
+        c.set_option(_abi.GC_OPT_ASK_BLOCK, 0)
 
 
 def _tv_problem(iteration, stopping, block_iters):
@@ -146,3 +148,67 @@ def test_option_values_are_checked():
     for opt, bad in ((_abi.GC_OPT_GRAPHS, 2), (_abi.GC_OPT_MCMC_BLOCK, 96), (99, 0)):
         with pytest.raises(_abi.GacelleError):
             c.set_option(opt, bad)
+
+
+def _blocked_problem(loss, norms):
+    """Two volumes of uneven size (the second not a multiple of 64 voxels)
+    for the blocked kernel, with a loss-convergence stop inside a block."""
+    vols = []
+    for nv, seed in ((1500, 21), (333, 22)):
+        y, _, _ = r2s_problem(nv, seed=seed)
+        vols.append(dict(y=y, z0=None))
+    extra = dict(iteration=400, verbose=False, lossFunction=loss, initialLearnRate=2e-2, convergenceValue=1e-4,
+                 convergenceStepTol=1e-9 if norms else 0, convergenceGradTol=0)
+    _, f, md = r2s_setup(extra)
+    cfg = askadam().make_cfg(f, None)
+    for v in vols:
+        nv = v["y"].shape[1]
+        v["z0"] = logit_start(np.stack([np.ones(nv), np.full(nv, 30.0)]), md)
+    return md, cfg, vols
+
+
+@pytest.mark.parametrize("loss,norms", [("L1", False), ("L2", True)])
+def test_askadam_block_threads_do_not_change_results(loss, norms):
+    """The blocked kernel at 64, 128 and 256 threads per block and at the
+    autotuner's pick: bit-identical fits, loss traces, stops and residuals
+    (every wavefront sums its own 64 voxels; the loss-only path through its
+    LDS ring, the norms path directly)."""
+    md, cfg, vols = _blocked_problem(loss, norms)
+    runs = {}
+    for bs in (64, 128, 256, 0):
+        runs[bs] = _with_option(_abi.GC_OPT_ASK_BLOCK, bs, lambda: gpu_abi.askadam(md, cfg, vols, trace=True))
+        for o in runs[bs]:
+            assert o["block_threads"] == (bs if bs else o["block_threads"]) and o["block_threads"] in (64, 128, 256)
+    ref = runs[256]
+    if loss == "L1":
+        assert ref[0]["stop_reason"] != _abi.GC_STOP_NONE
+    for bs in (64, 128, 0):
+        for i, (g, r) in enumerate(zip(runs[bs], ref)):
+            tag = f"block {bs}, volume {i}"
+            assert (g["n_iteration"], g["stop_reason"], g["min_iteration"]) == \
+                (r["n_iteration"], r["stop_reason"], r["min_iteration"]), tag
+            np.testing.assert_array_equal(g["trace"], r["trace"], err_msg=tag)
+            for k in ("z_final", "z_min", "resloss", "residual"):
+                np.testing.assert_array_equal(g[k], r[k], err_msg=f"{tag} {k}")
+
+
+def test_askadam_autotuned_launch_shape():
+    """With stopping criteria on and no iterationsPerLaunch, the autotuner
+    sets K from its measured launch and iteration costs (16 .. 256), keeps
+    its pick for the next call of the same model variant, and the fit
+    equals one with a fixed K of 1 (the block replay is exact)."""
+    md, cfg, vols = _blocked_problem("L1", False)
+    a = gpu_abi.askadam(md, cfg, vols, trace=True)
+    b = gpu_abi.askadam(md, cfg, vols, trace=True)
+    assert 16 <= a[0]["iters_per_launch"] <= 256
+    assert (b[0]["block_threads"], b[0]["iters_per_launch"]) == (a[0]["block_threads"], a[0]["iters_per_launch"])
+    cfg1 = _abi.gc_askadam_cfg.from_buffer_copy(cfg)
+    cfg1.block_iters = 1
+    c = gpu_abi.askadam(md, cfg1, vols, trace=True)
+    assert c[0]["iters_per_launch"] == 1
+    for g, r in zip(a, c):
+        assert (g["n_iteration"], g["stop_reason"], g["min_iteration"]) == \
+            (r["n_iteration"], r["stop_reason"], r["min_iteration"])
+        np.testing.assert_array_equal(g["trace"], r["trace"])
+        for k in ("z_final", "z_min", "resloss", "residual"):
+            np.testing.assert_array_equal(g[k], r[k], err_msg=k)

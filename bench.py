@@ -790,7 +790,7 @@ def main():
         kms.append(vol.kernel_ms)
         kl.append(vol.kernel_launches)
         log(f"askadam step {s + 1}/{args.steps}: {times[-1] * 1e3:.2f} ms (kernel {vol.kernel_ms:.2f} ms, "
-            f"{vol.kernel_launches} launches, K={vol.iters_per_launch})")
+            f"{vol.kernel_launches} launches, K={vol.iters_per_launch}, {vol.block_threads} threads per block)")
     t_step = float(np.median(times))
     vox_per_s = nv / t_step
     zf_h = zf.cpu().numpy()
@@ -963,7 +963,8 @@ def main():
             "unit_of_work": "voxel-iteration",
         },
         "cpu_baseline": cpu,
-        "fused": {"final_loss": fused_loss, "median_rel_err": err_fused, "oracle_rel_err": oracle_rel_err},
+        "fused": {"final_loss": fused_loss, "median_rel_err": err_fused, "oracle_rel_err": oracle_rel_err,
+                  "iterations_per_launch": int(vol.iters_per_launch), "threads_per_block": int(vol.block_threads)},
         "tape": tape,
         "mcmc": mh,
         "workloads": extra,

@@ -194,6 +194,14 @@ GC_DEV float opt_step(const OptParams& o, const EpochScal& es, float g, float& m
   return dz;
 }
 
+// Adam in its default (effective learning-rate) form: the arithmetic of that
+// branch of opt_step, for loops instantiated with the optimiser known
+GC_DEV float adam_eff_step(const OptParams& o, const EpochScal& es, float g, float& m1, float& m2) {
+  m1 = o.beta1 * m1 + (1.f - o.beta1) * g;
+  m2 = o.beta2 * m2 + (1.f - o.beta2) * g * g;
+  return -es.lr * m1 * rcpf(sqrtv(m2) + o.eps);
+}
+
 struct AskParams {
   const VolDev* vols;
   int32_t n_vol;
@@ -299,10 +307,13 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   const XRow xr = xrow_addr(lane);
 
   // The epoch loop, instantiated per parameter transform and per norms flag
-  // so that neither costs a per-iteration selection.
-  auto epochs = [&](auto tr, auto nm) {
+  // so that neither costs a per-iteration selection, and with the sigmoid
+  // transform also for the default optimiser (Adam, effective form), whose
+  // update then runs without the optimiser switch.
+  auto epochs = [&](auto tr, auto nm, auto oa) {
     constexpr int TR = decltype(tr)::value;
     constexpr bool norms = decltype(nm)::value;
+    constexpr bool adam_eff = decltype(oa)::value;
     for (int k = 0; k < V.n_epochs; ++k) {
       const int e = V.epoch0 + k;
       float th[NP], dth[NP];
@@ -378,7 +389,11 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
         const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2, 0.f};
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-          const float dz = opt_step(ap.o, es, g[p], m1[p], m2[p]);
+          float dz;
+          if constexpr (adam_eff)
+            dz = adam_eff_step(ap.o, es, g[p], m1[p], m2[p]);
+          else
+            dz = opt_step(ap.o, es, g[p], m1[p], m2[p]);
           const float zn = z[p] + dz;
           if constexpr (norms) {
             const float d = zn - z[p];
@@ -422,12 +437,18 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
     }
   };
   const bool with_norms = ap.need_norms != 0;
+  const bool adam_eff = ap.o.opt == GC_OPT_ADAM && ap.o.adam_form != GC_ADAM_KINGMA_BA;
+  using Lin = std::integral_constant<int, GC_TRANSFORM_LINEAR>;
+  using Sig = std::integral_constant<int, GC_TRANSFORM_SIGMOID>;
   if (ap.transform == GC_TRANSFORM_LINEAR) {
-    if (with_norms) epochs(std::integral_constant<int, GC_TRANSFORM_LINEAR>{}, std::true_type{});
-    else epochs(std::integral_constant<int, GC_TRANSFORM_LINEAR>{}, std::false_type{});
+    if (with_norms) epochs(Lin{}, std::true_type{}, std::false_type{});
+    else epochs(Lin{}, std::false_type{}, std::false_type{});
+  } else if (adam_eff) {
+    if (with_norms) epochs(Sig{}, std::true_type{}, std::true_type{});
+    else epochs(Sig{}, std::false_type{}, std::true_type{});
   } else {
-    if (with_norms) epochs(std::integral_constant<int, GC_TRANSFORM_SIGMOID>{}, std::true_type{});
-    else epochs(std::integral_constant<int, GC_TRANSFORM_SIGMOID>{}, std::false_type{});
+    if (with_norms) epochs(Sig{}, std::true_type{}, std::false_type{});
+    else epochs(Sig{}, std::false_type{}, std::false_type{});
   }
 
   if (valid) {

@@ -187,4 +187,10 @@ GC_DEV double wave_sum_d(double v) {
 
 GC_DEV float sgnf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
 
+// reciprocal with the hardware v_rcp_f32 (1 ulp)
+GC_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// square root with the hardware v_sqrt_f32 (1 ulp), not the correctly
+// rounded expansion
+GC_DEV float sqrtv(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 }  // namespace gc

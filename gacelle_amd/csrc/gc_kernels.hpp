@@ -27,11 +27,6 @@ constexpr int kMaxExtra = 12;
 // Kernels that are not templates have internal linkage (static): several
 // translation units include this header and launch them.
 
-// reciprocal with the hardware v_rcp_f32 (1 ulp)
-GC_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-// square root with the hardware v_sqrt_f32 (1 ulp), not the correctly
-// rounded expansion
-GC_DEV float sqrtv(float x) { return __builtin_amdgcn_sqrtf(x); }
 constexpr int kRing = 16;  // iterations buffered in LDS before a partial flush
 constexpr int kReduceChunk = 1024;  // block partials per k_reduce_rows block
 

@@ -142,25 +142,77 @@ struct NexiModel {
     const float* cost = md.table + 2 * ns + 2 * nn;
     bool clamped = false;
     if (x < -170.f) { x = -170.f; clamped = true; }
-    const float z = -0.5f * x;
-    float i0 = 0.f, i1 = 0.f, d0 = 0.f, d1 = 0.f;
+    // exp(z c) = 2^(z log2(e) c), z = -x/2
+    const float z2 = -0.7213475204444817f * x;
+    float i0 = 0.f, i1 = 0.f, d1 = 0.f;
+#pragma unroll
     for (int k = 0; k < 34; ++k) {
       const float c = cost[k];
       const float wk = (k == 0 || k == 33) ? 0.5f : 1.f;
-      const float e = __expf(z * c) * wk;
+      const float e = __builtin_amdgcn_exp2f(z2 * c) * wk;
+      const float ec = e * c;
       i0 += e;
-      i1 += e * c;
-      d0 += e * c;
-      d1 += e * c * c;
+      i1 += ec;
+      d1 = fmaf(ec, c, d1);
     }
     const float h = 1.f / 33.f;  // (pi/33) / pi
-    i0 *= h; i1 *= h; d0 *= h; d1 *= h;
-    const float ex2 = __expf(0.5f * x);
+    i0 *= h; i1 *= h; d1 *= h;
+    const float ex2 = __builtin_amdgcn_exp2f(0.7213475204444817f * x);  // exp(x/2)
     const float core = (1.f - x) * i0 - x * i1;
     L = ex2 * core;
-    // dI/dx = dI/dz * (-1/2)
-    const float dcore = -i0 + (1.f - x) * d0 * (-0.5f) - i1 - x * d1 * (-0.5f);
+    // dI/dx = dI/dz * (-1/2); the derivative of the c-weighted integral is i1 again
+    const float dcore = -i0 + (1.f - x) * i1 * (-0.5f) - i1 - x * d1 * (-0.5f);
     dL = clamped ? 0.f : (0.5f * L + ex2 * dcore);
+  }
+
+  // The Karger integrand of one shell at node x (NEXI/gpuNEXI.m:1090-1097):
+  //   M(x) = Pp E1 + (1 - Pp) E2,  E1 = exp(-(l1 + l2)),  E2 = exp(-(l1 - l2)),
+  //   l1 = (r1 + r2 + d1 + d2)/2,  l2 = sqrt((r1 - r2 + d1 - d2)^2 + 4 r1 r2)/2
+  //   (at least 1e-8),  Pp = (fa d1 + (1 - fa) d2 - (l1 - l2)) / (2 l2),
+  // with d1 = b Da x^2, d2 = b De, r1 = Delta ra, r2 = r1 fa / (1 - fa).
+  // The x-independent parts are folded once per shell, so a node costs 13
+  // VALU operations besides sqrt, rcp and two exp2.
+  struct Shell {
+    float dab;  // d1 / x^2 = b Da
+    float cu;   // r1 - r2 - d2
+    float c4;   // 4 r1 r2
+    float fam;  // (fa - 1/2) b Da
+    float cn;   // (1 - fa) d2 - (r1 + r2 + d2) / 2
+    float as0;  // -log2(e) (r1 + r2 + d2) / 2
+    float asd;  // -log2(e) b Da / 2
+  };
+  static constexpr float kHalfLog2e = 0.7213475204444817f;
+  GC_DEV static Shell shell(float fa, float dab, float d2, float r1, float r2) {
+    const float cs = r1 + r2 + d2;
+    Shell K;
+    K.dab = dab;
+    K.cu = r1 - r2 - d2;
+    K.c4 = 4.f * r1 * r2;
+    K.fam = (fa - 0.5f) * dab;
+    K.cn = (1.f - fa) * d2 - 0.5f * cs;
+    K.as0 = -kHalfLog2e * cs;
+    K.asd = -kHalfLog2e * dab;
+    return K;
+  }
+  // node values: 2 l2 (clamped), 1/(2 l2), Pp, E1, E2, M and the u = r1 - r2 + d1 - d2
+  // and clamp flag the derivatives need
+  struct Node {
+    float u, sqh, rq, Pp, E1, E2, M;
+    bool clamp;
+  };
+  GC_DEV static Node node(const Shell& K, float x2) {
+    Node n;
+    n.u = fmaf(K.dab, x2, K.cu);
+    const float sq = sqrtv(fmaf(n.u, n.u, K.c4));
+    n.clamp = sq < 2e-8f;                 // l2 < 1e-8
+    n.sqh = fmaxf(sq, 2e-8f);             // 2 l2
+    n.rq = rcpf(n.sqh);                   // 1 / (2 l2)
+    n.Pp = fmaf(0.5f, n.sqh, fmaf(K.fam, x2, K.cn)) * n.rq;
+    const float as = fmaf(K.asd, x2, K.as0);  // -log2(e) l1
+    n.E1 = __builtin_amdgcn_exp2f(fmaf(-kHalfLog2e, n.sqh, as));
+    n.E2 = __builtin_amdgcn_exp2f(fmaf(kHalfLog2e, n.sqh, as));
+    n.M = fmaf(n.Pp, n.E1 - n.E2, n.E2);
+    return n;
   }
 
   template <bool JAC, class F>
@@ -176,6 +228,13 @@ struct NexiModel {
     const float rden_re = 1.f / den_re;
     const float* xs = md.table + 2 * ns;
     const float* ws = md.table + 2 * ns + nn;
+    // Rician mean: -1/(2 sigma^2) and sigma sqrt(pi/2) of this voxel
+    float nis2 = 0.f, c0 = 0.f;
+    if (md.flags & GC_FLAG_NEXI_RICIAN) {
+      const float sigma = ex[0];
+      nis2 = -0.5f / (sigma * sigma);
+      c0 = sigma * 1.2533141373155003f;
+    }
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
       if (k < ns) {
@@ -185,34 +244,26 @@ struct NexiModel {
         // d re / d fa (through both numerator and 1/(1-fa))
         const float dre_dfa = (guard && omf < 1e-8f) ? rab * rden_re : rab * rden_re * rden_re;
         const float dre_dra = D * fa * rden_re;
+        const Shell K = shell(fa, dab, deb, rab, re);
         float s0 = 0.f, s2 = 0.f;
         float gA0 = 0, gD10 = 0, gD20 = 0, gR10 = 0, gR20 = 0;  // Sl0 partial sums
         float gA2 = 0, gD12 = 0, gD22 = 0, gR12 = 0, gR22 = 0;  // Sl2 partial sums
+#pragma unroll 2
         for (int j = 0; j < nn; ++j) {
           const float x = xs[j], w = ws[j];
           const float x2 = x * x;
-          const float d1 = dab * x2, d2 = deb, r1 = rab, r2 = re;
-          const float l1 = 0.5f * (r1 + r2 + d1 + d2);
-          const float u = r1 - r2 + d1 - d2;
-          const float q = u * u + 4.f * r1 * r2;
-          const float sq = sqrtf(q);
-          float l2 = 0.5f * sq;
-          const bool l2c = l2 < 1e-8f;
-          if (l2c) l2 = 1e-8f;
-          const float lm = l1 - l2;
-          const float rl2 = 1.f / l2;
-          const float num = fa * d1 + (1.f - fa) * d2 - lm;
-          const float Pp = 0.5f * num * rl2;
-          const float E1 = __expf(-(l1 + l2));
-          const float E2 = __expf(-lm);
-          const float Mx = Pp * E1 + (1.f - Pp) * E2;
+          const Node n = node(K, x2);
           const float w2 = w * (1.5f * x2 - 0.5f);
-          s0 += w * Mx;
-          if (L2) s2 += w2 * Mx;
+          s0 = fmaf(w, n.M, s0);
+          if (L2) s2 = fmaf(w2, n.M, s2);
           if (JAC) {
+            const float d1 = dab * x2, d2 = deb, r1 = rab, r2 = re;
+            const float rl2 = 2.f * n.rq;  // 1 / l2
             // partials of l2 with respect to d1, d2, r1, r2
-            const float c = l2c ? 0.f : 0.25f * rl2;
+            const float c = n.clamp ? 0.f : 0.5f * n.rq;
+            const float u = n.u;
             const float l2_d1 = c * u, l2_d2 = -c * u, l2_r1 = c * (u + 2.f * r2), l2_r2 = c * (2.f * r1 - u);
+            const float Pp = n.Pp, E1 = n.E1, E2 = n.E2;
             const float EmE = E1 - E2;
             // dM/dz = dPp/dz*(E1-E2) - Pp*E1*(dl1+dl2) - (1-Pp)*E2*(dl1-dl2)
             // dPp/dz = (dnum/dz)/(2 l2) - Pp*dl2/dz / l2 ; dnum = da*(d1-d2) + a dd1 + (1-a) dd2 - dlm
@@ -241,14 +292,12 @@ struct NexiModel {
           if (L2) J0[4] = 0.f;
         }
         if (md.flags & GC_FLAG_NEXI_RICIAN) {
-          const float sigma = ex[0];
-          const float xr = -(s0 * s0) * 0.5f / (sigma * sigma);
+          const float xr = (s0 * s0) * nis2;  // -s0^2 / (2 sigma^2)
           float L, dL;
           laguerre_half(md, xr, L, dL);
-          const float c0 = sigma * 1.2533141373155003f;  // sigma*sqrt(pi/2)
           const float m1 = c0 * L;
           if (m1 >= s0) {
-            const float ds = c0 * dL * (-s0 / (sigma * sigma));
+            const float ds = c0 * dL * (2.f * s0 * nis2);  // d xr / d s0 = -s0 / sigma^2
             if (JAC) {
 #pragma unroll
               for (int p = 0; p < NP; ++p) J0[p] *= ds;

@@ -47,12 +47,26 @@ def _run(cmd: list[str]) -> tuple[int, str]:
     return res.returncode, res.stdout + res.stderr
 
 
-def build_library(verbose: bool = False, jobs: int | None = None) -> str:
+def _stale(src: str, obj: str) -> bool:
+    """The object is missing or older than its source or any shared header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src, *glob.glob(os.path.join(CSRC, "*.hpp")),
+            os.path.join(HERE, "..", "include", "gacelle.h"), os.path.abspath(__file__)]
+    return os.path.getmtime(obj) < max(os.path.getmtime(d) for d in deps)
+
+
+def build_library(verbose: bool = False, jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every translation unit and link the library.  incremental
+    recompiles only the objects older than their sources or the headers."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     os.makedirs(OBJ_DIR, exist_ok=True)
-    srcs = sources()
-    objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
-    jobs = jobs or min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
+    all_srcs = sources()
+    all_objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in all_srcs]
+    pairs = [(s, o) for s, o in zip(all_srcs, all_objs) if not incremental or _stale(s, o)]
+    srcs = [s for s, _ in pairs]
+    objs = [o for _, o in pairs]
+    jobs = jobs or min(max(len(srcs), 1), max(1, min(16, os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda so: (so[0], *_run([hipcc, *HIPCC_FLAGS,
                                                                        *EXTRA_FLAGS.get(os.path.basename(so[0]), []),
@@ -63,7 +77,7 @@ def build_library(verbose: bool = False, jobs: int | None = None) -> str:
         for s, log in failed:
             sys.stderr.write(f"--- {s}\n{log}\n")
         raise RuntimeError(f"hipcc failed on {len(failed)} translation unit(s)")
-    rc, log = _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs])
+    rc, log = _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *all_objs])
     if rc != 0:
         sys.stderr.write(log)
         raise RuntimeError("linking libgacelle_hip.so failed")
@@ -74,4 +88,4 @@ def build_library(verbose: bool = False, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    build_library(verbose=True)
+    build_library(verbose=True, incremental="--incremental" in sys.argv)

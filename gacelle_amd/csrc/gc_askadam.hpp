@@ -480,6 +480,7 @@ struct AskDriver {
       } else {
         s.lr = float(lr * std::sqrt(bc2) / bc1);
         s.c1 = s.c2 = 1.f;
+        s.lr_s = float(lr * std::sqrt(bc2) / bc1 * (1.0 - cfg.beta1) / std::sqrt(1.0 - cfg.beta2));
       }
     } else {
       s.lr = float(lr);
@@ -497,6 +498,7 @@ struct AskDriver {
     o.momentum = float(cfg.momentum);
     o.rms_decay = float(cfg.rms_decay);
     o.rms_eps = float(cfg.rms_eps);
+    o.eps_s = float(cfg.eps / std::sqrt(1.0 - cfg.beta2));
     return o;
   }
 

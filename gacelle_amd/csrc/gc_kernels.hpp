@@ -68,6 +68,25 @@ GC_DEV void transform_fwd_t(float z, float lb, float ub, float& th, float& dth) 
   }
 }
 
+// The same theta with d theta / d z scaled by the loss mean's 1/count c, for
+// the blocked loop: rs = (ub - lb) c is launch-uniform, and the sigmoid's
+// derivative sg (1 - sg) is formed as sg - sg^2 (one FMA), so the scaled
+// derivative costs an FMA and a multiply.
+template <int TR>
+GC_DEV void transform_fwd_scaled(float z, float lb, float ub, float rs, float c, float& th, float& dths) {
+  if (TR == GC_TRANSFORM_SIGMOID) {
+    const float sg = rcpf(1.f + __expf(-z));
+    th = fmaf(ub - lb, sg, lb);
+    dths = fmaf(-sg, sg, sg) * rs;
+  } else if (TR == GC_TRANSFORM_LINEAR) {
+    th = fminf(fmaxf(z, 0.f), 1.f) * (ub - lb) + lb;
+    dths = (z >= 0.f && z <= 1.f) ? rs : 0.f;
+  } else {
+    th = z;
+    dths = c;
+  }
+}
+
 // Run-time selection for the single-evaluation kernels.  Branch-free: the
 // selection is a select, never control flow around values that are later
 // combined.
@@ -132,10 +151,10 @@ GC_DEV void model_adjoint(const gc_model& md, const float* th, const float* ex, 
 
 // ---- askadam block kernel -------------------------------------------------
 struct EpochScal {
-  float lr;  // adam: lr*sqrt(1-b2^t)/(1-b1^t) (effective form) or lr; sgdm/rmsprop: lr
-  float c1;  // 1/(1-b1^t) (Kingma-Ba form)
-  float c2;  // 1/(1-b2^t) (Kingma-Ba form)
-  float pad;
+  float lr;    // adam: lr*sqrt(1-b2^t)/(1-b1^t) (effective form) or lr; sgdm/rmsprop: lr
+  float c1;    // 1/(1-b1^t) (Kingma-Ba form)
+  float c2;    // 1/(1-b2^t) (Kingma-Ba form)
+  float lr_s;  // adam, effective form: lr (1-b1) / sqrt(1-b2), the step on scaled moments
 };
 
 struct VolDev {
@@ -167,6 +186,7 @@ struct OptParams {
   int32_t opt;
   int32_t adam_form;
   float beta1, beta2, eps, momentum, rms_decay, rms_eps;
+  float eps_s;  // eps / sqrt(1 - beta2), the epsilon of the scaled moments
 };
 
 // the parameter step dz of one slot; m1 / m2 are its moment estimates
@@ -189,12 +209,18 @@ GC_DEV float opt_step(const OptParams& o, const EpochScal& es, float g, float& m
   return dz;
 }
 
-// Adam in its default (effective learning-rate) form: the arithmetic of that
-// branch of opt_step, for loops instantiated with the optimiser known
+// Adam in its default (effective learning-rate) form for loops instantiated
+// with the optimiser known, on scaled moments m1 = m / (1 - beta1) and
+// m2 = v / (1 - beta2): m1 <- beta1 m1 + g and m2 <- beta2 m2 + g^2 drop the
+// (1 - beta) factors, and the step
+//   lr m / (sqrt(v) + eps) = [lr (1 - beta1) / sqrt(1 - beta2)] m1 / (sqrt(m2) + eps / sqrt(1 - beta2))
+// takes the bracket (es.lr_s) and the scaled epsilon (o.eps_s) from the host.
+// The blocked kernel keeps its Adam state in this scaled form whenever it
+// runs this update (every launch of such a call does).
 GC_DEV float adam_eff_step(const OptParams& o, const EpochScal& es, float g, float& m1, float& m2) {
-  m1 = o.beta1 * m1 + (1.f - o.beta1) * g;
-  m2 = o.beta2 * m2 + (1.f - o.beta2) * g * g;
-  return -es.lr * m1 * rcpf(sqrtv(m2) + o.eps);
+  m1 = fmaf(o.beta1, m1, g);
+  m2 = fmaf(g, g, o.beta2 * m2);
+  return -es.lr_s * m1 * rcpf(sqrtv(m2) + o.eps_s);
 }
 
 struct AskParams {
@@ -299,6 +325,9 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   extern __shared__ float gc_ask_ring[];
   float* const ring = gc_ask_ring + wave * (kRing * 64);
   const float inv_count = V.inv_count;
+  float rsp[NP];  // (ub - lb) / count
+#pragma unroll
+  for (int p = 0; p < NP; ++p) rsp[p] = (ubp[p] - lbp[p]) * inv_count;
   const XRow xr = xrow_addr(lane);
 
   // The epoch loop, instantiated per parameter transform and per norms flag
@@ -311,9 +340,9 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
     constexpr bool adam_eff = decltype(oa)::value;
     for (int k = 0; k < V.n_epochs; ++k) {
       const int e = V.epoch0 + k;
-      float th[NP], dth[NP];
+      float th[NP], dths[NP];
 #pragma unroll
-      for (int p = 0; p < NP; ++p) transform_fwd_t<TR>(z[p], lbp[p], ubp[p], th[p], dth[p]);
+      for (int p = 0; p < NP; ++p) transform_fwd_scaled<TR>(z[p], lbp[p], ubp[p], rsp[p], inv_count, th[p], dths[p]);
       float lsum = 0.f;
       float gth[NP];
       // Fast pass without per-element checks; a non-finite model signal makes
@@ -328,8 +357,10 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
         constexpr bool CHECK = decltype(checked)::value;
         constexpr bool L1FAST = !CHECK && LOSS == GC_LOSS_L1;
         lsum = 0.f;
+        // -0 is the additive identity (-0 + x = x for every x, +0 + -0 is
+        // +0), so the first gradient term needs no add
 #pragma unroll
-        for (int p = 0; p < NP; ++p) gth[p] = 0.f;
+        for (int p = 0; p < NP; ++p) gth[p] = -0.f;
         if constexpr (SIGNS) {
           if (valid)
             for (int j = 0; j < ((md.n_meas + 15) >> 4); ++j) sw[int64_t(j) * nv] = 0u;
@@ -368,7 +399,7 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
       float den = dead_z2, gn = 0.f, step = 0.f;
       float g[NP];
 #pragma unroll
-      for (int p = 0; p < NP; ++p) g[p] = gth[p] * dth[p] * inv_count;
+      for (int p = 0; p < NP; ++p) g[p] = gth[p] * dths[p];
       if constexpr (norms) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
@@ -381,7 +412,7 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
         for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
       }
       if (e < V.iteration) {  // update_parameters skips the last iteration (:677)
-        const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2, 0.f};
+        const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2, es_table[k].lr_s};
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
           float dz;
@@ -432,7 +463,8 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
     }
   };
   const bool with_norms = ap.need_norms != 0;
-  const bool adam_eff = ap.o.opt == GC_OPT_ADAM && ap.o.adam_form != GC_ADAM_KINGMA_BA;
+  // (the scaled moments need beta2 < 1)
+  const bool adam_eff = ap.o.opt == GC_OPT_ADAM && ap.o.adam_form != GC_ADAM_KINGMA_BA && ap.o.beta2 < 1.f;
   using Lin = std::integral_constant<int, GC_TRANSFORM_LINEAR>;
   using Sig = std::integral_constant<int, GC_TRANSFORM_SIGMOID>;
   if (ap.transform == GC_TRANSFORM_LINEAR) {

@@ -36,11 +36,13 @@ struct R2sModel {
   GC_DEV static void eval(const gc_model& md, const float* th, const float*, F&& f) {
     const int nte = EXACT ? MT : md.i[0];
     const float m0 = th[0], r2s = th[1];
-    const float kr = -1.4426950408889634f * r2s;  // -log2(e) * R2*
+    constexpr float kNegLog2e = -1.4426950408889634f;
+    const float kr = kNegLog2e * r2s;  // -log2(e) * R2*
     float e = 0.f, q = 0.f;
     if (UNI) {
-      e = __builtin_amdgcn_exp2f(kr * md.table[0]);
-      q = __builtin_amdgcn_exp2f(kr * md.f[8]);
+      // -log2(e) TE_0 and -log2(e) dTE are uniform over the launch
+      e = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.table[0]));
+      q = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.f[8]));
     }
 #pragma unroll
     for (int t = 0; t < MT; ++t) {

@@ -149,6 +149,22 @@ GC_DEV void model_adjoint(const gc_model& md, const float* th, const float* ex, 
   Model::post_grad(th, gth);
 }
 
+// Models whose gradient factors per group of measurements (kBaseAdj): for
+// measurement r of a group, d s_r / d theta_p = base_r (alpha_p + beta_p tw_r)
+// with base_r >= 0 and group constants alpha, beta, so the model accumulates
+// sum_r dl_r base_r and sum_r dl_r base_r tw_r and applies alpha and beta
+// once per group.  eval_base_adj(md, th, ex, f, gth) calls
+// f(r, s_r, base_r) -> dl_r base_r (0 for a measurement without gradient)
+// for every used slot and adds d loss / d theta to gth.
+template <class Model>
+struct has_base_adj {
+  template <class M>
+  static constexpr bool f(decltype(M::kBaseAdj)*) { return M::kBaseAdj; }
+  template <class M>
+  static constexpr bool f(...) { return false; }
+  static constexpr bool value = f<Model>(nullptr);
+};
+
 // ---- askadam block kernel -------------------------------------------------
 struct EpochScal {
   float lr;    // adam: lr*sqrt(1-b2^t)/(1-b1^t) (effective form) or lr; sgdm/rmsprop: lr
@@ -365,8 +381,11 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
           if (valid)
             for (int j = 0; j < ((md.n_meas + 15) >> 4); ++j) sw[int64_t(j) * nv] = 0u;
         }
-        Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
-          bool fin = true;
+        // loss of measurement r at model value s: residual, loss sum, the
+        // sign record (SIGNS); returns the residual and, through dl and
+        // fin, the loss derivative and whether the measurement carries one
+        auto measure = [&](int r, float s, float& dl, bool& fin) {
+          fin = true;
           float sv = s;
           if (CHECK) {
             fin = isfinite(s);
@@ -374,7 +393,8 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
           }
           const float w = HAS_W ? wr[HAS_W ? r : 0] : 1.f;
           const float res = sv - yr[r];
-          float l, dl = 0.f;
+          float l;
+          dl = 0.f;
           if (L1FAST) {
             l = w * fabsf(res);
             amin = fminf(amin, fabsf(res));
@@ -382,7 +402,6 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
             loss_fn<LOSS>(res, w, l, dl);
           }
           lsum += l;
-          if (!CHECK || fin) accum_grad<Model, L1FAST, HAS_W>(L1FAST ? copysignf(w, res) : dl, w, res, J, gth);
           if constexpr (SIGNS) {
             if (valid) {
               const int m = Model::meas(md, r);
@@ -391,7 +410,29 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
               *p = *p | (code << ((m & 15) * 2));
             }
           }
-        });
+          return res;
+        };
+        if constexpr (has_base_adj<Model>::value) {
+          // the model accumulates its gradient from d loss / d s times the
+          // measurement's base signal (>= 0): with the L1 fast-path
+          // derivative copysign(w, r) that product is one bit operation
+          Model::eval_base_adj(md, th, ex, [&](int r, float s, float base) {
+            float dl;
+            bool fin;
+            const float res = measure(r, s, dl, fin);
+            const float w = HAS_W ? wr[HAS_W ? r : 0] : 1.f;
+            if (L1FAST) return copysignf(HAS_W ? w * base : base, res);
+            return fin ? dl * base : 0.f;
+          }, gth);
+        } else {
+          Model::template eval<true>(md, th, ex, [&](int r, float s, const float* J) {
+            float dl;
+            bool fin;
+            const float res = measure(r, s, dl, fin);
+            const float w = HAS_W ? wr[HAS_W ? r : 0] : 1.f;
+            if (!CHECK || fin) accum_grad<Model, L1FAST, HAS_W>(L1FAST ? copysignf(w, res) : dl, w, res, J, gth);
+          });
+        }
       };
       accumulate(std::false_type{});
       if (!isfinite(lsum) || !(amin > 0.f)) accumulate(std::true_type{});

@@ -74,39 +74,70 @@ struct R2sModel {
 //   s = m0*sin(a)*(1-E1)/(1-cos(a)*E1)*exp(-te*r2s), E1 = exp(-TR*R1),
 //   a = B1*FA (:384-391) or the supplied true flip angle.
 // theta = {M0, R1, R2star}; register slot t + MT*f holds measurement t + nte*f.
-template <int MT, int MF>
+// EXACT: the echo and flip-angle counts equal MT and MF (no guards).
+template <int MT, int MF, bool EXACT = false>
 struct JointModel {
   static constexpr int kAskWaves = 4;  // blocked askadam kernel within 128 VGPRs
   static constexpr int NP = 3, MR = MT * MF, NE = 1;
   static constexpr bool kRelJ = false;
   static constexpr bool kFusedAdj = false, kStreamOnly = false;
+  // the blocked kernel's adjoint: one group per flip angle, base = exp(-te R2*)
+  static constexpr bool kBaseAdj = true;
   GC_DEV static void post_grad(const float*, float*) {}
-  GC_DEV static bool valid(const gc_model& md, int r) { return (r % MT) < md.i[0] && (r / MT) < md.i[1]; }
-  GC_DEV static int meas(const gc_model& md, int r) { return (r % MT) + md.i[0] * (r / MT); }
+  GC_DEV static bool valid(const gc_model& md, int r) { return EXACT || ((r % MT) < md.i[0] && (r / MT) < md.i[1]); }
+  GC_DEV static int meas(const gc_model& md, int r) { return (r % MT) + (EXACT ? MT : md.i[0]) * (r / MT); }
+
+  // exp(-te R2*) of every echo; with GC_FLAG_UNIFORM_TE by recurrence from
+  // two exponentials (relative error <= t ulp at echo t)
+  GC_DEV static void decays(const gc_model& md, float r2s, float* ete) {
+    const int nte = EXACT ? MT : md.i[0];
+    if (md.flags & GC_FLAG_UNIFORM_TE) {
+      constexpr float kNegLog2e = -1.4426950408889634f;
+      float e = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.table[0]));
+      const float q = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.f[8]));
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        ete[t] = (EXACT || t < nte) ? e : 0.f;
+        e *= q;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < MT; ++t) ete[t] = (EXACT || t < nte) ? __expf(-md.table[t] * r2s) : 0.f;
+    }
+  }
+  // steady-state amplitude per unit M0, g = sin(a)(1 - E1)/(1 - cos(a) E1),
+  // and its R1 derivative per unit M0, at flip angle fi
+  GC_DEV static void fa_terms(const gc_model& md, const float* ex, int fi, float e1, float& g, float& dg_r1) {
+    const int nte = EXACT ? MT : md.i[0];
+    const float tr = md.f[0];
+    const float alpha = (md.flags & GC_FLAG_R1R2S_TRUE_FA) ? ex[fi] : ex[0] * md.table[nte + fi];
+    float sa, ca;
+    __sincosf(alpha, &sa, &ca);
+    const float den = 1.f - ca * e1;
+    const float rden = rcpf(den);
+    g = sa * (1.f - e1) * rden;
+    // d g / d R1 = sa*TR*E1*(1-ca)/den^2
+    dg_r1 = sa * tr * e1 * (1.f - ca) * rden * rden;
+  }
+
   template <bool JAC, class F>
   GC_DEV static void eval(const gc_model& md, const float* th, const float* ex, F&& f) {
-    const int nte = md.i[0], nfa = md.i[1];
+    const int nte = EXACT ? MT : md.i[0], nfa = EXACT ? MF : md.i[1];
     const float tr = md.f[0];
     const float m0 = th[0], r1 = th[1], r2s = th[2];
     float ete[MT];
-#pragma unroll
-    for (int t = 0; t < MT; ++t) ete[t] = (t < nte) ? __expf(-md.table[t] * r2s) : 0.f;
+    decays(md, r2s, ete);
     const float e1 = __expf(-tr * r1);
 #pragma unroll
     for (int fi = 0; fi < MF; ++fi) {
-      if (fi < nfa) {
-        const float alpha = (md.flags & GC_FLAG_R1R2S_TRUE_FA) ? ex[fi] : ex[0] * md.table[nte + fi];
-        float sa, ca;
-        __sincosf(alpha, &sa, &ca);
-        const float den = 1.f - ca * e1;
-        const float rden = 1.f / den;
-        const float g = sa * (1.f - e1) * rden;  // amplitude per unit M0
+      if (EXACT || fi < nfa) {
+        float g, dg_r1;
+        fa_terms(md, ex, fi, e1, g, dg_r1);
         const float amp = m0 * g;
-        // d amp / d R1 = m0*sa*TR*E1*(1-ca)/den^2
-        const float damp_r1 = m0 * sa * tr * e1 * (1.f - ca) * rden * rden;
+        const float damp_r1 = m0 * dg_r1;
 #pragma unroll
         for (int t = 0; t < MT; ++t) {
-          if (t < nte) {
+          if (EXACT || t < nte) {
             const float s = amp * ete[t];
             float J[NP];
             if (JAC) {
@@ -117,6 +148,39 @@ struct JointModel {
             f(t + MT * fi, s, J);
           }
         }
+      }
+    }
+  }
+
+  // d s / d M0 = g ete, d s / d R1 = M0 dg_r1 ete, d s / d R2* = -te M0 g ete:
+  // per flip angle the sums A = sum dl ete and T = sum dl ete te carry the
+  // whole gradient (alpha = {g, M0 dg_r1, 0}, beta = {0, 0, -M0 g})
+  template <class F>
+  GC_DEV static void eval_base_adj(const gc_model& md, const float* th, const float* ex, F&& f, float* gth) {
+    const int nte = EXACT ? MT : md.i[0], nfa = EXACT ? MF : md.i[1];
+    const float tr = md.f[0];
+    const float m0 = th[0], r1 = th[1], r2s = th[2];
+    float ete[MT];
+    decays(md, r2s, ete);
+    const float e1 = __expf(-tr * r1);
+#pragma unroll
+    for (int fi = 0; fi < MF; ++fi) {
+      if (EXACT || fi < nfa) {
+        float g, dg_r1;
+        fa_terms(md, ex, fi, e1, g, dg_r1);
+        const float amp = m0 * g;
+        float A = -0.f, T = -0.f;
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          if (EXACT || t < nte) {
+            const float a = f(t + MT * fi, amp * ete[t], ete[t]);
+            A += a;
+            T = fmaf(a, md.table[t], T);
+          }
+        }
+        gth[0] = fmaf(g, A, gth[0]);
+        gth[1] = fmaf(m0 * dg_r1, A, gth[1]);
+        gth[2] = fmaf(-amp, T, gth[2]);
       }
     }
   }

@@ -2,5 +2,5 @@
 #include "gc_ops_impl.hpp"
 
 namespace gc {
-const ModelOps* ops_joint12x4() { return ops_table<JointModel<12, 4>>(); }
+const ModelOps* ops_joint12x4() { return ops_table<JointModel<12, 4, true>>(); }
 }  // namespace gc

@@ -370,6 +370,10 @@ class gpuJointR1R2starMapping(_ModelBase):
         off = _fill_table(md, self.te)
         _fill_table(md, np.deg2rad(self.fa.astype(np.float32)).astype(np.float32), off)
         md.n_extra = 1
+        dte = uniform_spacing(self.te)
+        if dte is not None:
+            md.flags |= _abi.GC_FLAG_UNIFORM_TE
+            md.f[8] = dte
         if nmeas is not None and nmeas != md.n_meas:
             raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The size of TE/FA does not match the image.")
         return md

@@ -55,8 +55,21 @@ GC_DEV float u01(uint32_t x) { return (float(x >> 8) + 0.5f) * (1.0f / 16777216.
 // needs no range reduction.
 GC_DEV float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
 GC_DEV float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
+// sin and cos of 2 pi x for any x (in revolutions): x - rint(x) is exact
+// and lies in [-1/2, 1/2], the hardware instructions' best range
+GC_DEV void sincos_rev(float x, float& s, float& c) {
+  const float f = x - __builtin_rintf(x);
+  s = sin_rev(f);
+  c = cos_rev(f);
+}
+constexpr float kInv2PiF = 0.15915494309189535f;
+// ln(x) = ln(2) log2(x) with the hardware v_log_f32 (x normal and positive
+// here: uniforms are at least 2^-25)
+constexpr float kLn2F = 0.6931471805599453f;
+GC_DEV float log2v(float x) { return __builtin_amdgcn_logf(x); }
 GC_DEV void box_muller(float u1, float u2, float& n0, float& n1) {
-  const float r = sqrtf(-2.0f * logf(u1));
+  // sqrt(-2 ln u1) = sqrt(-2 ln(2) log2(u1))
+  const float r = __builtin_amdgcn_sqrtf((-2.0f * kLn2F) * log2v(u1));
   n0 = r * cos_rev(u2);
   n1 = r * sin_rev(u2);
 }
@@ -67,7 +80,7 @@ GC_DEV void box_muller(float u1, float u2, float& n0, float& n1) {
 // 4 u (1 - u), exact near the tails.
 GC_DEV float norm_inv(float u) {
   const float x = 2.f * u - 1.f;
-  float w = -logf(4.f * u * (1.f - u));
+  float w = -kLn2F * log2v(4.f * u * (1.f - u));
   float p;
   if (w < 5.f) {
     w -= 2.5f;
@@ -81,7 +94,7 @@ GC_DEV float norm_inv(float u) {
     p = 0.246640727f + p * w;
     p = 1.50140941f + p * w;
   } else {
-    w = sqrtf(w) - 3.f;
+    w = __builtin_amdgcn_sqrtf(w) - 3.f;
     p = -0.000200214257f;
     p = 0.000100950558f + p * w;
     p = 0.00134934322f + p * w;

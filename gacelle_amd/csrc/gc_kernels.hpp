@@ -1182,9 +1182,10 @@ GC_DEV bool keep_sample(const McmcParams& p, int k) {
 GC_DEV float min1(float x) { return isnan(x) ? 1.f : fminf(x, 1.f); }
 
 // logP_Gaussian, utils/logP_Gaussian.m:18-22
+// (hardware reciprocal and log2: -sos / (2 n^2) - nm/2 ln(2) log2(2 pi n^2))
 GC_DEV float logp_gaussian(float sos, float noise, int nm) {
   const float n2 = noise * noise;
-  return -sos / (2.f * n2) - 0.5f * float(nm) * logf(2.f * kPiF * n2);
+  return (-0.5f * sos) * rcpf(n2) - (0.5f * kLn2F * float(nm)) * log2v(2.f * kPiF * n2);
 }
 
 template <class Model, bool HAS_W>
@@ -1404,6 +1405,7 @@ void k_gw(const gc_model* __restrict__ mdp, const McmcParams P) {
   const int64_t vv = active ? v : 0;
   const bool rb = P.ens == GC_ENSEMBLE_REDBLACK;
   const int h = W / 2;
+  const float inv_a = 1.f / P.step_size;  // the stretch z = t^2 / a (:301-304)
 
   float yr[MR];
   float wr[HAS_W ? MR : 1];
@@ -1504,7 +1506,7 @@ void k_gw(const gc_model* __restrict__ mdp, const McmcParams P) {
       }
       // stretch move (:299-304)
       const float t = (P.step_size - 1.f) * u1 + 1.f;
-      const float zz = t * t / P.step_size;
+      const float zz = t * t * inv_a;
       float xp[Q];
       bool oob = false;
       // partner walker pw of this (voxel, repetition): lane base + pw

@@ -450,7 +450,7 @@ GC_DEV DualF<N> operator*(const DualF<N>& a, const DualF<N>& b) {
 template <int N>
 GC_DEV DualF<N> operator/(const DualF<N>& a, const DualF<N>& b) {
   DualF<N> r;
-  const float ib = 1.f / b.v;
+  const float ib = rcpf(b.v);
   r.v = a.v * ib;
 #pragma unroll
   for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
@@ -485,8 +485,8 @@ GC_DEV DualF<N> operator-(const DualF<N>& a) {
 template <int N>
 GC_DEV DualF<N> dsqrt(const DualF<N>& a) {
   DualF<N> r;
-  r.v = sqrtf(a.v);
-  const float h = (r.v > 0.f) ? 0.5f / r.v : 0.f;
+  r.v = sqrtv(a.v);
+  const float h = (r.v > 0.f) ? 0.5f * rcpf(r.v) : 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
   return r;
@@ -571,7 +571,7 @@ struct McrmwiModel {
     const D tv = dmax(S0IW + S0EW + (1.f / rho) * S0MW, 1e-8f);
     const float famp = md.table[nte + fi] * ex[0];
     float sa, ca;
-    sincosf(famp, &sa, &ca);
+    sincos_rev(famp * kInv2PiF, sa, ca);
     D aIEW;
     if constexpr (EXCH) {
       const D kfr = D::var(th[4], 4);
@@ -662,10 +662,12 @@ struct McrmwiModel {
         }
       pini = th[NA + 5 + MF] + pex;
     }
-    constexpr float k2pi = 6.283185307179586f;
+    // phases in revolutions: frequencies (freq + fbkg) B0 gamma in Hz, the
+    // initial and IEW phases divided by 2 pi
     float om[3], wr[3], wi[3];
 #pragma unroll
-    for (int p = 0; p < 3; ++p) om[p] = k2pi * (freq[p] + fbkg) * B0g;
+    for (int p = 0; p < 3; ++p) om[p] = (freq[p] + fbkg) * B0g;
+    const float pinr = pini * kInv2PiF, phr = ph * kInv2PiF;
     if (md.flags & GC_FLAG_UNIFORM_TE) {
       // a loop of its own: two echoes per trip, so the step kernel's
       // read-ahead queue (two values per stream) returns to the same
@@ -676,11 +678,11 @@ struct McrmwiModel {
       for (int p = 0; p < 3; ++p) {
         float e, c, sn;
         e = __expf(-te0 * r2s[p]);
-        sincosf(te0 * om[p] + pini + (p > 0 ? ph : 0.f), &sn, &c);
+        sincos_rev(te0 * om[p] + pinr + (p > 0 ? phr : 0.f), sn, c);
         wr[p] = e * c;
         wi[p] = e * sn;
         e = __expf(-dte * r2s[p]);
-        sincosf(dte * om[p], &sn, &c);
+        sincos_rev(dte * om[p], sn, c);
         qr[p] = e * c;
         qi[p] = e * sn;
       }
@@ -702,7 +704,7 @@ struct McrmwiModel {
         for (int p = 0; p < 3; ++p) {
           float c, sn;
           const float e = __expf(-te * r2s[p]);
-          sincosf(te * om[p] + pini + (p > 0 ? ph : 0.f), &sn, &c);
+          sincos_rev(te * om[p] + pinr + (p > 0 ? phr : 0.f), sn, c);
           wr[p] = e * c;
           wi[p] = e * sn;
         }
@@ -729,7 +731,7 @@ struct McrmwiModel {
           f(t + nte * fi, re, nullptr);
           f(t + nte * fi + nte * nfa, im, nullptr);
         } else {
-          f(t + nte * fi, sqrtf(re * re + im * im), nullptr);
+          f(t + nte * fi, sqrtv(re * re + im * im), nullptr);
         }
       }, ph.v);
     }
@@ -767,9 +769,9 @@ struct McrmwiModel {
           dre = dloss(t + nte * fi, re);
           dim = dloss(t + nte * fi + nte * nfa, im);
         } else {
-          const float mag = sqrtf(re * re + im * im);
+          const float mag = sqrtv(re * re + im * im);
           const float dm = dloss(t + nte * fi, mag);
-          const float im_ = (mag > 0.f) ? 1.f / mag : 0.f;
+          const float im_ = (mag > 0.f) ? rcpf(mag) : 0.f;
           dre = dm * re * im_;
           dim = dm * im * im_;
         }

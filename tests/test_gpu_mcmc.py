@@ -219,8 +219,11 @@ def test_nexi_gl64_rician_gw_statistics_match_oracle(redblack):
     posteriors are broad and the Monte-Carlo error is calibrated, not
     assumed: the oracle run twice with different seeds gives the spread of
     posterior means that chance alone produces, and the device's departure
-    from the oracle must be of that size (median |d| at most 1.5x), unbiased,
-    with posterior SDs that agree."""
+    from the oracle must be of that size (median |d| at most 1.5x), unbiased
+    (the median departure within 4 standard errors of a sample median,
+    1.2533 x 1.4826 MAD / sqrt(n); oracle-vs-oracle and device-vs-oracle
+    pairs over three seeds put the median at up to 0.6 of the chance
+    spread), with posterior SDs that agree."""
     from gacelle_amd import models
     from gacelle_amd.mcmc import mcmc as M
 
@@ -254,6 +257,7 @@ def test_nexi_gl64_rician_gw_statistics_match_oracle(redblack):
         chance = np.median(np.abs(o2["mean"][q] - o["mean"][q]))
         dev = g["mean"][q] - o["mean"][q]
         assert np.median(np.abs(dev)) <= 1.5 * chance, (names[q], np.median(np.abs(dev)), chance)
-        assert abs(np.median(dev)) <= 0.5 * chance, (names[q], np.median(dev), chance)
+        se_med = 1.2533 * 1.4826 * np.median(np.abs(dev - np.median(dev))) / np.sqrt(nv)
+        assert abs(np.median(dev)) <= 4.0 * se_med, (names[q], np.median(dev), se_med)
         ratio = g["std"][q] / o["std"][q]
         assert 0.8 < np.median(ratio) < 1.25, (names[q], np.median(ratio))

@@ -234,7 +234,12 @@ GC_DEV float opt_step(const OptParams& o, const EpochScal& es, float g, float& m
 // The blocked kernel keeps its Adam state in this scaled form whenever it
 // runs this update (every launch of such a call does).
 GC_DEV float adam_eff_step(const OptParams& o, const EpochScal& es, float g, float& m1, float& m2) {
-  m1 = fmaf(o.beta1, m1, g);
+  // m1 <- beta1 m1 + g as a three-address v_fma_f32 (the compiler's
+  // accumulator form would put the result in g's register and copy it
+  // back to m1's every iteration)
+  float m1n;
+  asm("v_fma_f32 %0, %1, %2, %3" : "=v"(m1n) : "s"(o.beta1), "v"(m1), "v"(g));
+  m1 = m1n;
   m2 = fmaf(g, g, o.beta2 * m2);
   return -es.lr_s * m1 * rcpf(sqrtv(m2) + o.eps_s);
 }
@@ -340,6 +345,9 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   // per-iteration VALU path.
   extern __shared__ float gc_ask_ring[];
   float* const ring = gc_ask_ring + wave * (kRing * 64);
+  // voxels of this wavefront inside the volume (wave-uniform)
+  const int64_t nvw = nv - (v - lane);
+  const bool wave_full = nvw >= 64;
   const float inv_count = V.inv_count;
   float rsp[NP];  // (ub - lb) / count
 #pragma unroll
@@ -435,7 +443,9 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
         }
       };
       accumulate(std::false_type{});
-      if (!isfinite(lsum) || !(amin > 0.f)) accumulate(std::true_type{});
+      // one test for both: lsum * 0 is NaN exactly when lsum is not finite
+      // (lsum >= 0), and otherwise the sum is amin itself
+      if (!(fmaf(lsum, 0.f, amin) > 0.f)) accumulate(std::true_type{});
       Model::post_grad(th, gth);
       float den = dead_z2, gn = 0.f, step = 0.f;
       float g[NP];
@@ -469,12 +479,14 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
           z[p] = zn;
         }
       }
-      if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
       if constexpr (norms) {
+        if (!valid) { lsum = 0.f; step = 0.f; den = 0.f; gn = 0.f; }
         // four sums per iteration: loss, step^2, |z|^2, |g|^2
         const float tot = wave_sum4(lsum, step, den, gn, lane, xr);
         if (lane < 4) V.partials[((lw * V.n_epochs + k) << 2) + wave_sum4_index(lane)] = tot;
       } else {
+        // every lane parks its loss; the lanes past the volume's end (only
+        // in its last wavefront) are dropped when the ring is summed
         const int slot = k & (kRing - 1);
         ring[slot * 64 + lane] = lsum;
         if (slot == kRing - 1 || k == V.n_epochs - 1) {
@@ -489,10 +501,20 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
           float s = 0.f;
           if (kk < cnt) {
             const float4* p4 = reinterpret_cast<const float4*>(&ring[kk * 64 + part * 16]);
+            if (wave_full) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const float4 q = p4[i];
-              s += (q.x + q.y) + (q.z + q.w);
+              for (int i = 0; i < 4; ++i) {
+                const float4 q = p4[i];
+                s += (q.x + q.y) + (q.z + q.w);
+              }
+            } else {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const float4 q = p4[i];
+                const int l0 = part * 16 + 4 * i;
+                s += ((l0 < nvw ? q.x : 0.f) + (l0 + 1 < nvw ? q.y : 0.f)) +
+                     ((l0 + 2 < nvw ? q.z : 0.f) + (l0 + 3 < nvw ? q.w : 0.f));
+              }
             }
           }
           s += dpp<kDppXor1>(s);

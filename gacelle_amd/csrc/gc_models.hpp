@@ -77,7 +77,9 @@ struct R2sModel {
 // EXACT: the echo and flip-angle counts equal MT and MF (no guards).
 template <int MT, int MF, bool EXACT = false>
 struct JointModel {
-  static constexpr int kAskWaves = 4;  // blocked askadam kernel within 128 VGPRs
+  // blocked askadam kernel within 128 VGPRs (68 bytes per lane spilled);
+  // three waves without spill measured 59.1 against 57.7 ms on 2.9M voxels
+  static constexpr int kAskWaves = 4;
   static constexpr int NP = 3, MR = MT * MF, NE = 1;
   static constexpr bool kRelJ = false;
   static constexpr bool kFusedAdj = false, kStreamOnly = false;
@@ -195,7 +197,9 @@ struct JointModel {
 template <int MS, bool L2>
 struct NexiModel {
   static constexpr int NP = L2 ? 5 : 4, MR = L2 ? 2 * MS : MS, NE = 1;
-  static constexpr int kGwWaves = 4;  // GW sampler within 128 VGPRs
+  // GW sampler within 128 VGPRs (104 bytes per lane spilled); three waves
+  // without spill measured 17.9 against 16.5 s on the 500k-voxel ensemble
+  static constexpr int kGwWaves = 4;
   static constexpr bool kRelJ = false;
   static constexpr bool kFusedAdj = false, kStreamOnly = false;
   GC_DEV static void post_grad(const float*, float*) {}

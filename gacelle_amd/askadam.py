@@ -172,6 +172,8 @@ class askadam:
                 if np.all(w == 1):
                     w = None
             mdi = model.gc_model(fitting, "askadam", nmeas)
+            if hasattr(model, "model_for_extra"):
+                mdi = model.model_for_extra(mdi, extra_data)
             # the acquisition part (network parameter pointers aside) is shared
             acq = bytes(mdi)[: _abi.gc_model.mlp.offset]
             if md is None:

@@ -138,8 +138,10 @@ class mcmc:
         Q = len(names)
         if "noise" not in names or names.index("noise") != Q - 1:
             raise GacelleError(_abi.GC_ERR_UNSUPPORTED, "MCMC expects 'noise' as the last model parameter")
-        md = model.gc_model(fitting, "mcmc", nmeas)
         extra_data = varargin[0] if varargin and isinstance(varargin[0], dict) else None
+        md = model.gc_model(fitting, "mcmc", nmeas)
+        if hasattr(model, "model_for_extra"):
+            md = model.model_for_extra(md, extra_data)
         extra = model.extra_rows(extra_data, mask, vox, nv)
         cfg = _abi.gc_mcmc_cfg()
         cfg.algorithm = algo

@@ -80,6 +80,11 @@ class _ModelBase:
     def extra_rows(self, extraData, mask, vox, nv):
         return None
 
+    def model_for_extra(self, md, extraData):
+        """The model description as the caller's extraData selects it (a
+        model whose FWD switches on an extraData field overrides this)."""
+        return md
+
     def FWD(self, pars, *args):
         """Forward signal [M, Nv] of G-D parameter maps (pars: dict of [1, Nv]
         arrays) computed by the library's gc_fwd."""
@@ -87,7 +92,7 @@ class _ModelBase:
                    "lb": [self.lb[self.modelParams.index(k)] if k in self.modelParams else 0 for k in pars],
                    "ub": [self.ub[self.modelParams.index(k)] if k in self.modelParams else 1 for k in pars]}
         extra = args[0] if args and isinstance(args[0], dict) else None
-        md = self.gc_model(fitting, "askadam", None)
+        md = self.model_for_extra(self.gc_model(fitting, "askadam", None), extra)
         first = np.asarray(next(iter(pars.values())))
         nv = first.size
         theta = np.concatenate([np.asarray(pars[k], dtype=np.float32).reshape(-1, nv) for k in fitting["modelParams"]],
@@ -378,8 +383,32 @@ class gpuJointR1R2starMapping(_ModelBase):
             raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH, "The size of TE/FA does not match the image.")
         return md
 
+    def model_for_extra(self, md, extraData):
+        """FWD takes extraData.trueFlipAngle, when given, in place of
+        b1 .* FA (R1R2s/gpuJointR1R2starMapping.m:387-391): one extra row per
+        flip angle (radians) and GC_FLAG_R1R2S_TRUE_FA."""
+        if extraData is not None and extraData.get("trueFlipAngle") is not None:
+            md.flags |= _abi.GC_FLAG_R1R2S_TRUE_FA
+            md.n_extra = self.fa.size
+        return md
+
     def extra_rows(self, extraData, mask, vox, nv):
-        """extraData.b1 masked to G-D (:264); ones when absent (:466-468)."""
+        """extraData.b1 masked to G-D (:264); ones when absent (:466-468).
+        With extraData.trueFlipAngle (radians; an N-D map whose last
+        dimension is the flip angle, [X, Y, Z, Nfa] or [X, Y, Z, 1, Nfa], or
+        already G-D, [Nfa, Nv]) its rows, one per flip angle."""
+        nfa = self.fa.size
+        if extraData is not None and extraData.get("trueFlipAngle") is not None:
+            a = np.asarray(extraData["trueFlipAngle"], dtype=np.float32)
+            if a.size != nfa * nv and a.size != nfa * int(np.prod(np.shape(mask))):
+                raise GacelleError(_abi.GC_ERR_INPUT_MISMATCH,
+                                   "extraData.trueFlipAngle must hold one angle per voxel and flip angle.")
+            if a.ndim == 2 and a.shape == (nfa, nv):
+                return np.ascontiguousarray(a)
+            if a.size == nfa * nv:
+                return np.ascontiguousarray(a.reshape(nv, nfa, order="F").T)
+            a = a.reshape(-1, nfa, order="F")
+            return np.ascontiguousarray(a[vox].T)
         if extraData is None or "b1" not in (extraData or {}):
             return np.ones((1, nv), np.float32)
         b1 = np.asarray(extraData["b1"], dtype=np.float32)

@@ -106,7 +106,9 @@ enum { GC_QUAD_TRAPZ = 0, GC_QUAD_TABLE = 1 };
  *   R2STAR  : i[0]=n_te; table[0:n_te]=TE [s].              P=2 {M0,R2star}
  *   JOINT   : i[0]=n_te, i[1]=n_fa; f[0]=TR [s];
  *             table[0:n_te]=TE [s], table[n_te:n_te+n_fa]=FA [rad];
- *             extra row 0 = B1 (or n_fa rows of true flip angle [rad]).
+ *             extra row 0 = B1 (or, with GC_FLAG_R1R2S_TRUE_FA, n_fa rows
+ *             of true flip angle [rad]); f[8] = TE spacing with
+ *             GC_FLAG_UNIFORM_TE.
  *                                                        P=3 {M0,R1,R2star}
  *   NEXI    : i[0]=n_shell, i[1]=lmax (0|2), i[2]=n_nodes, i[3]=quad kind;
  *             table[0:ns]=b [ms/um^2], table[ns:2ns]=Delta [ms],

@@ -295,3 +295,32 @@ def test_reused_cached_blocks_give_identical_results():
         np.testing.assert_array_equal(first[key], again[key])
     assert (first["n_iteration"], first["min_iteration"]) == (again["n_iteration"], again["min_iteration"])
     assert runtime.context() is not None
+
+
+@pytest.mark.parametrize("true_fa", [False, True])
+def test_joint_askadam_unequal_echoes_matches_oracle(true_fa):
+    """The joint model's grouped blocked adjoint on the guarded variant
+    (5 unequally spaced echoes x 3 flip angles: per-echo exponentials, runtime
+    echo and flip-angle counts), with b1 or with extraData.trueFlipAngle
+    (R1R2s/gpuJointR1R2starMapping.m:387-391), against the oracle."""
+    te = np.array([2.0, 3.5, 6.0, 9.0, 13.0]) * 1e-3
+    fa = np.array([4.0, 12.0, 20.0])
+    m = models.gpuJointR1R2starMapping(te, 25e-3, fa)
+    f = askadam.check_set_default_basic(dict(modelParams=["M0", "R1", "R2star"], lb=[0, 0.1, 0.1], ub=[2, 10, 200],
+                                             **no_stop(iteration=300, lossFunction="L2", initialLearnRate=1e-2)))
+    nv = 3000
+    rng = np.random.default_rng(14)
+    b1 = rng.uniform(0.8, 1.2, (1, nv))
+    extra_data = {"trueFlipAngle": (b1.T * np.deg2rad(fa)).T} if true_fa else {"b1": b1}
+    md = m.model_for_extra(m.gc_model(f, "askadam", te.size * fa.size), extra_data)
+    assert not (md.flags & _abi.GC_FLAG_UNIFORM_TE)
+    ex = m.extra_rows(extra_data, np.ones(nv, bool), np.arange(nv), nv).astype(np.float32)
+    th = np.stack([rng.uniform(0.5, 2, nv), rng.uniform(0.5, 2, nv), rng.uniform(10, 60, nv)])
+    s = orc.fwd_value(md, th, ex.astype(np.float64))
+    y = (s + 0.01 * rng.standard_normal(s.shape)).astype(np.float32)
+    cfg = askadam().make_cfg(f, None)
+    z0 = logit_start(np.stack([np.ones(nv), np.ones(nv), np.full(nv, 30.0)]), md)
+    got = gpu_abi.askadam(md, cfg, [dict(y=y, z0=z0, extra=ex)])[0]
+    ref = orc.askadam(md, cfg, y, z0, extra=ex.astype(np.float64))
+    np.testing.assert_allclose(got["loss"], ref["loss"], rtol=1e-5)
+    np.testing.assert_allclose(got["z_final"], ref["z_final"], rtol=1e-4, atol=1e-4)

@@ -257,3 +257,42 @@ def test_gpunexirice_is_the_rician_model_with_the_reference_argument_order(monke
     models.gpuNEXI(b, D).fit(dwi, mask, {"start": "default"}, p0, None)
     np.testing.assert_allclose(np.asarray(seen["pars0"]["fa"]), 0.3)
     assert "gpuMCRMWI" in ga.__all__ and ga.gpuMCRMWI is models.gpuMCRMWI
+
+
+def test_joint_true_flip_angle_rows_and_flag():
+    """extraData.trueFlipAngle replaces b1 .* FA in FWD
+    (R1R2s/gpuJointR1R2starMapping.m:387-391): one extra row per flip angle
+    in G-D order and GC_FLAG_R1R2S_TRUE_FA; given as b1 .* FA the oracle's
+    signal equals the b1 form."""
+    from gacelle_amd import _abi, utils
+    from gacelle_amd.askadam import askadam as A
+    from oracle import oracle as orc
+
+    te = np.array([2.0, 3.5, 6.0, 9.0, 13.0]) * 1e-3
+    fa = np.array([4.0, 12.0, 20.0])
+    m = models.gpuJointR1R2starMapping(te, 25e-3, fa)
+    dims = (4, 3, 2)
+    rng = np.random.default_rng(3)
+    mask = rng.uniform(size=dims) > 0.3
+    vox = utils.find_mask(mask)
+    nv = vox.size
+    b1 = rng.uniform(0.8, 1.2, dims)
+    tfa = b1[..., None] * np.deg2rad(fa)  # [X, Y, Z, Nfa]
+    f = A.check_set_default_basic(dict(modelParams=["M0", "R1", "R2star"], lb=[0, 0.1, 0.1], ub=[2, 10, 200]))
+    md = m.model_for_extra(m.gc_model(f, "askadam", None), {"trueFlipAngle": tfa})
+    assert md.flags & _abi.GC_FLAG_R1R2S_TRUE_FA and md.n_extra == fa.size
+    rows = m.extra_rows({"trueFlipAngle": tfa}, mask, vox, nv)
+    assert rows.shape == (fa.size, nv)
+    np.testing.assert_array_equal(rows, tfa.reshape(-1, fa.size, order="F")[vox].T.astype(np.float32))
+    # G-D input ([1, Nv, 1, 1, Nfa], the reference's gpu_reshape_ND2GD_struct form)
+    gd = rows.T.reshape(1, nv, 1, 1, fa.size)
+    np.testing.assert_array_equal(m.extra_rows({"trueFlipAngle": gd}, mask, vox, nv), rows)
+    # without the field: the b1 row and no flag
+    md_b1 = m.model_for_extra(m.gc_model(f, "askadam", None), {"b1": b1})
+    assert not (md_b1.flags & _abi.GC_FLAG_R1R2S_TRUE_FA) and md_b1.n_extra == 1
+    b1_row = m.extra_rows({"b1": b1}, mask, vox, nv)
+    th = np.stack([rng.uniform(0.5, 2, nv), rng.uniform(0.5, 2, nv), rng.uniform(10, 60, nv)])
+    np.testing.assert_allclose(orc.fwd_value(md, th, rows.astype(np.float64)),
+                               orc.fwd_value(md_b1, th, b1_row.astype(np.float64)), rtol=1e-6)
+    with pytest.raises(GacelleError):
+        m.extra_rows({"trueFlipAngle": tfa[..., :2]}, mask, vox, nv)

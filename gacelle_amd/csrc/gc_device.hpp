@@ -47,8 +47,10 @@ GC_DEV void draw4(uint64_t seed, uint32_t it, uint64_t chain, uint32_t purpose, 
   philox4x32_10(it, uint32_t(chain), uint32_t(chain >> 32), purpose, uint32_t(seed), uint32_t(seed >> 32), out);
 }
 
-// uint32 -> (0,1): ((x >> 8) + 0.5) * 2^-24, exact in float.
-GC_DEV float u01(uint32_t x) { return (float(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+// uint32 -> (0,1): ((x >> 8) + 0.5) * 2^-24, formed as one FMA
+// (x >> 8) * 2^-24 + 2^-25: the product is exact, so the FMA rounds the same
+// exact value once, as the add-then-scale form does.
+GC_DEV float u01(uint32_t x) { return fmaf(float(x >> 8), 1.0f / 16777216.0f, 1.0f / 33554432.0f); }
 
 // Box-Muller pair from two uniforms: r = sqrt(-2 ln u1), angle 2 pi u2.
 // v_sin_f32 / v_cos_f32 take their argument in revolutions, so the angle

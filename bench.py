@@ -868,6 +868,32 @@ def main():
             replay_draw_bytes=int(normals.numel() * 4 + uniforms.numel() * 4))
         log(f"MH replay vs native ({nsub} voxels): {t_rep * 1e3:.1f} ms vs {t_nat * 1e3:.1f} ms")
         del normals, uniforms
+        # the plain per-op GPU path of the reference's MH (one kernel per
+        # operation, repetitions in turn) on the first 500,000 voxels, beside
+        # the fused sampler's result at the same voxels
+        if not args.skip_tape:
+            from gacelle_amd.tape import mh_r2star_tape
+
+            ntp = min(500_000, nv)
+            te_t = torch.tensor(TE, dtype=torch.float32, device=device)
+            gen_t = torch.Generator(device=device)
+            gen_t.manual_seed(12)
+            yt, xt = y[:, :ntp].contiguous(), x0[:, :ntp].contiguous()
+            mh_r2star_tape(yt, te_t, xt, [0, 0.1, 0.001], [2, 200, 0.1], [0.01, 1, 0.005], 10, 1, 20, 1, gen_t)
+            nburn = int(args.mh_iterations * 0.1)
+            t_tp, mt = timed(lambda: mh_r2star_tape(yt, te_t, xt, [0, 0.1, 0.001], [2, 200, 0.1], [0.01, 1, 0.005],
+                                                     args.mh_iterations, args.mh_chains, 20, nburn, gen_t))
+            mt = mt.cpu().numpy()
+            steps_tp = float(ntp) * args.mh_chains * args.mh_iterations
+            mh["tape_subset"] = dict(
+                voxels=ntp, samples_per_s=steps_tp / t_tp, seconds=t_tp,
+                speedup_fused_vs_tape=mh["samples_per_s"] / (steps_tp / t_tp),
+                posterior_mean_median_rel_err=dict(M0=float(np.median(np.abs(mt[0] - s0[:ntp]) / s0[:ntp])),
+                                                   R2star=float(np.median(np.abs(mt[1] - r2[:ntp]) / r2[:ntp]))),
+                fused_posterior_mean_median_rel_err=dict(
+                    M0=float(np.median(np.abs(m[0, :ntp] - s0[:ntp]) / s0[:ntp])),
+                    R2star=float(np.median(np.abs(m[1, :ntp] - r2[:ntp]) / r2[:ntp]))))
+            log(f"MH tape ({ntp} voxels): {t_tp:.2f} s, {steps_tp / t_tp:.3e} chain-steps/s")
 
     # ---- further workloads, reported beside the line ---------------------------
     extra = {}

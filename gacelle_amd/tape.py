@@ -41,3 +41,46 @@ def askadam_r2star_tape(y, te, lb, ub, z0, iterations, lr=1e-3, beta1=0.9, beta2
                 lr_eff = lr * math.sqrt(1 - beta2 ** epoch) / (1 - beta1 ** epoch)
                 z.sub_(lr_eff * m / (torch.sqrt(v) + eps))
     return z, float(last)
+
+
+def mh_r2star_tape(y, te, x0, lb, ub, steps, iterations, reps, thinning, n_burnin, generator):
+    """Metropolis-Hastings (utils/mcmc.m:82-190) for the mono-exponential
+    model as the reference executes it on the GPU: per iteration a normal
+    draw for every variable, the proposal, the out-of-bounds flag and clamp,
+    the forward signal and its sum of squares, logP_Gaussian
+    (utils/logP_Gaussian.m:18-22), a uniform draw and the acceptance, each
+    its own kernel over every voxel; repetitions run one after the other.
+
+    y [M, Nv], te [M], x0 [3, Nv] (M0, R2*, noise) on the GPU.  Returns the
+    posterior mean [3, Nv] over the retained samples of every repetition."""
+    M, nv = y.shape
+    dev = y.device
+    te = te.reshape(-1, 1)
+    lbt = torch.tensor(lb, dtype=torch.float32, device=dev).reshape(3, 1)
+    ubt = torch.tensor(ub, dtype=torch.float32, device=dev).reshape(3, 1)
+    st = torch.tensor(steps, dtype=torch.float32, device=dev).reshape(3, 1)
+
+    def logp(x):
+        s = x[0:1] * torch.exp(-te * x[1:2])
+        sos = ((s - y) ** 2).sum(0)
+        n2 = x[2] * x[2]
+        return -sos / (2.0 * n2) - 0.5 * M * torch.log(2.0 * math.pi * n2)
+
+    sums = torch.zeros((3, nv), dtype=torch.float64, device=dev)
+    n = 0
+    for _ in range(reps):
+        x = torch.minimum(torch.maximum(x0, lbt), ubt)
+        lp = logp(x)
+        for k in range(1, iterations + 1):
+            xp = x + st * torch.randn((3, nv), generator=generator, device=dev)
+            oob = ((xp < lbt) | (xp > ubt)).any(0)
+            xp = torch.minimum(torch.maximum(xp, lbt), ubt)
+            lpp = logp(xp)
+            ratio = torch.nan_to_num(torch.exp(lpp - lp), nan=1.0).clamp(max=1.0)
+            acc = (ratio > torch.rand(nv, generator=generator, device=dev)) & ~oob
+            x = torch.where(acc, xp, x)
+            lp = torch.where(acc, lpp, lp)
+            if k > n_burnin and (k - n_burnin + 1) % thinning == 0:
+                sums += x.double()
+                n += 1
+    return (sums / max(n, 1)).float()

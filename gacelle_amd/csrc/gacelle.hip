@@ -314,7 +314,10 @@ int gc_fwd_adj_noalloc(gc_ctx* ctx, const gc_model* md, int32_t loss, int32_t tr
   double* part = reinterpret_cast<double*>(base + kNoAllocModelBytes);
   const unsigned nb = nblocks(nv);
   // every block writes its own partial, and the reduction reads only those
-  GC_HIP(ctx, hipMemcpyAsync(dmd, md, sizeof(gc_model), hipMemcpyHostToDevice, ctx->stream));
+  // the model with its derived fields, as upload_model writes it
+  ctx->h_model = *md;
+  derive_model(ctx->h_model);
+  GC_HIP(ctx, hipMemcpyAsync(dmd, &ctx->h_model, sizeof(gc_model), hipMemcpyHostToDevice, ctx->stream));
   rc = ops->fwd_adj(ctx, dmd, loss, transform, x, y, w, extra, nv, scale, grad, loss_sum ? part : nullptr,
                     residual, nullptr);
   if (rc) return rc;

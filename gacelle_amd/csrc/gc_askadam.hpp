@@ -11,7 +11,9 @@
 // through HBM): it carries the spatial TV regulariser, whose stencil needs
 // the neighbours' previous parameters, and models whose data are streamed.
 #pragma once
+#include <cfloat>
 #include <chrono>
+#include <cstring>
 
 #include "gc_host.hpp"
 
@@ -98,8 +100,8 @@ struct AskDriver {
   const gc_model* dmd = nullptr;
   bool has_w = false;
   std::vector<VolRun> run;
-  DevBuf d_vols, d_es, d_sums;
-  HostBuf h_vols, h_es, h_sums;
+  DevBuf d_vols, d_es, d_sums, d_rows;
+  HostBuf h_vols, h_es, h_rows;
   int max_chunks = 1;
   std::vector<double> row_sums;  // [vol][K*kSum] after fold_sums
   bool desc_in_flight = false;  // an async copy may still read h_vols/h_es
@@ -170,22 +172,9 @@ struct AskDriver {
     ev_used = 0;
   }
 
-  // add the chunk sums of every row in a fixed order (after a stream sync)
-  void fold_sums(const std::vector<int>& n_ep) {
-    const double* h = h_sums.as<double>();
-    const int npv = npv_count();
-    for (int i = 0; i < n_vol; ++i) {
-      const int C = run[i].chunks;
-      double* dst = row_sums.data() + size_t(i) * K * kSum;
-      std::fill(dst, dst + size_t(K) * kSum, 0.0);
-      for (int r = 0; r < n_ep[i] * npv; ++r) {
-        const double* src = h + size_t(i) * K * kSum * max_chunks + size_t(r) * C;
-        double t = 0.0;
-        for (int c = 0; c < C; ++c) t += src[c];
-        dst[(r / npv) * kSum + (r % npv)] = t;
-      }
-    }
-  }
+  // the per-iteration sums of every volume, folded on the device by
+  // k_fold_chunks (after a stream sync)
+  void fold_sums() { std::memcpy(row_sums.data(), h_rows.p, sizeof(double) * K * kSum * n_vol); }
 
   // partial sums per iteration: all five in the step form, else loss,
   // step^2, |z|^2, |g|^2 or the loss alone
@@ -348,7 +337,8 @@ struct AskDriver {
     GC_HIP(ctx, d_es.alloc(sizeof(EpochScal) * K * n_vol));
     GC_HIP(ctx, h_es.alloc(sizeof(EpochScal) * K * n_vol));
     GC_HIP(ctx, d_sums.alloc(sizeof(double) * K * kSum * max_chunks * n_vol));
-    GC_HIP(ctx, h_sums.alloc(sizeof(double) * K * kSum * max_chunks * n_vol));
+    GC_HIP(ctx, d_rows.alloc(sizeof(double) * K * kSum * n_vol));
+    GC_HIP(ctx, h_rows.alloc(sizeof(double) * K * kSum * n_vol));
     row_sums.assign(size_t(K) * kSum * n_vol, 0.0);
     if (step_mode) {
       GC_HIP(ctx, d_steps.alloc(sizeof(StepVol) * K * n_vol));
@@ -480,12 +470,40 @@ struct AskDriver {
       } else {
         s.lr = float(lr * std::sqrt(bc2) / bc1);
         s.c1 = s.c2 = 1.f;
-        s.lr_s = float(lr * std::sqrt(bc2) / bc1 * (1.0 - cfg.beta1) / std::sqrt(1.0 - cfg.beta2));
+        // (a zero step gives 1 / ls = FLT_MAX: a zero update, not 0 * inf)
+        const double ls = lr * std::sqrt(bc2) / bc1 * (1.0 - cfg.beta1) / std::sqrt(1.0 - cfg.beta2);
+        const double fmax = double(FLT_MAX);
+        s.inv_ls = float(ls != 0.0 ? std::max(-fmax, std::min(fmax, 1.0 / ls)) : fmax);
       }
     } else {
       s.lr = float(lr);
     }
     return s;
+  }
+
+  // Adam's folded effective form (adam_eff_den) for this call: the form,
+  // beta2 < 1, at most kEpsSlots parameters and a positive finite
+  // rs_p = (ub_p - lb_p) inv_count (as the kernel forms it) for every
+  // parameter at the given inverse counts
+  template <class Model>
+  bool adam_fold(const float* inv_counts, int n) const {
+    if (cfg.transform != GC_TRANSFORM_SIGMOID || cfg.optimiser != GC_OPT_ADAM ||
+        cfg.adam_form == GC_ADAM_KINGMA_BA || !(float(cfg.beta2) < 1.f) || Model::NP > kEpsSlots)
+      return false;
+    for (int i = 0; i < n; ++i)
+      for (int p = 0; p < Model::NP; ++p) {
+        const float rs = (md.ub[p] - md.lb[p]) * inv_counts[i];
+        if (!(rs > 0.f) || !std::isfinite(rs)) return false;
+      }
+    return true;
+  }
+  // eps_s / (ls rs_p) of the folded Adam step, per parameter
+  void fold_eps(EpochScal& s, float inv_count, int np) const {
+    const double eps_s = cfg.eps / std::sqrt(1.0 - cfg.beta2);
+    for (int p = 0; p < np && p < kEpsSlots; ++p) {
+      const float rs = (md.ub[p] - md.lb[p]) * inv_count;
+      s.eps_ls[p] = float(std::min(double(FLT_MAX), eps_s * double(s.inv_ls) / double(rs)));
+    }
   }
 
   OptParams opt_params() const {
@@ -498,7 +516,6 @@ struct AskDriver {
     o.momentum = float(cfg.momentum);
     o.rms_decay = float(cfg.rms_decay);
     o.rms_eps = float(cfg.rms_eps);
-    o.eps_s = float(cfg.eps / std::sqrt(1.0 - cfg.beta2));
     return o;
   }
 
@@ -517,14 +534,21 @@ struct AskDriver {
     return GC_OK;
   }
 
-  // FP64 sums of the block partials of every volume that ran
+  // FP64 sums of the block partials of every volume that ran, folded into
+  // d_rows ([vol][K*kSum], zero for the volumes that did not run)
   int reduce_partials(const std::vector<int>& n_ep) {
+    const int npv = npv_count();
     for (int i = 0; i < n_vol; ++i) {
-      if (n_ep[i] <= 0) continue;
-      const int nrows = n_ep[i] * npv_count();
-      hipLaunchKernelGGL(k_reduce_rows, dim3(run[i].chunks, (nrows + 63) / 64), dim3(kBlock), 0, ctx->stream,
-                         run[i].partials.as<float>(), int64_t(run[i].nprod), nrows, int64_t(kReduceChunk),
-                         d_sums.as<double>() + size_t(i) * K * kSum * max_chunks);
+      const int nrows = n_ep[i] > 0 ? n_ep[i] * npv : 0;
+      double* chunk_sums = d_sums.as<double>() + size_t(i) * K * kSum * max_chunks;
+      if (nrows > 0) {
+        launch_reduce_rows(run[i].partials.as<float>(), int64_t(run[i].nprod), nrows, run[i].chunks, chunk_sums,
+                           ctx->stream);
+        GC_HIP(ctx, hipGetLastError());
+      }
+      const int nout = K * kSum;
+      hipLaunchKernelGGL(k_fold_chunks, dim3(nblocks(nout)), dim3(kBlock), 0, ctx->stream, chunk_sums,
+                         run[i].chunks, nrows, npv, kSum, nout, d_rows.as<double>() + size_t(i) * K * kSum);
       GC_HIP(ctx, hipGetLastError());
     }
     return GC_OK;
@@ -553,6 +577,9 @@ struct AskDriver {
     EpochScal* hes = h_es.as<EpochScal>();
     bool want_signs = false;
     int blk0 = 0;
+    std::vector<float> invc(n_vol);
+    for (int i = 0; i < n_vol; ++i) invc[i] = float(1.0 / (double(md.n_meas) * run[i].nmean));
+    const bool fold = adam_fold<Model>(invc.data(), n_vol);
     for (int i = 0; i < n_vol; ++i) {
       VolRun& R = run[i];
       VolDev d{};
@@ -575,11 +602,15 @@ struct AskDriver {
       d.epoch0 = e0[i];
       d.n_epochs = n_ep[i];
       d.iteration = iteration_cap;
-      d.inv_count = float(1.0 / (double(md.n_meas) * R.nmean));
+      d.inv_count = invc[i];
       d.signs = vols[i].sign_trace_dev;
       if (d.signs) want_signs = true;
       hv[i] = d;
-      for (int k = 0; k < n_ep[i]; ++k) hes[size_t(i) * K + k] = epoch_scal(e0[i] + k);
+      for (int k = 0; k < n_ep[i]; ++k) {
+        EpochScal& es = hes[size_t(i) * K + k];
+        es = epoch_scal(e0[i] + k);
+        if (fold) fold_eps(es, d.inv_count, Model::NP);
+      }
       blk0 += R.nblk;
       if (n_ep[i] > 0) R.launches++;
     }
@@ -591,6 +622,7 @@ struct AskDriver {
     ap.n_vol = n_vol;
     ap.transform = cfg.transform;
     ap.need_norms = npv_count() == 4 ? 1 : 0;
+    ap.adam_eff = fold ? 1 : 0;
     ap.o = opt_params();
     hipEvent_t ev_a, ev_b;
     int rce = next_events(ev_a, ev_b);
@@ -666,8 +698,13 @@ struct AskDriver {
       GC_HIP(ctx, dsum.alloc(sizeof(double) * npv * chunks));
       GC_HIP(ctx, hipMemsetAsync(mt.p, 0, pbytes, ctx->stream));
       GC_HIP(ctx, hipMemsetAsync(vt.p, 0, pbytes, ctx->stream));
+      const float inv_trial = float(1.0 / (double(md.n_meas) * double(nvt)));
+      const bool fold = adam_fold<Model>(&inv_trial, 1);
       std::vector<EpochScal> hes(kTrialMax);
-      for (int k = 0; k < kTrialMax; ++k) hes[k] = epoch_scal(1 + k);
+      for (int k = 0; k < kTrialMax; ++k) {
+        hes[k] = epoch_scal(1 + k);
+        if (fold) fold_eps(hes[k], inv_trial, Model::NP);
+      }
       GC_HIP(ctx, hipMemcpyAsync(est.p, hes.data(), sizeof(EpochScal) * kTrialMax, hipMemcpyHostToDevice,
                                  ctx->stream));
       VolDev d{};
@@ -687,12 +724,13 @@ struct AskDriver {
       d.nblk = nbt;
       d.epoch0 = 1;
       d.iteration = INT32_MAX;
-      d.inv_count = float(1.0 / (double(md.n_meas) * double(nvt)));
+      d.inv_count = inv_trial;
       AskParams ap{};
       ap.vols = dvt.as<VolDev>();
       ap.n_vol = 1;
       ap.transform = cfg.transform;
       ap.need_norms = npv == 4 ? 1 : 0;
+      ap.adam_eff = fold ? 1 : 0;
       ap.o = opt_params();
       hipEvent_t e0, e1;
       GC_HIP(ctx, hipEventCreate(&e0));
@@ -707,9 +745,8 @@ struct AskDriver {
         int e = launch_fused<Model, LOSS, HAS_W>(false, nbt, bs, ap, e0, e1);
         if (e) return e;
         if (cycle) {
-          hipLaunchKernelGGL(k_reduce_rows, dim3(chunks, (n * npv + 63) / 64), dim3(kBlock), 0, ctx->stream,
-                             pt.as<float>(), int64_t(nbt) * (kBlock / 64), n * npv, int64_t(kReduceChunk),
-                             dsum.as<double>());
+          launch_reduce_rows(pt.as<float>(), int64_t(nbt) * (kBlock / 64), n * npv, chunks, dsum.as<double>(),
+                             ctx->stream);
           GC_HIP(ctx, hipGetLastError());
           GC_HIP(ctx, hipMemcpyAsync(hsum.data(), dsum.p, sizeof(double) * hsum.size(), hipMemcpyDeviceToHost,
                                      ctx->stream));
@@ -1032,7 +1069,7 @@ struct AskDriver {
     const int P = md.n_par;
     const int iter = cfg.iteration;
     const double* sums = row_sums.data();
-    const size_t sum_bytes = sizeof(double) * K * kSum * max_chunks * n_vol;
+    const size_t sum_bytes = sizeof(double) * K * kSum * n_vol;
     std::vector<int> e0(n_vol), nep(n_vol), in(n_vol), out(n_vol), zl(n_vol);
 
     if (iter <= 0) {
@@ -1040,9 +1077,9 @@ struct AskDriver {
       for (int i = 0; i < n_vol; ++i) { e0[i] = 1; nep[i] = 1; in[i] = 0; out[i] = 1; zl[i] = 0; }
       int rc = launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, 0);
       if (rc) return rc;
-      GC_HIP(ctx, hipMemcpyAsync(h_sums.p, d_sums.p, sum_bytes, hipMemcpyDeviceToHost, ctx->stream));
+      GC_HIP(ctx, hipMemcpyAsync(h_rows.p, d_rows.p, sum_bytes, hipMemcpyDeviceToHost, ctx->stream));
       GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      fold_sums(nep);
+      fold_sums();
       for (int i = 0; i < n_vol; ++i) {
         VolRun& R = run[i];
         const double* s = sums + size_t(i) * K * kSum;
@@ -1072,11 +1109,11 @@ struct AskDriver {
       }
       int rc = launch<Model, LOSS, HAS_W>(e0, nep, in, out, zl, iter);
       if (rc) return rc;
-      GC_HIP(ctx, hipMemcpyAsync(h_sums.p, d_sums.p, sum_bytes, hipMemcpyDeviceToHost, ctx->stream));
+      GC_HIP(ctx, hipMemcpyAsync(h_rows.p, d_rows.p, sum_bytes, hipMemcpyDeviceToHost, ctx->stream));
       GC_HIP(ctx, hipStreamSynchronize(ctx->stream));
       desc_in_flight = false;
       collect_events();
-      fold_sums(nep);
+      fold_sums();
       any = false;
       for (int i = 0; i < n_vol; ++i) {
         VolRun& R = run[i];

@@ -208,4 +208,38 @@ GC_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 // rounded expansion
 GC_DEV float sqrtv(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// r[i] = 1 / d[i] for N values with one v_rcp_f32 (the transcendental unit
+// bounds the fitting loops): the prefix products P_i = d_0 .. d_i, R = 1 / P,
+// then r_i = P_{i-1} R_i and R_{i-1} = R_i d_i downwards (3 (N-1) products,
+// a few ulp).  A product outside [2^-100, 2^100] (or not a number) takes one
+// reciprocal per value instead, in a branch that the wavefront skips when
+// no lane needs it.
+template <int N>
+GC_DEV void rcp_batch(const float* d, float* r) {
+  if constexpr (N == 1) {
+    r[0] = rcpf(d[0]);
+  } else {
+    float pre[N];
+    pre[0] = d[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) pre[i] = pre[i - 1] * d[i];
+    float R = rcpf(pre[N - 1]);
+#pragma unroll
+    for (int i = N - 1; i > 0; --i) {
+      r[i] = pre[i - 1] * R;
+      R *= d[i];
+    }
+    r[0] = R;
+    const bool ok = pre[N - 1] >= 0x1p-100f && pre[N - 1] <= 0x1p100f;
+    if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+      // (the empty volatile asm keeps the reciprocals inside the branch:
+      // speculated, they would run on every iteration)
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+        if (!ok) r[i] = rcpf(d[i]);
+    }
+  }
+}
+
 }  // namespace gc

@@ -30,6 +30,7 @@ struct gc_ctx {
   bool own_stream = false;
   std::string err;
   gc_model* d_model = nullptr;  // device copy of the last model description
+  gc_model h_model{};           // its host staging copy (with the derived fields)
   int use_graphs = 1;           // GC_OPT_GRAPHS: step-form blocks replayed from hipGraphs
   int mcmc_block = 0;           // GC_OPT_MCMC_BLOCK: MCMC threads per block (0 = autotuned)
   // sampler block sizes picked by the autotuner, per model variant and
@@ -234,9 +235,34 @@ inline unsigned nblocks(int64_t n, int b = kBlock) { return unsigned((n + b - 1)
 // Model descriptions are read by kernels from device memory (uniform
 // addresses: scalar loads), not from the kernel-argument segment, so that
 // table lookups with a loop index never force a private copy.
+//
+// Derived fields of the device copy, for R2* and joint R1-R2* with equally
+// spaced echo times: when the spacing dTE = k TE_0 for k = 1 or 2 (relative
+// 1e-6), i[7] = k and f[9] = -log2(e) TE_0, and both decay factors come from
+// one exponential; otherwise i[7] = 0.  Any caller values in these slots are
+// replaced.
+inline void derive_model(gc_model& m) {
+  if (m.model_id != GC_MODEL_R2STAR && m.model_id != GC_MODEL_JOINT_R1R2STAR) return;
+  m.i[7] = 0;
+  m.f[9] = 0.f;
+  if (!(m.flags & GC_FLAG_UNIFORM_TE)) return;
+  const double te0 = m.table[0], dte = m.f[8];
+  if (!(te0 > 0.0) || !(dte > 0.0)) return;
+  for (int k = 1; k <= 2; ++k) {
+    if (std::fabs(te0 * k - dte) <= 1e-6 * dte) {
+      m.i[7] = k;
+      m.f[9] = float(-1.4426950408889634 * te0);
+      return;
+    }
+  }
+}
+
 inline int upload_model(gc_ctx* ctx, const gc_model* md, const gc_model** out) {
   if (!ctx->d_model) GC_HIP(ctx, hipMalloc(&ctx->d_model, sizeof(gc_model)));
-  GC_HIP(ctx, hipMemcpyAsync(ctx->d_model, md, sizeof(gc_model), hipMemcpyHostToDevice, ctx->stream));
+  // (a pageable source: the copy has read it when the call returns)
+  ctx->h_model = *md;
+  derive_model(ctx->h_model);
+  GC_HIP(ctx, hipMemcpyAsync(ctx->d_model, &ctx->h_model, sizeof(gc_model), hipMemcpyHostToDevice, ctx->stream));
   *out = ctx->d_model;
   return GC_OK;
 }

@@ -166,11 +166,17 @@ struct has_base_adj {
 };
 
 // ---- askadam block kernel -------------------------------------------------
+// parameters of the folded Adam step (the blocked models have at most five)
+constexpr int kEpsSlots = 8;
+
 struct EpochScal {
   float lr;    // adam: lr*sqrt(1-b2^t)/(1-b1^t) (effective form) or lr; sgdm/rmsprop: lr
   float c1;    // 1/(1-b1^t) (Kingma-Ba form)
   float c2;    // 1/(1-b2^t) (Kingma-Ba form)
-  float lr_s;  // adam, effective form: lr (1-b1) / sqrt(1-b2), the step on scaled moments
+  // adam, effective form (adam_eff_den): 1 / ls for the step
+  // ls = lr (1-b1) / sqrt(1-b2), and per parameter eps_s / (ls rs_p)
+  float inv_ls;
+  float eps_ls[kEpsSlots];
 };
 
 struct VolDev {
@@ -202,7 +208,6 @@ struct OptParams {
   int32_t opt;
   int32_t adam_form;
   float beta1, beta2, eps, momentum, rms_decay, rms_eps;
-  float eps_s;  // eps / sqrt(1 - beta2), the epsilon of the scaled moments
 };
 
 // the parameter step dz of one slot; m1 / m2 are its moment estimates
@@ -229,11 +234,19 @@ GC_DEV float opt_step(const OptParams& o, const EpochScal& es, float g, float& m
 // with the optimiser known, on scaled moments m1 = m / (1 - beta1) and
 // m2 = v / (1 - beta2): m1 <- beta1 m1 + g and m2 <- beta2 m2 + g^2 drop the
 // (1 - beta) factors, and the step
-//   lr m / (sqrt(v) + eps) = [lr (1 - beta1) / sqrt(1 - beta2)] m1 / (sqrt(m2) + eps / sqrt(1 - beta2))
-// takes the bracket (es.lr_s) and the scaled epsilon (o.eps_s) from the host.
+//   lr m / (sqrt(v) + eps) = [lr (1 - beta1) / sqrt(1 - beta2)] m1 / (sqrt(m2) + eps / sqrt(1 - beta2)).
+// The blocked kernel also drops the positive factor rs_p = (ub_p - lb_p) / N
+// of the sigmoid's derivative, which is uniform over the launch: it runs on
+// g / rs_p, with moments m1 / rs_p and m2 / rs_p^2, and Adam's step is
+// invariant under that scaling once eps is divided by rs_p too.  With the
+// bracket ls and the scaled epsilon eps_s = eps / sqrt(1 - beta2) the
+// denominator is (sqrt(m2) + eps_s / rs_p) / ls = fma(sqrt(m2), 1 / ls,
+// eps_s / (ls rs_p)), whose constants come from the host per iteration.
 // The blocked kernel keeps its Adam state in this scaled form whenever it
-// runs this update (every launch of such a call does).
-GC_DEV float adam_eff_step(const OptParams& o, const EpochScal& es, float g, float& m1, float& m2) {
+// runs this update (every launch of such a call does).  adam_eff_den updates
+// the moments and returns the denominator; the step is -m1 / den (the
+// kernel takes the reciprocals of all slots together, rcp_batch).
+GC_DEV float adam_eff_den(const OptParams& o, float inv_ls, float eps_ls, float g, float& m1, float& m2) {
   // m1 <- beta1 m1 + g as a three-address v_fma_f32 (the compiler's
   // accumulator form would put the result in g's register and copy it
   // back to m1's every iteration)
@@ -241,7 +254,8 @@ GC_DEV float adam_eff_step(const OptParams& o, const EpochScal& es, float g, flo
   asm("v_fma_f32 %0, %1, %2, %3" : "=v"(m1n) : "s"(o.beta1), "v"(m1), "v"(g));
   m1 = m1n;
   m2 = fmaf(g, g, o.beta2 * m2);
-  return -es.lr_s * m1 * rcpf(sqrtv(m2) + o.eps_s);
+  // (sqrt(m2) + eps_s / rs_p) / ls: the step folded into the denominator
+  return fmaf(sqrtv(m2), inv_ls, eps_ls);
 }
 
 struct AskParams {
@@ -249,7 +263,7 @@ struct AskParams {
   int32_t n_vol;
   int32_t transform;
   int32_t need_norms;  // step / |z|^2 / |g|^2 sums wanted (step or gradient tolerance active)
-  int32_t pad;
+  int32_t adam_eff;    // Adam in the folded effective form (adam_eff_den); sigmoid transform
   OptParams o;
 };
 
@@ -365,8 +379,23 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
     for (int k = 0; k < V.n_epochs; ++k) {
       const int e = V.epoch0 + k;
       float th[NP], dths[NP];
+      if constexpr (TR == GC_TRANSFORM_SIGMOID) {
+        // transform_fwd_scaled's sigmoid with the reciprocals of all slots
+        // taken together
+        float den1[NP], sg[NP];
 #pragma unroll
-      for (int p = 0; p < NP; ++p) transform_fwd_scaled<TR>(z[p], lbp[p], ubp[p], rsp[p], inv_count, th[p], dths[p]);
+        for (int p = 0; p < NP; ++p) den1[p] = 1.f + __expf(-z[p]);
+        rcp_batch<NP>(den1, sg);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          th[p] = fmaf(ubp[p] - lbp[p], sg[p], lbp[p]);
+          dths[p] = fmaf(-sg[p], sg[p], sg[p]) * (adam_eff ? 1.f : rsp[p]);
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+          transform_fwd_scaled<TR>(z[p], lbp[p], ubp[p], rsp[p], inv_count, th[p], dths[p]);
+      }
       float lsum = 0.f;
       float gth[NP];
       // Fast pass without per-element checks; a non-finite model signal makes
@@ -455,7 +484,8 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
           den += z[p] * z[p];
-          gn += g[p] * g[p];
+          const float gt = adam_eff ? g[p] * rsp[p] : g[p];  // d loss / d z
+          gn += gt * gt;
         }
       }
       if (k == V.n_epochs - 1 && V.z_last != nullptr && valid) {
@@ -463,12 +493,24 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
         for (int p = 0; p < NP; ++p) V.z_last[int64_t(p) * nv + v] = z[p];
       }
       if (e < V.iteration) {  // update_parameters skips the last iteration (:677)
-        const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2, es_table[k].lr_s};
+        const EpochScal es{es_table[k].lr, es_table[k].c1, es_table[k].c2};
+        // 1 / ls in a VGPR, so that each parameter's fma reads one SGPR
+        // (its eps) instead of copying it to a VGPR
+        float inv_ls = 0.f;
+        float rden[NP];
+        if constexpr (adam_eff) {
+          asm("v_mov_b32 %0, %1" : "=v"(inv_ls) : "s"(es_table[k].inv_ls));
+          float dn[NP];
+#pragma unroll
+          for (int p = 0; p < NP; ++p)
+            dn[p] = adam_eff_den(ap.o, inv_ls, es_table[k].eps_ls[p < kEpsSlots ? p : 0], g[p], m1[p], m2[p]);
+          rcp_batch<NP>(dn, rden);
+        }
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
           float dz;
           if constexpr (adam_eff)
-            dz = adam_eff_step(ap.o, es, g[p], m1[p], m2[p]);
+            dz = -m1[p] * rden[p];
           else
             dz = opt_step(ap.o, es, g[p], m1[p], m2[p]);
           const float zn = z[p] + dz;
@@ -526,8 +568,9 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
     }
   };
   const bool with_norms = ap.need_norms != 0;
-  // (the scaled moments need beta2 < 1)
-  const bool adam_eff = ap.o.opt == GC_OPT_ADAM && ap.o.adam_form != GC_ADAM_KINGMA_BA && ap.o.beta2 < 1.f;
+  // (the host sets it for Adam's effective form with beta2 < 1, at most
+  // kEpsSlots parameters and ub > lb)
+  const bool adam_eff = NP <= kEpsSlots && ap.adam_eff != 0;
   using Lin = std::integral_constant<int, GC_TRANSFORM_LINEAR>;
   using Sig = std::integral_constant<int, GC_TRANSFORM_SIGMOID>;
   if (ap.transform == GC_TRANSFORM_LINEAR) {
@@ -946,27 +989,78 @@ static __global__ void k_neighbours(const int64_t* __restrict__ vox, int64_t nv,
 
 // ---- deterministic FP64 reduction of block partials ------------------------
 // in is [nb][nrows] (each block's row sums contiguous).  The blocks are split
-// into `gridDim.x` chunks of `clen`; CUDA block (c, t) sums rows
-// 64t..64t+63 over chunk c with 4 lanes per row (coalesced 256-byte reads)
-// and writes out[row * gridDim.x + c].  The host adds the chunk sums of a
+// into `gridDim.x` chunks of `clen`; block (c, t) sums rows
+// 64 RL t .. 64 RL (t+1) - 1 over chunk c, RL consecutive rows per lane (one
+// 4 RL-byte load) and four wavefronts per row set, and writes
+// out[row * gridDim.x + c].  Every row is added in the same order whatever
+// RL: wavefront q takes the partials b0 + q, b0 + q + 4, ... and the four
+// wavefront sums are added in order.  k_fold_chunks adds the chunk sums of a
 // row in order, so the result does not depend on timing.
-static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb,
-                                                                int32_t nrows, int64_t clen,
-                                                                double* __restrict__ out) {
+template <int RL>
+__global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* __restrict__ in, int64_t nb, int32_t nrows,
+                                                         int64_t clen, double* __restrict__ out) {
+  static_assert(RL == 1 || RL == 4, "one or four rows per lane");
   const int c = blockIdx.x;
-  const int row = blockIdx.y * 64 + (threadIdx.x & 63);
+  const int lane = threadIdx.x & 63;
+  const int row = (blockIdx.y * 64 + lane) * RL;  // first row of the lane
   const int sub = threadIdx.x >> 6;
   const int64_t b0 = int64_t(c) * clen, b1 = min(nb, b0 + clen);
-  double s = 0.0;
-  if (row < nrows)
-    for (int64_t b = b0 + sub; b < b1; b += 4) s += double(in[b * nrows + row]);
-  __shared__ double sh[4][64];
-  sh[sub][threadIdx.x & 63] = s;
+  double s[RL];
+#pragma unroll
+  for (int j = 0; j < RL; ++j) s[j] = 0.0;
+  if (row < nrows) {  // with RL = 4, nrows is a multiple of 4
+    for (int64_t b = b0 + sub; b < b1; b += 4) {
+      if constexpr (RL == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(in + b * nrows + row);
+        s[0] += double(q.x);
+        s[1] += double(q.y);
+        s[2] += double(q.z);
+        s[3] += double(q.w);
+      } else {
+        s[0] += double(in[b * nrows + row]);
+      }
+    }
+  }
+  __shared__ double sh[4][64 * RL];
+#pragma unroll
+  for (int j = 0; j < RL; ++j) sh[sub][lane * RL + j] = s[j];
   __syncthreads();
   if (sub == 0 && row < nrows) {
-    const int l = threadIdx.x;
-    out[int64_t(row) * gridDim.x + c] = ((sh[0][l] + sh[1][l]) + sh[2][l]) + sh[3][l];
+#pragma unroll
+    for (int j = 0; j < RL; ++j) {
+      const int l = lane * RL + j;
+      out[int64_t(row + j) * gridDim.x + c] = ((sh[0][l] + sh[1][l]) + sh[2][l]) + sh[3][l];
+    }
   }
+}
+
+// The chunk sums of rows [0, nrows) added in chunk order, written in the
+// per-iteration layout dst[k * kSlots + j] (row k * npv + j; every other
+// slot of the K kSlots outputs is 0).
+static __global__ __launch_bounds__(kBlock) void k_fold_chunks(const double* __restrict__ sums, int32_t chunks,
+                                                                int32_t nrows, int32_t npv, int32_t kslots,
+                                                                int32_t nout, double* __restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nout) return;
+  const int k = t / kslots, j = t - k * kslots;
+  const int r = k * npv + j;
+  double acc = 0.0;
+  if (j < npv && r < nrows) {
+    const double* src = sums + int64_t(r) * chunks;
+    for (int c = 0; c < chunks; ++c) acc += src[c];
+  }
+  dst[t] = acc;
+}
+
+// k_reduce_rows with four rows per lane where the row count allows
+static inline void launch_reduce_rows(const float* in, int64_t nb, int32_t nrows, int32_t chunks,
+                                       double* out, hipStream_t st) {
+  if (nrows % 4 == 0)
+    hipLaunchKernelGGL(k_reduce_rows<4>, dim3(chunks, (nrows / 4 + 63) / 64), dim3(kBlock), 0, st, in, nb, nrows,
+                       int64_t(kReduceChunk), out);
+  else
+    hipLaunchKernelGGL(k_reduce_rows<1>, dim3(chunks, (nrows + 63) / 64), dim3(kBlock), 0, st, in, nb, nrows,
+                       int64_t(kReduceChunk), out);
 }
 
 static __global__ __launch_bounds__(kBlock) void k_reduce_rows_d(const double* __restrict__ in, int64_t nb,

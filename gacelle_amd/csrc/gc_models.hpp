@@ -20,6 +20,22 @@
 namespace gc {
 
 // ---------------------------------------------------------------------------
+// exp(-TE_0 R2*) and exp(-dTE R2*) of equally spaced echo times (TE_0 = first
+// echo, dTE = f[8]).  When dTE = k TE_0 for k = 1 or 2 (i[7] = k, set on
+// upload with f[9] = -log2(e) TE_0) the spacing's factor is a power of the
+// first echo's, else it is a second exponential.  The branch is wave-uniform.
+GC_DEV void uniform_decays(const gc_model& md, float r2s, float& e0, float& q) {
+  constexpr float kNegLog2e = -1.4426950408889634f;
+  if (md.i[7] > 0) {
+    e0 = __builtin_amdgcn_exp2f(r2s * md.f[9]);
+    q = md.i[7] == 2 ? e0 * e0 : e0;
+  } else {
+    e0 = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.table[0]));
+    q = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.f[8]));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // s = m0 * exp(-te * r2s): R2star/gpuR2starMapping.m:427-434 (FWD :321-363)
 // theta = {M0, R2star}.  EXACT: the echo count equals MT (no guards).
 // With GC_FLAG_UNIFORM_TE the decays exp(-TE_k R2*) are formed as
@@ -39,11 +55,7 @@ struct R2sModel {
     constexpr float kNegLog2e = -1.4426950408889634f;
     const float kr = kNegLog2e * r2s;  // -log2(e) * R2*
     float e = 0.f, q = 0.f;
-    if (UNI) {
-      // -log2(e) TE_0 and -log2(e) dTE are uniform over the launch
-      e = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.table[0]));
-      q = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.f[8]));
-    }
+    if (UNI) uniform_decays(md, r2s, e, q);
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
       if (EXACT || t < nte) {
@@ -94,9 +106,8 @@ struct JointModel {
   GC_DEV static void decays(const gc_model& md, float r2s, float* ete) {
     const int nte = EXACT ? MT : md.i[0];
     if (md.flags & GC_FLAG_UNIFORM_TE) {
-      constexpr float kNegLog2e = -1.4426950408889634f;
-      float e = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.table[0]));
-      const float q = __builtin_amdgcn_exp2f(r2s * (kNegLog2e * md.f[8]));
+      float e, q;
+      uniform_decays(md, r2s, e, q);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         ete[t] = (EXACT || t < nte) ? e : 0.f;

@@ -153,9 +153,10 @@ GC_DEV void model_adjoint(const gc_model& md, const float* th, const float* ex, 
 // measurement r of a group, d s_r / d theta_p = base_r (alpha_p + beta_p tw_r)
 // with base_r >= 0 and group constants alpha, beta, so the model accumulates
 // sum_r dl_r base_r and sum_r dl_r base_r tw_r and applies alpha and beta
-// once per group.  eval_base_adj(md, th, ex, f, gth) calls
+// once per group.  eval_base_adj(md, th, vp, f, gth) calls
 // f(r, s_r, base_r) -> dl_r base_r (0 for a measurement without gradient)
-// for every used slot and adds d loss / d theta to gth.
+// for every used slot and adds d loss / d theta to gth; vp holds the
+// model's per-voxel constants (voxel_prep(md, ex, vp), kVoxPrep floats).
 template <class Model>
 struct has_base_adj {
   template <class M>
@@ -163,6 +164,17 @@ struct has_base_adj {
   template <class M>
   static constexpr bool f(...) { return false; }
   static constexpr bool value = f<Model>(nullptr);
+};
+
+// per-voxel constants of a grouped-adjoint model (Model::voxel_prep fills
+// kVoxPrep floats that eval_base_adj reads)
+template <class Model>
+struct vox_prep {
+  template <class M>
+  static constexpr int f(decltype(M::kVoxPrep)*) { return M::kVoxPrep; }
+  template <class M>
+  static constexpr int f(...) { return 1; }
+  static constexpr int value = f<Model>(nullptr);
 };
 
 // ---- askadam block kernel -------------------------------------------------
@@ -330,6 +342,9 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
   float ex[kMaxExtra];
 #pragma unroll
   for (int e = 0; e < kMaxExtra; ++e) ex[e] = (e < md.n_extra) ? V.ex[int64_t(e) * nv + vv] : 0.f;
+  // the grouped-adjoint models' per-voxel constants, formed once
+  float vp[has_base_adj<Model>::value ? vox_prep<Model>::value : 1];
+  if constexpr (has_base_adj<Model>::value) Model::voxel_prep(md, ex, vp);
 
   float z[NP], m1[NP], m2[NP], lbp[NP], ubp[NP];
 #pragma unroll
@@ -453,7 +468,7 @@ void k_askadam(const gc_model* __restrict__ mdp, const AskParams ap) {
           // the model accumulates its gradient from d loss / d s times the
           // measurement's base signal (>= 0): with the L1 fast-path
           // derivative copysign(w, r) that product is one bit operation
-          Model::eval_base_adj(md, th, ex, [&](int r, float s, float base) {
+          Model::eval_base_adj(md, th, vp, [&](int r, float s, float base) {
             float dl;
             bool fin;
             const float res = measure(r, s, dl, fin);

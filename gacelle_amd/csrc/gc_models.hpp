@@ -275,8 +275,8 @@ struct NexiModel {
   //   l1 = (r1 + r2 + d1 + d2)/2,  l2 = sqrt((r1 - r2 + d1 - d2)^2 + 4 r1 r2)/2
   //   (at least 1e-8),  Pp = (fa d1 + (1 - fa) d2 - (l1 - l2)) / (2 l2),
   // with d1 = b Da x^2, d2 = b De, r1 = Delta ra, r2 = r1 fa / (1 - fa).
-  // The x-independent parts are folded once per shell, so a node costs 13
-  // VALU operations besides sqrt, rcp and two exp2.
+  // The x-independent parts are folded once per shell, so a node costs 14
+  // VALU operations besides one rsq and two exp2.
   struct Shell {
     float dab;  // d1 / x^2 = b Da
     float cu;   // r1 - r2 - d2
@@ -308,10 +308,13 @@ struct NexiModel {
   GC_DEV static Node node(const Shell& K, float x2) {
     Node n;
     n.u = fmaf(K.dab, x2, K.cu);
-    const float sq = sqrtv(fmaf(n.u, n.u, K.c4));
-    n.clamp = sq < 2e-8f;                 // l2 < 1e-8
-    n.sqh = fmaxf(sq, 2e-8f);             // 2 l2
-    n.rq = rcpf(n.sqh);                   // 1 / (2 l2)
+    // 2 l2 = sqrt(v) and its reciprocal from one v_rsq_f32: sqrt(v) = v rsq(v)
+    const float v = fmaf(n.u, n.u, K.c4);
+    const float t = __builtin_amdgcn_rsqf(v);
+    const float sq = v * t;
+    n.clamp = !(sq >= 2e-8f);             // l2 < 1e-8 (v = 0 gives 0 * inf)
+    n.sqh = n.clamp ? 2e-8f : sq;         // 2 l2
+    n.rq = n.clamp ? (1.f / 2e-8f) : t;   // 1 / (2 l2)
     n.Pp = fmaf(0.5f, n.sqh, fmaf(K.fam, x2, K.cn)) * n.rq;
     const float as = fmaf(K.asd, x2, K.as0);  // -log2(e) l1
     n.E1 = __builtin_amdgcn_exp2f(fmaf(-kHalfLog2e, n.sqh, as));

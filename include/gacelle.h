@@ -132,6 +132,11 @@ enum { GC_QUAD_TRAPZ = 0, GC_QUAD_TABLE = 1 };
  * n_par may exceed the model's own parameter count: trailing slots are
  * carried (and optimised with zero gradient) exactly like a modelParams
  * entry that the forward model never reads.
+ *
+ * i[7] and f[9] of R2STAR and JOINT belong to the library: with
+ * GC_FLAG_UNIFORM_TE and a spacing of one or two first-echo times it
+ * writes there (in its device copy) the multiple and -log2(e) TE_0, so that
+ * one exponential gives both decay factors; caller values are replaced.
  */
 typedef struct gc_model {
   int32_t model_id;

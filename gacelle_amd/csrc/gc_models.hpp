@@ -90,7 +90,8 @@ struct R2sModel {
 template <int MT, int MF, bool EXACT = false>
 struct JointModel {
   // blocked askadam kernel within 128 VGPRs (some bytes per lane spilled);
-  // three waves without spill measured 44.0 against 42.5 ms on 2.9M voxels
+  // three waves without spill measured 44.0 and five waves (more spill)
+  // 49.3 against 42.5 ms on 2.9M voxels
   static constexpr int kAskWaves = 4;
   static constexpr int NP = 3, MR = MT * MF, NE = 1;
   static constexpr bool kRelJ = false;
